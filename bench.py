@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""ScatCov forward + VJP evaluations per second on one MI355X.
+
+Workload: one HEALPix NESTED map at nside 1024, J = 9 scales, L = 4
+orientations, kernel 5, the complete plan (11354 coefficients).  A step is one
+forward sweep plus one VJP of the synthesis loss sum |S(x) - S(t)|^2 (its
+cotangent 2 (S(x) - S(t)) computed in the step), with the maps resident in HBM.
+The map x and the target t follow the lognormal recipe: g ~ N(0, 1) at nside
+32, copied to the NESTED children, plus N(0, 0.1^2) per pixel, x = exp(g / 2)
+- mean (seeds 1 and 2).
+
+The headline value is the mixed-precision engine (FP32 bulk fields and
+operators, FP64 sky-wide sums); the plain FP64 engine runs the same steps for
+the ratio, and its statistics are the reference for the mixed path's error.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "stl-dev-mpi_amd"))
+
+METRIC = "fwd+VJP evals/s, nside 1024 L=4: mixed vs FP64 GPU at ≤1e-6 rel err; HBM GB/s"
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
+FAMILIES = ["mean", "var", "S1", "S2", "S3", "S4"]
+
+
+def lognormal_map(nside, seed, base=32, sigma=0.1):
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal(12 * base * base)
+    g = np.repeat(g, (nside // base) ** 2) + sigma * rng.standard_normal(12 * nside * nside)
+    x = np.exp(0.5 * g)
+    return x - x.mean()
+
+
+def family_errors(values, ref, plan):
+    fam = np.array([FAMILIES.index(c["family"]) for c in plan])
+    out = {}
+    for f, name in enumerate(FAMILIES):
+        sel = fam == f
+        scale = np.abs(ref[..., sel]).max()
+        if name == "mean":
+            scale = np.sqrt(np.abs(ref[..., fam == 1]).max())
+        out[name] = float(np.abs(values[..., sel] - ref[..., sel]).max() / max(scale, 1e-300))
+    return out
+
+
+def run_engine(precision, nside, J, x, t, steps, warmup, timing):
+    import torch
+    from stl_port import NativeEngine
+
+    eng = NativeEngine(nside, J, batch=1, backend="gpu", precision=precision, timing=timing)
+    xd = torch.from_numpy(x).to("cuda:0").reshape(1, -1)
+    td = torch.from_numpy(t).to("cuda:0").reshape(1, -1)
+    target, _ = eng.forward_device(td, generation=0, objective="target")
+    grad = torch.empty_like(xd)
+    values = None
+
+    def step(gen):
+        nonlocal values
+        values, _ = eng.forward_device(xd, generation=gen)
+        cot = 2.0 * (values - target)
+        eng.vjp_device(cot, grad, generation=gen)
+
+    for i in range(warmup):
+        step(i + 1)
+    torch.cuda.synchronize()
+    info0 = eng.info() if timing else None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(warmup + i + 1)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    info1 = eng.info()
+    timers = {}
+    if timing:
+        for name, v in info1["timers"].items():
+            v0 = info0["timers"].get(name, {"ms": 0.0, "launches": 0, "bytes": 0.0})
+            timers[name] = {k: v[k] - v0[k] for k in ("ms", "launches", "bytes")}
+    out = dict(elapsed=elapsed, values=values.copy(), grad=grad.cpu().numpy().ravel(), plan=eng.plan,
+               setup_s=info1["setup_seconds"], device_bytes=info1["device_memory"]["total_bytes"],
+               peak_bytes=info1["device_memory"]["peak_bytes"], timers=timers)
+    eng.close()
+    del xd, td, grad
+    torch.cuda.empty_cache()
+    return out
+
+
+def roofline(timers, steps):
+    """Dominant kernel (family at one level) by total time: algorithmic bytes
+    per launch over its mean launch duration (HIP events on the engine stream)."""
+    name, t = max(timers.items(), key=lambda kv: kv[1]["ms"])
+    per_launch_ms = t["ms"] / t["launches"]
+    bytes_per_launch = t["bytes"] / t["launches"]
+    achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+    traffic = None
+    prof = ROOT / "profiles" / "traffic.json"
+    if prof.is_file():
+        measured = json.loads(prof.read_text())
+        traffic = measured.get(name.split("@")[0], {}).get("bytes_per_launch")
+    total_bytes = sum(v["bytes"] for v in timers.values()) / steps
+    total_ms = sum(v["ms"] for v in timers.values()) / steps
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+            "bytes_per_launch": bytes_per_launch, "ms_per_launch": per_launch_ms,
+            "eval_algorithmic_GB": round(total_bytes / 1e9, 3), "eval_kernel_ms": round(total_ms, 3),
+            "eval_GBps": round(total_bytes / (total_ms * 1e-3) / 1e9, 1)}
+
+
+def cpu_baseline(seconds_budget=30.0):
+    """The FP64 oracle (C++ restatement of the reference CPU engine, geometry
+    rebuilt at every application like its per-block rebuild) on the host cores,
+    one forward + VJP at nside 128 (J = 6, same recipe), scaled to nside 1024
+    by the pixel ratio 64."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    nside, J = 128, 6
+    x, t = lognormal_map(nside, 1), lognormal_map(nside, 2)
+    e = oracle.Engine(nside, J, cache_rows=0)
+    e.load(t)
+    target = e.forward()
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        e.load(x)
+        v = e.forward()
+        e.gradient(2.0 * (v - target))
+        n += 1
+        if time.perf_counter() - t0 > seconds_budget / 3 or n >= 3:
+            break
+    s_per_eval = (time.perf_counter() - t0) / n
+    scale = (1024 // nside) ** 2
+    return {"value": round(1.0 / (s_per_eval * scale), 6), "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"{n} forward+VJP of the FP64 oracle at nside {nside}, J={J} ({s_per_eval:.2f} s each), "
+                      f"scaled to nside 1024 by the pixel ratio {scale}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nside", type=int, default=1024)
+    ap.add_argument("--J", type=int, default=9)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the whole sky fits one "
+              f"MI355X, and the multi-GPU sky split is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+
+    x, t = lognormal_map(args.nside, 1), lognormal_map(args.nside, 2)
+    mixed = run_engine("mixed", args.nside, args.J, x, t, args.steps, args.warmup, timing=True)
+    print(f"[bench] mixed: {args.steps / mixed['elapsed']:.2f} evals/s", file=sys.stderr, flush=True)
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = run_engine("fp64", args.nside, args.J, x, t, args.steps, args.warmup, timing=True)
+        print(f"[bench] fp64: {args.steps / fp64['elapsed']:.2f} evals/s", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    value = args.steps / mixed["elapsed"]
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * mixed["elapsed"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 bulk, f64 sums",
+        "data": "synthetic: lognormal HEALPix NESTED maps (x seed 1, target seed 2)",
+        "config": {"workload": f"ScatCov forward+VJP of sum|S(x)-S(t)|^2, HEALPix NESTED nside {args.nside}, "
+                               f"J={args.J}, L=4, kernel 5, 1 map, {len(mixed['plan'])} coefficients",
+                   "nside": args.nside, "J": args.J, "L": 4, "batch": 1, "precision": "mixed"},
+        "roofline": roofline(mixed["timers"], args.steps),
+        "cpu_baseline": cpu,
+        "setup_s": round(mixed["setup_s"], 3),
+        "device_bytes": mixed["device_bytes"],
+    }
+    if fp64 is not None:
+        v64 = args.steps / fp64["elapsed"]
+        line["fp64"] = {"value": round(v64, 3), "ms_per_step": round(1e3 * fp64["elapsed"] / args.steps, 3),
+                        "roofline": roofline(fp64["timers"], args.steps), "setup_s": round(fp64["setup_s"], 3),
+                        "device_bytes": fp64["device_bytes"]}
+        line["mixed_over_fp64"] = round(value / v64, 3)
+        line["mixed_error_vs_fp64"] = family_errors(mixed["values"], fp64["values"], mixed["plan"])
+        g64 = fp64["grad"]
+        line["mixed_error_vs_fp64"]["gradient"] = float(np.abs(mixed["grad"] - g64).max() / np.abs(g64).max())
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,153 @@
+/* C ABI of libstl_bridge.so, the MI355X ScatCov engine.
+ *
+ * Drop-in for the reference's stl-bridge C ABI (crates/stl-bridge/src/lib.rs)
+ * as bound by python/stl_ensemble/stl_port/native.py:55-116: same symbol
+ * names, argument meanings, shapes and failure values.  Maps are float64
+ * NESTED [batch, 12 nside^2]; values and cotangents are interleaved (re, im)
+ * float64 pairs [batch, ncoef]; the gradient is that of
+ * Re sum(conj(cotangent) * values).  Every forward state carries an identity
+ * (handle, generation, member, objective); stlp_vjp and stlp_restore refuse a
+ * stale or foreign one.  Failures return -1 or NULL and leave a message for
+ * stlp_last_error.  One caller thread per handle.
+ *
+ * Scope of this library: the GPU backend of the complete one-channel plan
+ * (mean, var, S1..S4; kernel 5, 4 orientations, gauge cosmo, fewer = false,
+ * dg(j) = j).  The other products the reference's bridge also exports
+ * (patches, spectra, U-Net, training, masks, spin 2, cross statistics) are
+ * present as symbols that fail with "not implemented".
+ */
+#ifndef STL_BRIDGE_H
+#define STL_BRIDGE_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct StlHandle StlHandle;
+typedef struct StlSnapshot StlSnapshot;
+
+/* Message of the last failure on the calling thread (lib.rs:257-263). */
+const char* stlp_last_error(void);
+/* Frees a string returned by stlp_info (lib.rs:265-271). */
+void stlp_string_free(char* s);
+
+/* New handle from a JSON configuration (lib.rs:273-351): nside (128), j (7),
+ * batch (1), kernel (5), orientations (4), block_pixels (accepted, unused:
+ * the whole sky is resident), backend ("gpu"; "cpu" is refused), device (0),
+ * and the keys of this library: precision ("fp64" | "mixed": FP32 bulk
+ * fields and operators with FP64 sky-wide sums), timing (false).  NULL on
+ * failure. */
+StlHandle* stlp_new(const char* config);
+/* Frees a handle (lib.rs:353-359); NULL is ignored. */
+void stlp_free(StlHandle* h);
+/* JSON description: version, sizes, plan, dtype, device memory, timers
+ * (lib.rs:361-391).  Free with stlp_string_free; NULL on failure. */
+char* stlp_info(StlHandle* h);
+
+/* Forward sweep of x (n = batch * 12 nside^2 float64, host memory): values
+ * to out (2 * batch * ncoef), with moments the spatial variances of the
+ * terms to var (batch * ncoef) (lib.rs:474-530).  0 or -1. */
+int stlp_forward(StlHandle* h, const double* x, size_t n, uint64_t generation, uint64_t member, uint64_t objective,
+                 bool moments, double* out, double* var);
+/* Map cotangent out (batch * 12 nside^2, host) of the cotangents cot
+ * (2 n, n = batch * ncoef) after the forward of the same identity
+ * (lib.rs:532-581).  0 or -1. */
+int stlp_vjp(StlHandle* h, const double* cot, size_t n, uint64_t generation, uint64_t member, uint64_t objective,
+             double* out);
+
+/* As stlp_forward with x a device address (float64, on the handle's device),
+ * the additive device-pointer form the reference gives its patch handles
+ * (stlp_patch_forward_device, lib.rs:1152-1176).  Values stay host arrays. */
+int stlp_forward_device(StlHandle* h, uint64_t x_dev, size_t n, uint64_t generation, uint64_t member,
+                        uint64_t objective, bool moments, double* out, double* var);
+/* As stlp_vjp with the map cotangent written to a device address (float64)
+ * (cf. stlp_patch_vjp_device, lib.rs:1152-1176). */
+int stlp_vjp_device(StlHandle* h, const double* cot, size_t n, uint64_t generation, uint64_t member,
+                    uint64_t objective, uint64_t out_dev);
+
+/* Pixels each coefficient averages (ncoef size_t) (lib.rs:456-472). */
+int stlp_native_pixels(StlHandle* h, size_t* out);
+/* Forward on maps given at every level (concatenated [batch, 12 (nside/2^d)^2]
+ * over levels d), no decimation; values as stlp_forward; no vjp may follow
+ * (lib.rs:583-623). */
+int stlp_forward_levels(StlHandle* h, const double* x, size_t n, double* out);
+
+/* Host copy of the current forward state and its restore (lib.rs:645-709). */
+StlSnapshot* stlp_checkpoint(StlHandle* h);
+void stlp_checkpoint_free(StlSnapshot* s);
+size_t stlp_checkpoint_bytes(const StlSnapshot* s);
+int stlp_restore(StlHandle* h, const StlSnapshot* s, uint64_t generation, uint64_t member, uint64_t objective);
+
+/* loss = callback(x, n, grad, context) fills grad at x (lib.rs:711-712). */
+typedef double (*stlp_callback)(const double* x, size_t n, double* grad, void* context);
+/* L-BFGS with strong Wolfe line search of the callback from x (n, updated in
+ * place); JSON options lr (1), max_iter (20), max_eval (max_iter * 5 / 4),
+ * tolerance_grad (1e-12), tolerance_change (1e-15), history_size (10).
+ * Returns the number of evaluations or -1 (lib.rs:714-741). */
+int64_t stlp_lbfgs(double* x, size_t n, const char* options, stlp_callback callback, void* context);
+
+/* Masks: k = 0 restores whole-sky averages; k > 0 is not implemented here
+ * (lib.rs:393-422). */
+int stlp_set_masks(StlHandle* h, const double* weights, size_t k);
+/* Spin-2 front end: not implemented here (lib.rs:625-643). */
+int stlp_spin_fields(StlHandle* h, const double* x, size_t n, size_t level, double* qu, double* chi);
+
+/* Symbols of the reference's other handles (patches, spectra, U-Net,
+ * training; lib.rs:745-1492), bound eagerly by native.py: they fail with
+ * "not implemented" (NULL / -1). */
+void* stlp_patch_new(const char* config);
+void stlp_patch_free(void* h);
+char* stlp_patch_info(void* h);
+int stlp_pixel_vectors(size_t nside, const int64_t* ids, size_t n, double* out);
+int stlp_patch_select(void* h, const double* x, size_t n, double a, double b, size_t* out);
+int stlp_patch_ids(void* h, size_t k, int64_t* out);
+int stlp_patch_core_pixels(void* h, size_t* out);
+int stlp_patch_forward(void* h, const double* x, size_t n, uint64_t g, bool moments, double* out, double* var);
+int stlp_patch_vjp(void* h, const double* cot, size_t n, uint64_t g, double* out);
+int stlp_patch_forward_device(void* h, uint64_t x, size_t n, uint64_t g, bool moments, double* out, double* var);
+int stlp_patch_vjp_device(void* h, const double* cot, size_t n, uint64_t g, uint64_t out);
+void* stlp_spec_new(const char* config);
+void stlp_spec_free(void* h);
+char* stlp_spec_info(void* h);
+int stlp_spec_set_masks(void* h, const double* w, size_t k);
+int stlp_spec_forward(void* h, const double* x, size_t n, uint64_t g, uint64_t m, uint64_t o, double* out);
+int stlp_spec_vjp(void* h, const double* cot, size_t n, uint64_t g, uint64_t m, uint64_t o, double* out);
+void* stlp_unet_new(const char* config);
+void stlp_unet_free(void* h);
+char* stlp_unet_info(void* h);
+int stlp_unet_get_params(void* h, float* out);
+int stlp_unet_set_params(void* h, const float* p, size_t n);
+int stlp_unet_get_grads(void* h, float* out);
+int stlp_unet_zero_grad(void* h);
+int stlp_unet_adam(void* h, double lr);
+int stlp_unet_set_cond(void* h, const float* c, size_t n);
+int stlp_unet_apply(void* h, const double* x, size_t n, double* a, double* b, double* c);
+int stlp_train_set_masks(void* h, const double* w, size_t k);
+int stlp_train_set_common(void* h, const double* v);
+int stlp_train_set_field(void* h, uint32_t f, const double* a, const double* b, const double* c, const double* d,
+                         const double* e, const double* g, size_t n, double w);
+int stlp_train_bias(void* h, uint32_t f, const uint64_t* a, const double* b, size_t n);
+int stlp_train_loss(void* h, uint32_t f, bool b, double* out);
+int stlp_train_step(void* h, uint32_t f, double lr, double* out);
+int stlp_train_live(void* h, uint32_t f, const uint64_t* a, const double* b, size_t n, const uint64_t* c,
+                    const double* d, size_t m, bool e, double w, double* out);
+void* stlp_train_lbfgs(void* h, const uint32_t* f, size_t n, const char* options);
+int stlp_train_set_joint(void* h, const double* a, const double* b, uint32_t c, uint32_t d, const char* e, double w);
+int stlp_train_front(void* h, const double* x, size_t n, bool b, double* out);
+
+/* Test hooks of this library (no counterpart in the reference): one level's
+ * operators of a handle applied to device arrays, for operator-level parity
+ * tests.  kind: 0 conv forward (x [rows][N] -> y [rows][4][N] complex),
+ * 1 conv adjoint (gy -> gx), 2 decimation forward level -> level + 1,
+ * 3 decimation adjoint.  Arrays are float64 on the device. */
+int stlx_apply_operator(StlHandle* h, int kind, int level, int rows, uint64_t in_dev, uint64_t out_dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STL_BRIDGE_H */

@@ -1,0 +1,460 @@
+// C ABI (include/stl_bridge.h): the reference's stlp_* surface
+// (crates/stl-bridge/src/lib.rs:257-741) over the MI355X engine.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cctype>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <optional>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/stl_bridge.h"
+#include "device_buffer.hpp"
+#include "engine.hpp"
+
+namespace {
+
+// Version of this C ABI and its JSON (stlp_info).
+const char* kVersion = "mi355x-2026-12-r1";
+
+thread_local std::string g_error;
+std::atomic<uint64_t> g_next_id{1};
+
+template <typename R, typename F>
+R guard(R fail, F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+  } catch (...) {
+    g_error = "native failure";
+  }
+  return fail;
+}
+
+[[noreturn]] void fail(const std::string& msg) { throw std::runtime_error(msg); }
+
+// Flat JSON objects: string, number, bool and null values.
+struct Json {
+  std::map<std::string, std::string> kv;  // raw scalar text (strings unquoted)
+  std::map<std::string, bool> is_string;
+
+  static Json parse(const char* text) {
+    if (!text) fail("null JSON input");
+    Json j;
+    const char* p = text;
+    auto ws = [&] { while (*p && std::isspace(static_cast<unsigned char>(*p))) ++p; };
+    auto str = [&]() -> std::string {
+      if (*p != '"') fail("valid JSON: expected a string");
+      ++p;
+      std::string s;
+      while (*p && *p != '"') {
+        if (*p == '\\' && p[1]) { s += p[1]; p += 2; }
+        else s += *p++;
+      }
+      if (*p != '"') fail("valid JSON: unterminated string");
+      ++p;
+      return s;
+    };
+    ws();
+    if (*p != '{') fail("valid JSON: expected an object");
+    ++p;
+    ws();
+    if (*p == '}') return j;
+    while (true) {
+      ws();
+      std::string key = str();
+      ws();
+      if (*p != ':') fail("valid JSON: expected ':'");
+      ++p;
+      ws();
+      if (*p == '"') {
+        j.kv[key] = str();
+        j.is_string[key] = true;
+      } else {
+        const char* s = p;
+        while (*p && *p != ',' && *p != '}' && !std::isspace(static_cast<unsigned char>(*p))) ++p;
+        j.kv[key] = std::string(s, p);
+        j.is_string[key] = false;
+      }
+      ws();
+      if (*p == ',') { ++p; continue; }
+      if (*p == '}') break;
+      fail("valid JSON: expected ',' or '}'");
+    }
+    return j;
+  }
+
+  bool has(const std::string& k) const { return kv.count(k) && kv.at(k) != "null"; }
+  long integer(const std::string& k, long def) const {
+    if (!has(k)) return def;
+    char* end = nullptr;
+    long v = std::strtol(kv.at(k).c_str(), &end, 10);
+    if (*end) fail(k + " must be an integer");
+    return v;
+  }
+  double real(const std::string& k, double def) const {
+    if (!has(k)) return def;
+    char* end = nullptr;
+    double v = std::strtod(kv.at(k).c_str(), &end);
+    if (*end) fail(k + " must be a number");
+    return v;
+  }
+  bool boolean(const std::string& k, bool def) const {
+    if (!has(k)) return def;
+    if (kv.at(k) == "true") return true;
+    if (kv.at(k) == "false") return false;
+    fail(k + " must be a boolean");
+  }
+  std::string string(const std::string& k, const std::string& def) const {
+    if (!has(k)) return def;
+    if (!is_string.at(k)) fail(k + " must be a string");
+    return kv.at(k);
+  }
+};
+
+struct Identity {
+  uint64_t owner, generation, member, objective;
+  bool operator==(const Identity& o) const {
+    return owner == o.owner && generation == o.generation && member == o.member && objective == o.objective;
+  }
+};
+
+const char* family_name(int f) {
+  static const char* names[] = {"mean", "var", "S1", "S2", "S3", "S4"};
+  return names[f];
+}
+
+std::string json_opt(int v) { return v < 0 ? "null" : std::to_string(v); }
+
+}  // namespace
+
+struct StlHandle {
+  std::unique_ptr<stl::Engine> engine;
+  uint64_t id = 0;
+  int nside = 0, batch = 1, J = 0;
+  size_t npix = 0;
+  std::optional<Identity> current;
+  stl::DevArray<double> xbuf, gbuf;
+  std::string precision;
+  size_t ncoef() const { return engine->plan().size(); }
+  size_t input_len() const { return size_t(batch) * npix; }
+  Identity identity(uint64_t g, uint64_t m, uint64_t o) const { return Identity{id, g, m, o}; }
+};
+
+struct StlSnapshot {
+  Identity identity;
+  std::vector<char> state;
+};
+
+extern "C" {
+
+const char* stlp_last_error(void) { return g_error.c_str(); }
+
+void stlp_string_free(char* s) { std::free(s); }
+
+StlHandle* stlp_new(const char* config) {
+  return guard<StlHandle*>(nullptr, [&]() -> StlHandle* {
+    Json v = Json::parse(config);
+    long nside = v.integer("nside", 128);
+    if (nside <= 0 || (nside & (nside - 1))) fail("nside must be a power of two");
+    long order = 0;
+    while ((1L << order) < nside) ++order;
+    long j = v.integer("j", 7);
+    if (j <= 0 || j > order + 1) fail("invalid number of scales");
+    long batch = v.integer("batch", 1);
+    if (batch <= 0) fail("batch must be positive");
+    if (v.integer("kernel", 5) != 5) fail("kernel: only 5 is implemented");
+    if (v.integer("orientations", 4) != 4) fail("orientations: only 4 is implemented");
+    if (v.integer("block_pixels", 4096) <= 0) fail("block_pixels must be positive");
+    std::string backend = v.string("backend", "gpu");
+    if (backend == "cpu") fail("backend cpu is not provided by this library (GPU engine only)");
+    if (backend != "gpu") fail("backend must be cpu or gpu, not \"" + backend + "\"");
+    if (v.boolean("cross", false)) fail("cross statistics are not implemented");
+    if (v.integer("spin", 0) != 0) fail("spin 2 is not implemented");
+    std::string precision = v.string("precision", "fp64");
+    if (precision != "fp64" && precision != "mixed") fail("precision must be fp64 or mixed");
+    stl::EngineOptions opt;
+    opt.nside = int(nside);
+    opt.J = int(j);
+    opt.batch = int(batch);
+    opt.fp64 = precision == "fp64";
+    opt.device = int(v.integer("device", 0));
+    opt.timing = v.boolean("timing", false);
+    auto h = std::make_unique<StlHandle>();
+    h->engine = stl::Engine::create(opt);
+    h->id = g_next_id.fetch_add(1);
+    h->nside = int(nside);
+    h->batch = int(batch);
+    h->J = int(j);
+    h->npix = size_t(12) * nside * nside;
+    h->precision = precision;
+    return h.release();
+  });
+}
+
+void stlp_free(StlHandle* h) { delete h; }
+
+char* stlp_info(StlHandle* h) {
+  return guard<char*>(nullptr, [&]() -> char* {
+    if (!h) fail("null handle");
+    std::ostringstream o;
+    o.precision(17);
+    o << "{\"version\": \"" << kVersion << "\", \"nside\": " << h->nside << ", \"j\": " << h->J
+      << ", \"batch\": " << h->batch << ", \"channels\": 1, \"cross\": false, \"fewer\": false, \"backend\": \"gpu\""
+      << ", \"precision\": \"" << h->precision << "\", \"dtype\": \""
+      << (h->precision == "fp64" ? "float64/complex128" : "float32 fields and operators, float64 sums") << "\""
+      << ", \"masks\": 0, \"spin\": 0, \"conv\": \"folded gather, 4-fold face symmetry\", \"fields\": \"keep\"";
+    o << ", \"device_memory\": {\"total_bytes\": " << h->engine->device_bytes()
+      << ", \"peak_bytes\": " << stl::device_bytes_peak() << "}";
+    o << ", \"setup_seconds\": " << h->engine->setup_seconds();
+    o << ", \"timers\": {";
+    bool first = true;
+    for (auto& t : h->engine->timers()) {
+      o << (first ? "" : ", ") << "\"" << t.name << "\": {\"ms\": " << t.ms << ", \"launches\": " << t.launches
+        << ", \"bytes\": " << t.bytes << "}";
+      first = false;
+    }
+    o << "}, \"plan\": [";
+    const auto& plan = h->engine->plan();
+    for (size_t i = 0; i < plan.size(); ++i) {
+      const auto& c = plan[i];
+      o << (i ? ", " : "") << "{\"index\": " << i << ", \"family\": \"" << family_name(c.family)
+        << "\", \"native_dg\": " << c.native_dg << ", \"c1\": " << c.c1 << ", \"c2\": " << c.c2
+        << ", \"j1\": " << json_opt(c.j1) << ", \"j2\": " << json_opt(c.j2) << ", \"j3\": " << json_opt(c.j3)
+        << ", \"l1\": " << json_opt(c.l1) << ", \"l2\": " << json_opt(c.l2) << ", \"l3\": " << json_opt(c.l3)
+        << "}";
+    }
+    o << "]}";
+    std::string s = o.str();
+    char* out = static_cast<char*>(std::malloc(s.size() + 1));
+    std::memcpy(out, s.c_str(), s.size() + 1);
+    return out;
+  });
+}
+
+static void write_values(const std::vector<stl::cplx>& v, double* out) {
+  for (size_t i = 0; i < v.size(); ++i) {
+    out[2 * i] = v[i].real();
+    out[2 * i + 1] = v[i].imag();
+  }
+}
+
+static int forward_impl(StlHandle* h, const double* x_dev, size_t n, uint64_t g, uint64_t m, uint64_t o,
+                        bool moments, double* out, double* var) {
+  if (!h) fail("null handle");
+  if (n != h->input_len()) fail("input shape mismatch");
+  if (!out) fail("null output");
+  if (moments && !var) fail("null variance output");
+  h->current.reset();
+  std::vector<stl::cplx> values(size_t(h->batch) * h->ncoef());
+  std::vector<double> variance(moments ? values.size() : 0);
+  h->engine->forward_device(x_dev, moments, values.data(), moments ? variance.data() : nullptr);
+  write_values(values, out);
+  if (moments) std::memcpy(var, variance.data(), variance.size() * sizeof(double));
+  h->current = h->identity(g, m, o);
+  return 0;
+}
+
+int stlp_forward(StlHandle* h, const double* x, size_t n, uint64_t g, uint64_t m, uint64_t o, bool moments,
+                 double* out, double* var) {
+  return guard(-1, [&] {
+    if (!h) fail("null handle");
+    if (n != h->input_len()) fail("input shape mismatch");
+    if (!x) fail("null input");
+    h->xbuf.resize(n);
+    STL_HIP(hipMemcpyAsync(h->xbuf.data(), x, n * sizeof(double), hipMemcpyHostToDevice, h->engine->stream()));
+    return forward_impl(h, h->xbuf.data(), n, g, m, o, moments, out, var);
+  });
+}
+
+int stlp_forward_device(StlHandle* h, uint64_t x_dev, size_t n, uint64_t g, uint64_t m, uint64_t o, bool moments,
+                        double* out, double* var) {
+  return guard(-1, [&] {
+    if (!x_dev) fail("null device input");
+    return forward_impl(h, reinterpret_cast<const double*>(x_dev), n, g, m, o, moments, out, var);
+  });
+}
+
+static std::vector<stl::cplx> read_cot(StlHandle* h, const double* cot, size_t n, uint64_t g, uint64_t m,
+                                       uint64_t o) {
+  if (!h) fail("null handle");
+  if (!h->current || !(*h->current == h->identity(g, m, o)))
+    fail("stale, wrong-member or wrong-objective forward state");
+  if (n != size_t(h->batch) * h->ncoef()) fail("cotangent shape mismatch");
+  if (!cot) fail("null cotangent");
+  std::vector<stl::cplx> c(n);
+  for (size_t i = 0; i < n; ++i) c[i] = stl::cplx(cot[2 * i], cot[2 * i + 1]);
+  return c;
+}
+
+int stlp_vjp(StlHandle* h, const double* cot, size_t n, uint64_t g, uint64_t m, uint64_t o, double* out) {
+  return guard(-1, [&] {
+    auto c = read_cot(h, cot, n, g, m, o);
+    if (!out) fail("null output");
+    h->gbuf.resize(h->input_len());
+    h->engine->vjp_device(c.data(), h->gbuf.data());
+    STL_HIP(hipMemcpy(out, h->gbuf.data(), h->gbuf.bytes(), hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+
+int stlp_vjp_device(StlHandle* h, const double* cot, size_t n, uint64_t g, uint64_t m, uint64_t o,
+                    uint64_t out_dev) {
+  return guard(-1, [&] {
+    auto c = read_cot(h, cot, n, g, m, o);
+    if (!out_dev) fail("null device output");
+    h->engine->vjp_device(c.data(), reinterpret_cast<double*>(out_dev));
+    return 0;
+  });
+}
+
+int stlp_native_pixels(StlHandle* h, size_t* out) {
+  return guard(-1, [&] {
+    if (!h || !out) fail("null argument");
+    const auto& plan = h->engine->plan();
+    for (size_t i = 0; i < plan.size(); ++i) out[i] = size_t(12) * (size_t(h->nside) >> plan[i].native_dg) *
+                                                     (size_t(h->nside) >> plan[i].native_dg);
+    return 0;
+  });
+}
+
+int stlp_forward_levels(StlHandle* h, const double* x, size_t n, double* out) {
+  return guard(-1, [&] {
+    if (!h || !x || !out) fail("null argument");
+    size_t total = 0;
+    for (int d = 0; d < h->engine->levels(); ++d)
+      total += size_t(h->batch) * 12 * (size_t(h->nside) >> d) * (size_t(h->nside) >> d);
+    if (n != total) fail("input shape mismatch");
+    h->current.reset();
+    stl::DevArray<double> buf(n);
+    STL_HIP(hipMemcpy(buf.data(), x, n * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<stl::cplx> values(size_t(h->batch) * h->ncoef());
+    h->engine->forward_levels_device(buf.data(), values.data());
+    write_values(values, out);
+    return 0;
+  });
+}
+
+StlSnapshot* stlp_checkpoint(StlHandle* h) {
+  return guard<StlSnapshot*>(nullptr, [&]() -> StlSnapshot* {
+    if (!h) fail("null handle");
+    if (!h->current) fail("checkpoint requires a forward state");
+    auto s = std::make_unique<StlSnapshot>();
+    s->identity = *h->current;
+    s->state = h->engine->save_state();
+    return s.release();
+  });
+}
+
+void stlp_checkpoint_free(StlSnapshot* s) { delete s; }
+
+size_t stlp_checkpoint_bytes(const StlSnapshot* s) { return s ? s->state.size() : 0; }
+
+int stlp_restore(StlHandle* h, const StlSnapshot* s, uint64_t g, uint64_t m, uint64_t o) {
+  return guard(-1, [&] {
+    if (!h || !s) fail("null argument");
+    if (!(s->identity == h->identity(g, m, o)))
+      fail("stale, wrong-member, wrong-objective or foreign checkpoint");
+    h->engine->load_state(s->state);
+    h->current = s->identity;
+    return 0;
+  });
+}
+
+int stlp_set_masks(StlHandle* h, const double* weights, size_t k) {
+  return guard(-1, [&] {
+    (void)weights;
+    if (!h) fail("null handle");
+    h->current.reset();
+    if (k != 0) fail("masks are not implemented by this library");
+    return 0;
+  });
+}
+
+int stlp_spin_fields(StlHandle*, const double*, size_t, size_t, double*, double*) {
+  g_error = "not implemented: spin 2";
+  return -1;
+}
+
+int stlx_apply_operator(StlHandle* h, int kind, int level, int rows, uint64_t in_dev, uint64_t out_dev) {
+  return guard(-1, [&] {
+    if (!h || !in_dev || !out_dev) fail("null argument");
+    h->engine->apply_operator(kind, level, rows, reinterpret_cast<const double*>(in_dev),
+                              reinterpret_cast<double*>(out_dev));
+    return 0;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// The reference's other handles: present, not implemented.
+// ---------------------------------------------------------------------------
+
+#define STL_NI_PTR(name, ...)               \
+  void* name(__VA_ARGS__) {                 \
+    g_error = "not implemented: " #name;    \
+    return nullptr;                         \
+  }
+#define STL_NI_INT(name, ...)               \
+  int name(__VA_ARGS__) {                   \
+    g_error = "not implemented: " #name;    \
+    return -1;                              \
+  }
+
+STL_NI_PTR(stlp_patch_new, const char*)
+void stlp_patch_free(void*) {}
+char* stlp_patch_info(void*) {
+  g_error = "not implemented: stlp_patch_info";
+  return nullptr;
+}
+STL_NI_INT(stlp_pixel_vectors, size_t, const int64_t*, size_t, double*)
+STL_NI_INT(stlp_patch_select, void*, const double*, size_t, double, double, size_t*)
+STL_NI_INT(stlp_patch_ids, void*, size_t, int64_t*)
+STL_NI_INT(stlp_patch_core_pixels, void*, size_t*)
+STL_NI_INT(stlp_patch_forward, void*, const double*, size_t, uint64_t, bool, double*, double*)
+STL_NI_INT(stlp_patch_vjp, void*, const double*, size_t, uint64_t, double*)
+STL_NI_INT(stlp_patch_forward_device, void*, uint64_t, size_t, uint64_t, bool, double*, double*)
+STL_NI_INT(stlp_patch_vjp_device, void*, const double*, size_t, uint64_t, uint64_t)
+STL_NI_PTR(stlp_spec_new, const char*)
+void stlp_spec_free(void*) {}
+char* stlp_spec_info(void*) {
+  g_error = "not implemented: stlp_spec_info";
+  return nullptr;
+}
+STL_NI_INT(stlp_spec_set_masks, void*, const double*, size_t)
+STL_NI_INT(stlp_spec_forward, void*, const double*, size_t, uint64_t, uint64_t, uint64_t, double*)
+STL_NI_INT(stlp_spec_vjp, void*, const double*, size_t, uint64_t, uint64_t, uint64_t, double*)
+STL_NI_PTR(stlp_unet_new, const char*)
+void stlp_unet_free(void*) {}
+char* stlp_unet_info(void*) {
+  g_error = "not implemented: stlp_unet_info";
+  return nullptr;
+}
+STL_NI_INT(stlp_unet_get_params, void*, float*)
+STL_NI_INT(stlp_unet_set_params, void*, const float*, size_t)
+STL_NI_INT(stlp_unet_get_grads, void*, float*)
+STL_NI_INT(stlp_unet_zero_grad, void*)
+STL_NI_INT(stlp_unet_adam, void*, double)
+STL_NI_INT(stlp_unet_set_cond, void*, const float*, size_t)
+STL_NI_INT(stlp_unet_apply, void*, const double*, size_t, double*, double*, double*)
+STL_NI_INT(stlp_train_set_masks, void*, const double*, size_t)
+STL_NI_INT(stlp_train_set_common, void*, const double*)
+STL_NI_INT(stlp_train_set_field, void*, uint32_t, const double*, const double*, const double*, const double*,
+           const double*, const double*, size_t, double)
+STL_NI_INT(stlp_train_bias, void*, uint32_t, const uint64_t*, const double*, size_t)
+STL_NI_INT(stlp_train_loss, void*, uint32_t, bool, double*)
+STL_NI_INT(stlp_train_step, void*, uint32_t, double, double*)
+STL_NI_INT(stlp_train_live, void*, uint32_t, const uint64_t*, const double*, size_t, const uint64_t*, const double*,
+           size_t, bool, double, double*)
+STL_NI_PTR(stlp_train_lbfgs, void*, const uint32_t*, size_t, const char*)
+STL_NI_INT(stlp_train_set_joint, void*, const double*, const double*, uint32_t, uint32_t, const char*, double)
+STL_NI_INT(stlp_train_front, void*, const double*, size_t, bool, double*)
+
+#undef STL_NI_PTR
+#undef STL_NI_INT
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// Shared definitions of the MI355X ScatCov library: error handling, the
+// face-rotation symmetry of the pyramid operators and the operator layouts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace stl {
+
+#define STL_HIP(call)                                                                                   \
+  do {                                                                                                  \
+    hipError_t err_ = (call);                                                                           \
+    if (err_ != hipSuccess)                                                                             \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(err_) + " at " + __FILE__ + \
+                               ":" + std::to_string(__LINE__) + " (" #call ")");                        \
+  } while (0)
+
+#define STL_CHECK(cond, msg)                                    \
+  do {                                                          \
+    if (!(cond)) throw std::runtime_error(std::string(msg));    \
+  } while (0)
+
+// Number of orientations / gauges handled by the kernels (L = 4).
+constexpr int kL = 4;
+// Complex outputs per conv row: kL gauges x (re, im).
+constexpr int kCoefPerSlot = 2 * kL;
+// Slot capacity of a conv row (distinct cells read with a nonzero weight:
+// at most 49 for kernel 5, L = 4, gauge cosmo, at every nside).
+constexpr int kMaxSlots = 64;
+
+// ---------------------------------------------------------------------------
+// Face-rotation symmetry.  HEALPix is invariant under the rotation by pi/2
+// about the polar axis, which maps face f to (f & ~3) | ((f + 1) & 3) with the
+// same in-face (x, y).  The stencil of gauge g at the rotated pixel is that of
+// gauge (g + 2) mod 4 at the original one, point-reflected when g + 2 wraps;
+// the Gabor kernel satisfies k(-x, -y) = conj(k(x, y)), so a point reflection
+// conjugates the folded coefficient.  For a shift of k faces:
+//   A[rot^k(p), rot^k(c)][g] = A[p, c][g ^ (2 (k & 1))], conjugated when
+//   k = 1 and g < 2, k = 2, or k = 3 and g >= 2.
+// Operators are therefore stored for the faces 0, 4 and 8 only ("rep" rows).
+// ---------------------------------------------------------------------------
+
+__host__ __device__ inline int rot_face(int f, int k) { return (f & ~3) | ((f + k) & 3); }
+
+__host__ __device__ inline int64_t rot_pix(int64_t pix, int k, int order) {
+  int64_t npf = int64_t(1) << (2 * order);
+  int f = int(pix >> (2 * order));
+  return int64_t(rot_face(f, k)) * npf + (pix & (npf - 1));
+}
+
+__host__ __device__ inline int gauge_src(int k, int g) { return g ^ (2 * (k & 1)); }
+
+__host__ __device__ inline bool gauge_conj(int k, int g) {
+  return (k == 1 && g < 2) || k == 2 || (k == 3 && g >= 2);
+}
+
+// Rep index of a pixel lying in a rep face (0, 4 or 8): (face / 4) * npf + local.
+__host__ __device__ inline int64_t rep_index(int64_t pix, int order) {
+  int64_t npf = int64_t(1) << (2 * order);
+  return (pix >> (2 * order)) / 4 * npf + (pix & (npf - 1));
+}
+
+// Pixel of a rep index: face 4 * (r / npf), local r % npf.
+__host__ __device__ inline int64_t rep_pixel(int64_t r, int order) {
+  int64_t npf = int64_t(1) << (2 * order);
+  return (r / npf) * 4 * npf + (r % npf);
+}
+
+// ---------------------------------------------------------------------------
+// Operator layouts (device memory, ELL by slot: entry (s, r) at s * R + r so
+// that consecutive rep rows read consecutive memory for one slot).
+// ---------------------------------------------------------------------------
+
+// Wavelet convolution of one level, complex outputs of kL gauges.
+template <typename T>
+struct ConvOp {
+  int order = 0;          // log2 nside of the level
+  int64_t npf = 0;        // pixels per face
+  int64_t R = 0;          // rep rows = 3 npf
+  int S = 0;              // slot capacity of the forward rows
+  int Tn = 0;             // slot capacity of the transposed rows
+  const uint8_t* cnt = nullptr;   // [R] slots per rep row
+  const int32_t* cell = nullptr;  // [S][R] NESTED cell read (rep row frame)
+  const T* coef = nullptr;        // [S][R][kL][2]
+  const uint8_t* tcnt = nullptr;  // [R] rows reading each rep cell
+  const int32_t* trow = nullptr;  // [Tn][R] NESTED row id (any face), ascending
+  const T* tcoef = nullptr;       // [Tn][R][kL][2] that row's coefficient (its own gauges)
+};
+
+// Decimation from level n (fine) to level n + 1 (coarse), real weights.
+template <typename T>
+struct DecOp {
+  int order_f = 0, order_c = 0;
+  int64_t npf_f = 0, npf_c = 0;
+  int64_t Rc = 0, Rf = 0;          // rep coarse rows, rep fine cells
+  int S = 0, Tn = 0;
+  const uint8_t* cnt = nullptr;    // [Rc]
+  const int32_t* col = nullptr;    // [S][Rc] fine cell (rep frame), ascending
+  const T* val = nullptr;          // [S][Rc]
+  const uint8_t* tcnt = nullptr;   // [Rf]
+  const int32_t* trow = nullptr;   // [Tn][Rf] coarse row id (any face), ascending
+  const T* tval = nullptr;         // [Tn][Rf]
+};
+
+}  // namespace stl

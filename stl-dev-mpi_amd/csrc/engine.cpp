@@ -1,0 +1,632 @@
+// Host orchestration of the level sweeps on one GPU (the reference's
+// LocalEngine::forward / reverse / map_cotangent, crates/stl-core/src/engine.rs
+// :500-779, with the whole sky resident instead of blocks and halos).
+//
+// Forward, per level n ascending:
+//   maps[n]  = D maps[n-1]                          (dec, rows B)
+//   z_n      = W maps[n], |z_n| -> modulus row n     (conv, rows B)
+//   modulus rows d < n = D modulus[n-1]              (dec, rows n B, inner kL)
+//   branch   = W modulus (all d <= n)                (conv, rows (n+1) B, inner kL)
+//   Gram sums of V = [branch(d)[l1][l3], z[l3]] and the S1/S2, mean/var sums.
+// Reverse, per level n descending (the cotangents of the Gram products are
+// the Hermitian matrices H = C + C^H of the coefficient cotangents / N):
+//   gV = H V -> branch cotangent, z part           (gv)
+//   chain[n] = W^T gbranch + D^T chain[n+1]          (conv adjoint, dec adjoint)
+//   gz = z part + S1/S2 terms + |.|' of chain row n  (gz)
+//   mg[n] = W^T gz + D^T mg[n+1]                      (conv adjoint, dec adjoint)
+// which is the reference's reverse with its Horner chain of D^T over levels.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+
+#include "geometry.hpp"
+#include "sweeps.hpp"
+
+namespace stl {
+
+size_t& device_bytes_live() {
+  static size_t v = 0;
+  return v;
+}
+size_t& device_bytes_peak() {
+  static size_t v = 0;
+  return v;
+}
+
+std::vector<Coefficient> build_plan(int J, int L) {
+  std::vector<Coefficient> plan;
+  plan.push_back(Coefficient{F_MEAN, 0, 0, 0});
+  plan.push_back(Coefficient{F_VAR, 0, 0, 0});
+  for (int j3 = 0; j3 < J; ++j3) {
+    int dg = j3;
+    for (int fam : {F_S1, F_S2})
+      for (int l = 0; l < L; ++l) {
+        Coefficient c{fam, dg, 0, 0};
+        c.j1 = j3;
+        c.l1 = l;
+        plan.push_back(c);
+      }
+    for (int j2 = 0; j2 <= j3; ++j2) {
+      for (int l2 = 0; l2 < L; ++l2)
+        for (int l3 = 0; l3 < L; ++l3) {
+          Coefficient c{F_S3, dg, 0, 0};
+          c.j1 = j2; c.j3 = j3; c.l1 = l2; c.l3 = l3;
+          plan.push_back(c);
+        }
+      for (int j1 = 0; j1 <= j2; ++j1)
+        for (int l1 = 0; l1 < L; ++l1)
+          for (int l2 = 0; l2 < L; ++l2)
+            for (int l3 = 0; l3 < L; ++l3) {
+              Coefficient c{F_S4, dg, 0, 0};
+              c.j1 = j1; c.j2 = j2; c.j3 = j3; c.l1 = l1; c.l2 = l2; c.l3 = l3;
+              plan.push_back(c);
+            }
+    }
+  }
+  std::stable_sort(plan.begin(), plan.end(),
+                   [](const Coefficient& a, const Coefficient& b) { return a.native_dg < b.native_dg; });
+  return plan;
+}
+
+namespace {
+
+inline int gram_k(int n) { return (n + 1) * kL + 1; }
+inline int gram_e(int n) { int K = gram_k(n); return K * (K + 1) / 2; }
+inline int tri_index(int K, int i, int j) { return i * K - i * (i - 1) / 2 + (j - i); }
+
+// Where a coefficient's sum lives.
+struct SumSlot {
+  enum Kind { MEANVAR, ZSTAT, GRAM } kind;
+  int n = 0;      // level
+  int l = 0;      // gauge (S1/S2: l1, Gram: l3)
+  int idx = 0;    // meanvar / zstat column, or Gram entry
+  bool conj = false;
+  int gi = -1, gj = -1;  // Gram operand indices (reverse)
+};
+
+SumSlot slot_of(const Coefficient& c) {
+  SumSlot s;
+  s.n = c.native_dg;
+  switch (c.family) {
+    case F_MEAN: s.kind = SumSlot::MEANVAR; s.idx = 0; break;
+    case F_VAR: s.kind = SumSlot::MEANVAR; s.idx = 1; break;
+    case F_S1: s.kind = SumSlot::ZSTAT; s.l = c.l1; s.idx = 0; break;
+    case F_S2: s.kind = SumSlot::ZSTAT; s.l = c.l1; s.idx = 1; break;
+    case F_S3:
+    case F_S4: {
+      s.kind = SumSlot::GRAM;
+      s.l = c.l3;
+      int K = gram_k(s.n);
+      int i = c.j1 * kL + c.l1;
+      int j = c.family == F_S3 ? K - 1 : c.j2 * kL + c.l2;
+      s.gi = i;
+      s.gj = j;
+      if (i <= j) { s.idx = tri_index(K, i, j); s.conj = false; }
+      else { s.idx = tri_index(K, j, i); s.conj = true; }
+      break;
+    }
+  }
+  return s;
+}
+
+// Timer name of a kernel family at a level.
+std::string tag(const char* family, int level) { return std::string(family) + "@L" + std::to_string(level); }
+
+// HIP-event timers by kernel family.
+class Profiler {
+ public:
+  void enable(bool on, hipStream_t s) { on_ = on; s_ = s; }
+  bool on() const { return on_; }
+  int start() {
+    if (!on_) return -1;
+    hipEvent_t a = get();
+    STL_HIP(hipEventRecord(a, s_));
+    pending_.push_back({"", a, nullptr, 0.0});
+    return int(pending_.size()) - 1;
+  }
+  void stop(int id, const std::string& name, double bytes) {
+    if (id < 0) return;
+    hipEvent_t b = get();
+    STL_HIP(hipEventRecord(b, s_));
+    pending_[id].name = name;
+    pending_[id].b = b;
+    pending_[id].bytes = bytes;
+  }
+  // After a stream synchronisation.
+  void collect() {
+    for (auto& p : pending_) {
+      float ms = 0.f;
+      STL_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+      auto& t = acc_[p.name];
+      t.name = p.name;
+      t.ms += ms;
+      t.launches += 1;
+      t.bytes += p.bytes;
+      pool_.push_back(p.a);
+      pool_.push_back(p.b);
+    }
+    pending_.clear();
+  }
+  std::vector<KernelTimer> timers() const {
+    std::vector<KernelTimer> v;
+    for (auto& kv : acc_) v.push_back(kv.second);
+    return v;
+  }
+  void reset() { acc_.clear(); }
+  ~Profiler() {
+    for (auto e : pool_) (void)hipEventDestroy(e);
+    for (auto& p : pending_) {
+      (void)hipEventDestroy(p.a);
+      if (p.b) (void)hipEventDestroy(p.b);
+    }
+  }
+
+ private:
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  hipEvent_t get() {
+    if (!pool_.empty()) {
+      hipEvent_t e = pool_.back();
+      pool_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    STL_HIP(hipEventCreate(&e));
+    return e;
+  }
+  bool on_ = false;
+  hipStream_t s_ = nullptr;
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> pool_;
+  std::map<std::string, KernelTimer> acc_;
+};
+
+template <typename T>
+class EngineT final : public Engine {
+  using C2 = cplx_t<T>;
+
+ public:
+  explicit EngineT(const EngineOptions& opt) : opt_(opt) {
+    auto t0 = std::chrono::steady_clock::now();
+    STL_CHECK(opt.nside >= 1 && (opt.nside & (opt.nside - 1)) == 0, "nside must be a power of two");
+    int order0 = 0;
+    while ((1 << order0) < opt.nside) ++order0;
+    STL_CHECK(opt.J >= 1 && opt.J <= order0 + 1, "invalid number of scales");
+    STL_CHECK(opt.J <= 12, "at most 12 scales");
+    STL_CHECK(opt.batch >= 1, "batch must be positive");
+    STL_HIP(hipSetDevice(opt.device));
+    STL_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    prof_.enable(opt.timing, s_);
+    plan_ = build_plan(opt.J, kL);
+    levels_ = opt.J;
+    for (auto& c : plan_) slots_.push_back(slot_of(c));
+    B_ = opt.batch;
+    std::vector<double> ker = wavelet_kernel_f32(5);
+    conv_.resize(levels_);
+    dec_.resize(levels_ > 1 ? levels_ - 1 : 0);
+    for (int n = 0; n < levels_; ++n) {
+      int order = order0 - n;
+      N_.push_back(12 * (int64_t(1) << (2 * order)));
+      if constexpr (std::is_same<T, double>::value) build_conv(order, ker, &conv_[n], nullptr, s_);
+      else build_conv(order, ker, nullptr, &conv_[n], s_);
+      conv_slots_.push_back(count_slots(conv_[n].cnt, conv_[n].view().R));
+      if (n + 1 < levels_) {
+        if constexpr (std::is_same<T, double>::value) build_dec(order, &dec_[n], nullptr, s_);
+        else build_dec(order, nullptr, &dec_[n], s_);
+        dec_slots_.push_back(count_slots(dec_[n].cnt, dec_[n].view().Rc));
+        dec_tslots_.push_back(count_slots(dec_[n].tcnt, dec_[n].view().Rf));
+      }
+      conv_tslots_.push_back(count_slots(conv_[n].tcnt, conv_[n].view().R));
+    }
+    geometry_bytes_ = device_bytes_live();
+    allocate();
+    STL_HIP(hipStreamSynchronize(s_));
+    setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  ~EngineT() override {
+    if (s_) (void)hipStreamDestroy(s_);
+  }
+
+  const EngineOptions& options() const override { return opt_; }
+  const std::vector<Coefficient>& plan() const override { return plan_; }
+  int levels() const override { return levels_; }
+  hipStream_t stream() const override { return s_; }
+  size_t device_bytes() const override { return device_bytes_live(); }
+  double setup_seconds() const override { return setup_s_; }
+  std::vector<KernelTimer> timers() const override { return prof_.timers(); }
+  void reset_timers() override { prof_.reset(); }
+
+  void forward_device(const double* x_dev, bool moments, cplx* values, double* variance) override {
+    load_maps(x_dev);
+    for (int n = 1; n < levels_; ++n) dec_maps(n);
+    sweep_forward(moments, values, variance);
+  }
+
+  void forward_levels_device(const double* x_dev, cplx* values) override {
+    size_t at = 0;
+    for (int n = 0; n < levels_; ++n) {
+      int64_t len = int64_t(B_) * N_[n];
+      cast_in(x_dev + at, maps_[n].data(), len);
+      at += size_t(len);
+    }
+    sweep_forward(false, values, nullptr);
+  }
+
+  void vjp_device(const cplx* cot, double* grad_dev) override {
+    size_t np = plan_.size();
+    for (int n = levels_ - 1; n >= 0; --n) {
+      const int64_t N = N_[n];
+      const int K = gram_k(n);
+      auto& H = hH_[n];
+      std::fill(H.begin(), H.end(), double2{0.0, 0.0});
+      auto& s1 = hs1_[n];
+      auto& s2 = hs2_[n];
+      std::fill(s1.begin(), s1.end(), 0.0);
+      std::fill(s2.begin(), s2.end(), 0.0);
+      if (n == 0) {
+        std::fill(hgm_.begin(), hgm_.end(), 0.0);
+        std::fill(hgv_.begin(), hgv_.end(), 0.0);
+      }
+      for (size_t i = 0; i < np; ++i) {
+        const SumSlot& sl = slots_[i];
+        if (sl.n != n) continue;
+        const double Nd = double(sl.kind == SumSlot::MEANVAR ? N_[0] : N);
+        for (int b = 0; b < B_; ++b) {
+          cplx g = cot[size_t(b) * np + i];
+          if (g == cplx(0.0, 0.0)) continue;
+          switch (sl.kind) {
+            case SumSlot::MEANVAR:
+              if (sl.idx == 0) hgm_[b] += g.real() / Nd;
+              else hgv_[b] += 2.0 * g.real() / Nd;
+              break;
+            case SumSlot::ZSTAT:
+              if (sl.idx == 0) s1[b * kL + sl.l] += g.real() / Nd;
+              else s2[b * kL + sl.l] += 2.0 * g.real() / Nd;
+              break;
+            case SumSlot::GRAM: {
+              double2* h = &H[size_t(b * kL + sl.l) * K * K];
+              cplx a = g / Nd, c = std::conj(g) / Nd;
+              h[sl.gi * K + sl.gj].x += a.real();
+              h[sl.gi * K + sl.gj].y += a.imag();
+              h[sl.gj * K + sl.gi].x += c.real();
+              h[sl.gj * K + sl.gi].y += c.imag();
+              break;
+            }
+          }
+        }
+      }
+      STL_HIP(hipMemcpyAsync(dH_[n].data(), H.data(), H.size() * sizeof(double2), hipMemcpyHostToDevice, s_));
+      STL_HIP(hipMemcpyAsync(ds1_[n].data(), s1.data(), s1.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+      STL_HIP(hipMemcpyAsync(ds2_[n].data(), s2.data(), s2.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+
+      int id = prof_.start();
+      launch_gv<T>(n, N, B_, br_[n].data(), z_[n].data(), dH_[n].data(), gbr_.data(), gzp_.data(), s_);
+      prof_.stop(id, tag("gv", n), double((n + 2) * B_ * kL) * N * kL * 2 * sizeof(T) * 2);
+
+      T* chain = acc_[n % 2].data();
+      T* chain_prev = acc_[(n + 1) % 2].data();
+      conv_adj(n, kL, gbr_.data(), (n + 1) * B_, chain);
+      if (n + 1 < levels_) dec_adj(n, kL, chain_prev, chain, (n + 1) * B_, chain);
+
+      id = prof_.start();
+      launch_gz<T>(n, N, B_, z_[n].data(), gzp_.data(), chain, ds1_[n].data(), ds2_[n].data(), gz_.data(), s_);
+      prof_.stop(id, tag("gz", n), double(B_) * kL * N * (6 * sizeof(T)));
+
+      T* mg = mg_[n % 2].data();
+      T* mg_prev = mg_[(n + 1) % 2].data();
+      conv_adj(n, 1, gz_.data(), B_, mg);
+      if (n + 1 < levels_) dec_adj(n, 1, mg_prev, mg, B_, mg);
+    }
+    STL_HIP(hipMemcpyAsync(dgm_.data(), hgm_.data(), hgm_.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+    STL_HIP(hipMemcpyAsync(dgv_.data(), hgv_.data(), hgv_.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+    launch_meanvar_grad<T>(N_[0], B_, maps_[0].data(), dgm_.data(), dgv_.data(), mg_[0].data(), s_);
+    cast_out(mg_[0].data(), grad_dev, int64_t(B_) * N_[0]);
+    STL_HIP(hipStreamSynchronize(s_));
+    prof_.collect();
+  }
+
+  void apply_operator(int kind, int n, int rows, const double* in, double* out) override {
+    STL_CHECK(n >= 0 && n < levels_, "level out of range");
+    STL_CHECK(rows >= 1, "rows must be positive");
+    const int64_t N = N_[n];
+    if (kind == 0) {
+      DevArray<T> x(size_t(rows) * N);
+      DevArray<C2> y(size_t(rows) * kL * N);
+      cast_in(in, x.data(), int64_t(rows) * N);
+      launch_conv_fwd<T>(conv_[n].view(), 1, x.data(), rows, y.data(), nullptr, s_);
+      cast_out(reinterpret_cast<const T*>(y.data()), out, int64_t(rows) * kL * N * 2);
+    } else if (kind == 1) {
+      DevArray<C2> gy(size_t(rows) * kL * N);
+      DevArray<T> gx(size_t(rows) * N);
+      cast_in(in, reinterpret_cast<T*>(gy.data()), int64_t(rows) * kL * N * 2);
+      launch_conv_adj<T>(conv_[n].view(), 1, gy.data(), rows, gx.data(), s_);
+      cast_out(gx.data(), out, int64_t(rows) * N);
+    } else if (kind == 2 || kind == 3) {
+      STL_CHECK(n + 1 < levels_, "no coarser level");
+      const int64_t Nc = N_[n + 1];
+      DevArray<T> a(size_t(rows) * (kind == 2 ? N : Nc)), b(size_t(rows) * (kind == 2 ? Nc : N));
+      cast_in(in, a.data(), int64_t(a.size()));
+      if (kind == 2) launch_dec_fwd<T>(dec_[n].view(), 1, a.data(), rows, b.data(), s_);
+      else launch_dec_adj<T>(dec_[n].view(), 1, a.data(), nullptr, rows, b.data(), s_);
+      cast_out(b.data(), out, int64_t(b.size()));
+    } else {
+      throw std::runtime_error("unknown operator kind");
+    }
+    STL_HIP(hipStreamSynchronize(s_));
+  }
+
+  std::vector<char> save_state() const override {
+    std::vector<char> out;
+    auto put = [&](const void* d, size_t bytes) {
+      size_t at = out.size();
+      out.resize(at + bytes);
+      STL_HIP(hipMemcpy(out.data() + at, d, bytes, hipMemcpyDeviceToHost));
+    };
+    for (int n = 0; n < levels_; ++n) {
+      put(maps_[n].data(), maps_[n].bytes());
+      put(z_[n].data(), z_[n].bytes());
+      put(M_[n].data(), M_[n].bytes());
+      put(br_[n].data(), br_[n].bytes());
+    }
+    return out;
+  }
+
+  void load_state(const std::vector<char>& s) override {
+    size_t at = 0;
+    auto get = [&](void* d, size_t bytes) {
+      STL_CHECK(at + bytes <= s.size(), "checkpoint size mismatch");
+      STL_HIP(hipMemcpy(d, s.data() + at, bytes, hipMemcpyHostToDevice));
+      at += bytes;
+    };
+    for (int n = 0; n < levels_; ++n) {
+      get(maps_[n].data(), maps_[n].bytes());
+      get(z_[n].data(), z_[n].bytes());
+      get(M_[n].data(), M_[n].bytes());
+      get(br_[n].data(), br_[n].bytes());
+    }
+    STL_CHECK(at == s.size(), "checkpoint size mismatch");
+  }
+
+ private:
+  double count_slots(const DevArray<uint8_t>& cnt, int64_t n) {
+    std::vector<uint8_t> h(static_cast<size_t>(n));
+    STL_HIP(hipMemcpy(h.data(), cnt.data(), size_t(n), hipMemcpyDeviceToHost));
+    double s = 0.0;
+    for (auto v : h) s += v;
+    return s;
+  }
+
+  void allocate() {
+    maps_.resize(levels_);
+    z_.resize(levels_);
+    M_.resize(levels_);
+    br_.resize(levels_);
+    gpart_.resize(levels_);
+    gpart2_.resize(levels_);
+    gsum_.resize(levels_);
+    gsum2_.resize(levels_);
+    zpart_.resize(levels_);
+    zsum_.resize(levels_);
+    hgsum_.resize(levels_);
+    hgsum2_.resize(levels_);
+    hzsum_.resize(levels_);
+    dH_.resize(levels_);
+    hH_.resize(levels_);
+    ds1_.resize(levels_);
+    ds2_.resize(levels_);
+    hs1_.resize(levels_);
+    hs2_.resize(levels_);
+    for (int n = 0; n < levels_; ++n) {
+      int64_t N = N_[n];
+      maps_[n].resize(size_t(B_) * N);
+      z_[n].resize(size_t(B_) * kL * N);
+      M_[n].resize(size_t(n + 1) * B_ * N * kL);
+      br_[n].resize(size_t(n + 1) * B_ * kL * N * kL);
+      int K = gram_k(n), E = gram_e(n);
+      int64_t tiles = (N + gram_tile(n) - 1) / gram_tile(n);
+      int nb = int(std::min<int64_t>(tiles, std::max<int64_t>(1, 2048 / (B_ * kL))));
+      nblk_gram_.push_back(nb);
+      gpart_[n].resize(size_t(nb) * B_ * kL * E * 2);
+      gpart2_[n].resize(size_t(nb) * B_ * kL * E);
+      gsum_[n].resize(size_t(B_) * kL * E * 2);
+      gsum2_[n].resize(size_t(B_) * kL * E);
+      hgsum_[n].resize(gsum_[n].size());
+      hgsum2_[n].resize(gsum2_[n].size());
+      int nz = int(std::min<int64_t>((N + 255) / 256, 256));
+      nblk_z_.push_back(nz);
+      zpart_[n].resize(size_t(nz) * B_ * kL * 4);
+      zsum_[n].resize(size_t(B_) * kL * 4);
+      hzsum_[n].resize(zsum_[n].size());
+      dH_[n].resize(size_t(B_) * kL * K * K);
+      hH_[n].resize(size_t(B_) * kL * K * K);
+      ds1_[n].resize(size_t(B_) * kL);
+      ds2_[n].resize(size_t(B_) * kL);
+      hs1_[n].resize(size_t(B_) * kL);
+      hs2_[n].resize(size_t(B_) * kL);
+    }
+    nblk_mv_ = int(std::min<int64_t>((N_[0] + 255) / 256, 256));
+    mvpart_.resize(size_t(nblk_mv_) * B_ * 4);
+    mvsum_.resize(size_t(B_) * 4);
+    hmvsum_.resize(mvsum_.size());
+    gbr_.resize(br_[0].size());
+    for (int n = 1; n < levels_; ++n) gbr_.resize(std::max(gbr_.size(), br_[n].size()));
+    gzp_.resize(z_[0].size());
+    gz_.resize(z_[0].size());
+    acc_[0].resize(M_[0].size());
+    acc_[1].resize(std::max(M_[0].size(), levels_ > 1 ? M_[1].size() : size_t(0)));
+    mg_[0].resize(maps_[0].size());
+    mg_[1].resize(maps_[0].size());
+    dgm_.resize(B_);
+    dgv_.resize(B_);
+    hgm_.assign(B_, 0.0);
+    hgv_.assign(B_, 0.0);
+  }
+
+  void cast_in(const double* x, T* out, int64_t n) {
+    if (std::is_same<T, double>::value)
+      STL_HIP(hipMemcpyAsync(out, x, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s_));
+    else
+      launch_cast<float, double>(n, x, reinterpret_cast<float*>(out), s_);
+  }
+
+  void cast_out(const T* g, double* out, int64_t n) {
+    if (std::is_same<T, double>::value)
+      STL_HIP(hipMemcpyAsync(out, g, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s_));
+    else
+      launch_cast<double, float>(n, reinterpret_cast<const float*>(g), out, s_);
+  }
+
+  void load_maps(const double* x_dev) { cast_in(x_dev, maps_[0].data(), int64_t(B_) * N_[0]); }
+
+  void dec_maps(int n) {
+    int id = prof_.start();
+    launch_dec_fwd<T>(dec_[n - 1].view(), 1, maps_[n - 1].data(), B_, maps_[n].data(), s_);
+    prof_.stop(id, tag("dec_fwd", n), dec_bytes(n - 1, 1, B_));
+  }
+
+  double conv_bytes(int n, int NI, int rows, bool adj) const {
+    double slots = adj ? conv_tslots_[n] : conv_slots_[n];
+    double coef = slots * (2 * kL * sizeof(T) + 4);
+    double in = adj ? double(rows) * kL * N_[n] * NI * 2 * sizeof(T) : double(rows) * N_[n] * NI * sizeof(T);
+    double out = adj ? double(rows) * N_[n] * NI * sizeof(T) : double(rows) * kL * N_[n] * NI * 2 * sizeof(T);
+    return coef + in + out;
+  }
+
+  double dec_bytes(int n, int NI, int rows, bool adj = false) const {
+    double slots = adj ? dec_tslots_[n] : dec_slots_[n];
+    return slots * (sizeof(T) + 4) + double(rows) * (N_[n] + N_[n + 1]) * NI * sizeof(T) * (adj ? 1.5 : 1.0);
+  }
+
+  void conv_adj(int n, int NI, const C2* gy, int rows, T* gx) {
+    int id = prof_.start();
+    launch_conv_adj<T>(conv_[n].view(), NI, gy, rows, gx, s_);
+    prof_.stop(id, tag(NI == 1 ? "conv_adj_z" : "conv_adj_branch", n), conv_bytes(n, NI, rows, true));
+  }
+
+  void dec_adj(int n, int NI, const T* gy, const T* add, int rows, T* gx) {
+    int id = prof_.start();
+    launch_dec_adj<T>(dec_[n].view(), NI, gy, add, rows, gx, s_);
+    prof_.stop(id, tag("dec_adj", n), dec_bytes(n, NI, rows, true));
+  }
+
+  void sweep_forward(bool moments, cplx* values, double* variance) {
+    for (int n = 0; n < levels_; ++n) {
+      const int64_t N = N_[n];
+      int id = prof_.start();
+      launch_conv_fwd<T>(conv_[n].view(), 1, maps_[n].data(), B_, z_[n].data(),
+                         M_[n].data() + size_t(n) * B_ * N * kL, s_);
+      prof_.stop(id, tag("conv_fwd_z", n), conv_bytes(n, 1, B_, false) + double(B_) * N * kL * sizeof(T));
+      if (n > 0) {
+        id = prof_.start();
+        launch_dec_fwd<T>(dec_[n - 1].view(), kL, M_[n - 1].data(), n * B_, M_[n].data(), s_);
+        prof_.stop(id, tag("dec_fwd", n), dec_bytes(n - 1, kL, n * B_));
+      }
+      id = prof_.start();
+      launch_conv_fwd<T>(conv_[n].view(), kL, M_[n].data(), (n + 1) * B_, br_[n].data(), nullptr, s_);
+      prof_.stop(id, tag("conv_fwd_branch", n), conv_bytes(n, kL, (n + 1) * B_, false));
+      id = prof_.start();
+      int nb = nblk_gram_[n];
+      int64_t m = int64_t(B_) * kL * gram_e(n);
+      launch_gram<T>(n, N, B_, br_[n].data(), z_[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(), s_);
+      launch_reduce_parts(gpart_[n].data(), nb, m * 2, gsum_[n].data(), s_);
+      if (moments) launch_reduce_parts(gpart2_[n].data(), nb, m, gsum2_[n].data(), s_);
+      prof_.stop(id, tag("gram", n), double(B_) * kL * N * (gram_k(n)) * 2 * sizeof(T));
+      id = prof_.start();
+      launch_zstats<T>(N, B_, z_[n].data(), nblk_z_[n], zpart_[n].data(), s_);
+      launch_reduce_parts(zpart_[n].data(), nblk_z_[n], int64_t(B_) * kL * 4, zsum_[n].data(), s_);
+      if (n == 0) {
+        launch_meanvar<T>(N, B_, maps_[0].data(), nblk_mv_, mvpart_.data(), s_);
+        launch_reduce_parts(mvpart_.data(), nblk_mv_, int64_t(B_) * 4, mvsum_.data(), s_);
+      }
+      prof_.stop(id, tag("zstats", n), double(B_) * kL * N * 2 * sizeof(T));
+      STL_HIP(hipMemcpyAsync(hgsum_[n].data(), gsum_[n].data(), gsum_[n].bytes(), hipMemcpyDeviceToHost, s_));
+      if (moments)
+        STL_HIP(hipMemcpyAsync(hgsum2_[n].data(), gsum2_[n].data(), gsum2_[n].bytes(), hipMemcpyDeviceToHost, s_));
+      STL_HIP(hipMemcpyAsync(hzsum_[n].data(), zsum_[n].data(), zsum_[n].bytes(), hipMemcpyDeviceToHost, s_));
+    }
+    STL_HIP(hipMemcpyAsync(hmvsum_.data(), mvsum_.data(), mvsum_.bytes(), hipMemcpyDeviceToHost, s_));
+    STL_HIP(hipStreamSynchronize(s_));
+    prof_.collect();
+    size_t np = plan_.size();
+    for (int b = 0; b < B_; ++b)
+      for (size_t i = 0; i < np; ++i) {
+        const SumSlot& sl = slots_[i];
+        cplx v;
+        double second = 0.0;
+        switch (sl.kind) {
+          case SumSlot::MEANVAR: {
+            double Nd = double(N_[0]);
+            v = cplx(hmvsum_[b * 4 + sl.idx] / Nd, 0.0);
+            second = hmvsum_[b * 4 + 2 + sl.idx] / Nd;
+            break;
+          }
+          case SumSlot::ZSTAT: {
+            double Nd = double(N_[sl.n]);
+            const double* z = &hzsum_[sl.n][size_t(b * kL + sl.l) * 4];
+            v = cplx(z[sl.idx] / Nd, 0.0);
+            second = z[2 + sl.idx] / Nd;
+            break;
+          }
+          case SumSlot::GRAM: {
+            double Nd = double(N_[sl.n]);
+            int E = gram_e(sl.n);
+            size_t e = size_t(b * kL + sl.l) * E + sl.idx;
+            double re = hgsum_[sl.n][2 * e], im = hgsum_[sl.n][2 * e + 1];
+            v = cplx(re / Nd, (sl.conj ? -im : im) / Nd);
+            if (moments) second = hgsum2_[sl.n][e] / Nd;
+            break;
+          }
+        }
+        values[b * np + i] = v;
+        if (moments && variance) variance[b * np + i] = std::max(second - std::norm(v), 0.0);
+      }
+  }
+
+  EngineOptions opt_;
+  hipStream_t s_ = nullptr;
+  Profiler prof_;
+  std::vector<Coefficient> plan_;
+  std::vector<SumSlot> slots_;
+  int levels_ = 0, B_ = 1;
+  std::vector<int64_t> N_;
+  std::vector<ConvStore<T>> conv_;
+  std::vector<DecStore<T>> dec_;
+  std::vector<double> conv_slots_, conv_tslots_, dec_slots_, dec_tslots_;
+  size_t geometry_bytes_ = 0;
+  double setup_s_ = 0.0;
+  // forward state
+  std::vector<DevArray<T>> maps_, M_;
+  std::vector<DevArray<C2>> z_, br_;
+  // reductions
+  std::vector<int> nblk_gram_, nblk_z_;
+  int nblk_mv_ = 1;
+  std::vector<DevArray<double>> gpart_, gpart2_, gsum_, gsum2_, zpart_, zsum_;
+  DevArray<double> mvpart_, mvsum_;
+  std::vector<std::vector<double>> hgsum_, hgsum2_, hzsum_;
+  std::vector<double> hmvsum_;
+  // reverse
+  DevArray<C2> gbr_, gzp_, gz_;
+  DevArray<T> acc_[2], mg_[2];
+  std::vector<DevArray<double2>> dH_;
+  std::vector<std::vector<double2>> hH_;
+  std::vector<DevArray<double>> ds1_, ds2_;
+  std::vector<std::vector<double>> hs1_, hs2_;
+  DevArray<double> dgm_, dgv_;
+  std::vector<double> hgm_, hgv_;
+};
+
+}  // namespace
+
+std::unique_ptr<Engine> Engine::create(const EngineOptions& opt) {
+  if (opt.fp64) return std::unique_ptr<Engine>(new EngineT<double>(opt));
+  return std::unique_ptr<Engine>(new EngineT<float>(opt));
+}
+
+}  // namespace stl

@@ -1,0 +1,76 @@
+// The ScatCov engine on one GPU: the whole sky, every level resident in HBM.
+// Mirrors the reference's Evaluator (crates/stl-core/src/engine.rs:254-271):
+// forward(x, moments) -> (values, variances), gradient(cotangent) -> map
+// cotangent, for the complete one-channel plan (fewer = false, dg(j) = j,
+// gauge cosmo, kernel 5, L = 4).
+#pragma once
+
+#include <complex>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+namespace stl {
+
+typedef std::complex<double> cplx;
+
+enum Family { F_MEAN = 0, F_VAR = 1, F_S1 = 2, F_S2 = 3, F_S3 = 4, F_S4 = 5 };
+
+// One coefficient of the plan (crates/stl-core/src/plan.rs:345-366); -1 = None.
+struct Coefficient {
+  int family, native_dg, c1, c2;
+  int j1 = -1, j2 = -1, j3 = -1, l1 = -1, l2 = -1, l3 = -1;
+};
+
+// The complete plan of one channel, sorted by native level (plan.rs:378-456).
+std::vector<Coefficient> build_plan(int J, int L);
+
+struct EngineOptions {
+  int nside = 128;
+  int J = 7;
+  int batch = 1;
+  bool fp64 = true;      // false: FP32 bulk fields and operators, FP64 sums
+  int device = 0;
+  bool timing = false;   // per-kernel HIP-event timers
+};
+
+// Per-kernel-family timing (HIP events on the engine stream).
+struct KernelTimer {
+  std::string name;
+  double ms = 0.0;
+  long launches = 0;
+  double bytes = 0.0;  // algorithmic bytes of the timed launches
+};
+
+class Engine {
+ public:
+  virtual ~Engine() = default;
+  static std::unique_ptr<Engine> create(const EngineOptions& opt);
+
+  virtual const EngineOptions& options() const = 0;
+  virtual const std::vector<Coefficient>& plan() const = 0;
+  virtual int levels() const = 0;
+  // Forward on maps [B][12 nside^2] in device memory (float64); values and
+  // variances ([B][ncoef]) to host memory.
+  virtual void forward_device(const double* x_dev, bool moments, cplx* values, double* variance) = 0;
+  // Forward on maps given at every level (concatenated, device memory), no decimation of level 0.
+  virtual void forward_levels_device(const double* x_dev, cplx* values) = 0;
+  // Map cotangent [B][12 nside^2] (device, float64) of Re sum(conj(cot) values) after forward.
+  virtual void vjp_device(const cplx* cot, double* grad_dev) = 0;
+  virtual hipStream_t stream() const = 0;
+  virtual size_t device_bytes() const = 0;
+  virtual double setup_seconds() const = 0;
+  virtual std::vector<KernelTimer> timers() const = 0;
+  virtual void reset_timers() = 0;
+  // One level's operator applied to float64 device arrays (kind 0 conv forward
+  // [rows][N] -> [rows][4][N] complex, 1 its adjoint, 2 decimation to level + 1,
+  // 3 its adjoint); for operator-level tests.
+  virtual void apply_operator(int kind, int level, int rows, const double* in_dev, double* out_dev) = 0;
+  // Host copy of the forward state (maps, z, moduli, branches) and its restore.
+  virtual std::vector<char> save_state() const = 0;
+  virtual void load_state(const std::vector<char>& s) = 0;
+};
+
+}  // namespace stl

@@ -1,0 +1,561 @@
+// One-off device build of the pyramid operators (FP64 geometry), stored for the
+// rep faces 0, 4, 8.  Compiled with -ffp-contract=off so that products and
+// sums round like the reference's scalar code.
+//
+// Convolution (reference conv.rs:73-135 + dia.rs:213-238): per rep row and
+// gauge, the kL x 25 stencil taps are interpolated on the ring scheme
+// (4 cells each, renormalised over the cells found -- all of them on the full
+// sky); the kernel is then folded into one coefficient per distinct cell read
+// with a nonzero weight, per gauge and (re, im), summed over taps then
+// bilinear neighbours like DiaGeometry::from_conv.  Slots are sorted by cell id.
+//
+// Decimation (decimation.rs:450-561): per rep coarse row, the candidates are
+// the children of the 2-ring (5 x 5 extended-frame window) of the coarse cell,
+// or every fine pixel below coarse nside 4; Gaussian-cone weights within
+// radius = 3 fine pixels, sigma = radius / 2, haversine distance, l1 rows,
+// zeros dropped, columns ascending.
+//
+// Transposes: every forward entry (row p, cell c) of a rep row is moved by the
+// one face shift that brings c into a rep face; entries of a rep cell are
+// sorted by row id, which fixes the summation order of the adjoints.
+#include "geometry.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "healpix_device.hpp"
+
+namespace stl {
+
+namespace {
+
+constexpr int kTaps = 25;  // kernel 5 x 5
+constexpr int kEntries = kL * kTaps * 4;
+constexpr int kDecCand = 256;
+
+__global__ void k_gridvec(int ksz, int64_t nside, double* out) {
+  int p = threadIdx.x;
+  if (p >= ksz * ksz) return;
+  // stencil.rs:33-47
+  int64_t half = ksz / 2;
+  double ns = double(nside);
+  double alpha_pix = sqrt(4.0 * hp::kPi / (12.0 * ns * ns));
+  int64_t x = (p % ksz) - half, y = (p / ksz) - half;
+  double dtheta = sqrt(double(x * x + y * y)) * alpha_pix;
+  double dphi = atan2(double(y), double(x));
+  out[p * 3 + 0] = sin(dtheta) * cos(dphi);
+  out[p * 3 + 1] = sin(dtheta) * sin(dphi);
+  out[p * 3 + 2] = cos(dtheta);
+}
+
+// Thread per (rep row of the chunk, gauge): 25 taps x 4 (cell, weight).
+__global__ void k_taps(int order, int64_t row0, int64_t nrows, const double* __restrict__ gridvec,
+                       int32_t* __restrict__ tcell, double* __restrict__ tw) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= nrows * kL) return;
+  int64_t i = t / kL;
+  int g = int(t % kL);
+  int64_t pix = rep_pixel(row0 + i, order);
+  double theta, phi;
+  hp::pix2ang_nest(pix, order, theta, phi);
+  // centres rounded through degrees (conv.rs:89-92)
+  theta = hp::deg2rad(90.0 - hp::rad2deg(hp::kPi / 2.0 - theta));
+  phi = hp::deg2rad(hp::rad2deg(phi));
+  double r[9];
+  hp::rotation_cosmo(theta, phi, g, kL, r);
+  for (int p = 0; p < kTaps; ++p) {
+    double v0 = gridvec[p * 3], v1 = gridvec[p * 3 + 1], v2 = gridvec[p * 3 + 2];
+    double d[3] = {r[0] * v0 + r[1] * v1 + r[2] * v2, r[3] * v0 + r[4] * v1 + r[5] * v2,
+                   r[6] * v0 + r[7] * v1 + r[8] * v2};
+    int64_t pc[4];
+    double w[4];
+    hp::interp_weights_nested(d, order, pc, w);
+    // Binding on the full sky: every cell is found; weights renormalised by their sum.
+    double colsum = w[0] + w[1] + w[2] + w[3];
+    if (colsum == 0.0) {
+      w[0] = 1.0;
+      colsum = w[0] + w[1] + w[2] + w[3];
+    }
+    double den = colsum > 1e-12 ? colsum : 1e-12;
+    size_t base = ((size_t(i) * kL + g) * kTaps + p) * 4;
+    for (int n = 0; n < 4; ++n) {
+      tcell[base + n] = int32_t(pc[n]);
+      tw[base + n] = w[n] / den;
+    }
+  }
+}
+
+// One wavefront per rep row: distinct cells with a nonzero weight, sorted, and
+// their folded coefficients.  WRITE = false only counts the slots.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_fold(int64_t row0, int64_t nrows, int64_t R,
+                                              const int32_t* __restrict__ tcell, const double* __restrict__ tw,
+                                              const double* __restrict__ ker, uint8_t* __restrict__ cnt,
+                                              int32_t* __restrict__ cell, double* __restrict__ coef64,
+                                              float* __restrict__ coef32, int* __restrict__ err) {
+  __shared__ int32_t ec[4][kEntries];
+  __shared__ double ew[4][kEntries];
+  __shared__ int32_t slots[4][kMaxSlots];
+  __shared__ int32_t sorted[4][kMaxSlots];
+  __shared__ int nsl[4];
+  int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int64_t i = int64_t(blockIdx.x) * 4 + wv;
+  bool active = i < nrows;
+  if (lane == 0) nsl[wv] = 0;
+  for (int e = lane; e < kEntries; e += 64) {
+    ec[wv][e] = active ? tcell[size_t(i) * kEntries + e] : -1;
+    ew[wv][e] = active ? tw[size_t(i) * kEntries + e] : 0.0;
+  }
+  __syncthreads();
+  for (int e = lane; e < kEntries; e += 64) {
+    if (ew[wv][e] == 0.0) continue;
+    int32_t c = ec[wv][e];
+    bool first = true;
+    for (int e2 = 0; e2 < e; ++e2)
+      if (ec[wv][e2] == c && ew[wv][e2] != 0.0) { first = false; break; }
+    if (first) {
+      int pos = atomicAdd(&nsl[wv], 1);
+      if (pos < kMaxSlots) slots[wv][pos] = c;
+    }
+  }
+  __syncthreads();
+  int ns = nsl[wv];
+  if (ns > kMaxSlots) {
+    if (lane == 0) atomicOr(err, 1);
+    ns = 0;
+  }
+  if (lane < ns) {
+    int32_t c = slots[wv][lane];
+    int rank = 0;
+    for (int j = 0; j < ns; ++j) rank += slots[wv][j] < c ? 1 : 0;
+    sorted[wv][rank] = c;
+  }
+  __syncthreads();
+  if (!active) return;
+  int64_t r = row0 + i;
+  if (lane == 0) cnt[r] = uint8_t(ns);
+  if (!WRITE || lane >= ns) return;
+  int32_t c = sorted[wv][lane];
+  double acc[kL][2];
+  for (int g = 0; g < kL; ++g) {
+    acc[g][0] = 0.0;
+    acc[g][1] = 0.0;
+    for (int p = 0; p < kTaps; ++p)
+      for (int n = 0; n < 4; ++n) {
+        int e = (g * kTaps + p) * 4 + n;
+        double w = ew[wv][e];
+        if (ec[wv][e] == c && w != 0.0) {
+          acc[g][0] += ker[p] * w;
+          acc[g][1] += ker[kTaps + p] * w;
+        }
+      }
+  }
+  size_t at = size_t(lane) * R + r;
+  cell[at] = c;
+  for (int g = 0; g < kL; ++g)
+    for (int o = 0; o < 2; ++o) {
+      if (coef64) coef64[(at * kL + g) * 2 + o] = acc[g][o];
+      if (coef32) coef32[(at * kL + g) * 2 + o] = float(acc[g][o]);
+    }
+}
+
+// One wavefront per rep coarse row of the decimation from order_f.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_dec_rows(int order_f, int64_t Rc, double radius, double sigma,
+                                                  uint8_t* __restrict__ cnt, int32_t* __restrict__ col,
+                                                  double* __restrict__ v64, float* __restrict__ v32, int S,
+                                                  int* __restrict__ err) {
+  __shared__ int32_t cand[4][kDecCand];
+  __shared__ int32_t srt[4][kDecCand];
+  __shared__ double w[4][kDecCand];
+  __shared__ int ncand[4];
+  __shared__ double scale[4];
+  int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int64_t r = int64_t(blockIdx.x) * 4 + wv;
+  bool active = r < Rc;
+  int order_c = order_f - 1;
+  int64_t nside_c = int64_t(1) << order_c;
+  int64_t c = rep_pixel(active ? r : 0, order_c);
+  if (lane == 0) ncand[wv] = 0;
+  __syncthreads();
+  if (active) {
+    if (nside_c >= 4) {
+      if (lane < 25) {
+        int64_t x, y;
+        int f;
+        hp::nest2xyf(c, order_c, x, y, f);
+        int64_t v = hp::ext_pix(f, x + (lane % 5) - 2, y + (lane / 5) - 2, order_c);
+        if (v >= 0) {
+          int pos = atomicAdd(&ncand[wv], 4);
+          for (int q = 0; q < 4; ++q) cand[wv][pos + q] = int32_t(4 * v + q);
+        }
+      }
+    } else {
+      int64_t nf = 12 * (int64_t(1) << (2 * order_f));
+      for (int64_t f = lane; f < nf; f += 64) cand[wv][f] = int32_t(f);
+      if (lane == 0) ncand[wv] = int(nf);
+    }
+  }
+  __syncthreads();
+  int n = active ? ncand[wv] : 0;
+  for (int j = lane; j < n; j += 64) {
+    int32_t v = cand[wv][j];
+    int rank = 0;
+    for (int q = 0; q < n; ++q) rank += cand[wv][q] < v ? 1 : 0;
+    srt[wv][rank] = v;
+  }
+  __syncthreads();
+  double lon_c = 0.0, lat_c = 0.0;
+  if (active) hp::centre_lonlat(c, order_c, lon_c, lat_c);
+  for (int j = lane; j < n; j += 64) {
+    double lon_f, lat_f;
+    hp::centre_lonlat(srt[wv][j], order_f, lon_f, lat_f);
+    double gamma = hp::haversine(lat_c, lon_c, lat_f, lon_f);
+    double t = gamma / sigma;
+    w[wv][j] = gamma > radius ? 0.0 : exp(-0.5 * (t * t));
+  }
+  __syncthreads();
+  if (lane == 0 && active) {
+    double row_sum = 0.0;
+    for (int j = 0; j < n; ++j) row_sum += w[wv][j];
+    if (row_sum <= 0.0 && n > 0) {
+      double u = 1.0 / double(n);
+      for (int j = 0; j < n; ++j) w[wv][j] = u;
+      row_sum = 0.0;
+      for (int j = 0; j < n; ++j) row_sum += w[wv][j];
+    }
+    scale[wv] = row_sum > 1e-300 ? row_sum : 1e-300;
+  }
+  __syncthreads();
+  if (!active) return;
+  int nnz = 0;
+  for (int j = 0; j < n; ++j) nnz += w[wv][j] != 0.0 ? 1 : 0;
+  if (lane == 0) {
+    if (nnz > S || nnz > 255) atomicOr(err, 2);
+    cnt[r] = uint8_t(nnz > 255 ? 255 : nnz);
+  }
+  if (!WRITE || nnz > S) return;
+  for (int j = lane; j < n; j += 64) {
+    if (w[wv][j] == 0.0) continue;
+    int pos = 0;
+    for (int q = 0; q < j; ++q) pos += w[wv][q] != 0.0 ? 1 : 0;
+    size_t at = size_t(pos) * Rc + r;
+    col[at] = srt[wv][j];
+    double v = w[wv][j] / scale[wv];
+    if (v64) v64[at] = v;
+    if (v32) v32[at] = float(v);
+  }
+}
+
+__global__ void k_max_u8(int64_t n, const uint8_t* __restrict__ a, int* __restrict__ out) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  int m = 0;
+  for (int64_t i = t; i < n; i += int64_t(gridDim.x) * blockDim.x) m = max(m, int(a[i]));
+  atomicMax(out, m);
+}
+
+__global__ void k_max_i32(int64_t n, const int* __restrict__ a, int* __restrict__ out) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  int m = 0;
+  for (int64_t i = t; i < n; i += int64_t(gridDim.x) * blockDim.x) m = max(m, a[i]);
+  atomicMax(out, m);
+}
+
+// Target rep cell of forward entry (s, r): the face shift k0 bringing its cell
+// into a rep face.
+__device__ inline void entry_target(int32_t c, int order_col, int& k0, int64_t& qr) {
+  int fc = int(int64_t(c) >> (2 * order_col));
+  k0 = (4 - (fc & 3)) & 3;
+  qr = rep_index(rot_pix(c, k0, order_col), order_col);
+}
+
+__global__ void k_tcount(int64_t R, int S, const uint8_t* __restrict__ cnt, const int32_t* __restrict__ cell,
+                         int order_col, int* __restrict__ tc) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(S) * R) return;
+  int s = int(t / R);
+  int64_t r = t % R;
+  if (s >= cnt[r]) return;
+  int k0;
+  int64_t qr;
+  entry_target(cell[t], order_col, k0, qr);
+  atomicAdd(&tc[qr], 1);
+}
+
+// C = kL * 2 with the gauge permutation (convolution) or 1 (decimation).
+template <typename T, int C>
+__global__ void k_tfill(int64_t R, int S, const uint8_t* __restrict__ cnt, const int32_t* __restrict__ cell,
+                        const T* __restrict__ coef, int order_row, int order_col, int64_t Rq,
+                        int* __restrict__ fill, int32_t* __restrict__ trow, T* __restrict__ tcoef) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(S) * R) return;
+  int s = int(t / R);
+  int64_t r = t % R;
+  if (s >= cnt[r]) return;
+  int k0;
+  int64_t qr;
+  entry_target(cell[t], order_col, k0, qr);
+  int slot = atomicAdd(&fill[qr], 1);
+  size_t at = size_t(slot) * Rq + qr;
+  trow[at] = int32_t(rot_pix(rep_pixel(r, order_row), k0, order_row));
+  if (C == 1) {
+    tcoef[at] = coef[t];
+  } else {
+    for (int g = 0; g < kL; ++g) {
+      int gs = gauge_src(k0, g);
+      T re = coef[(size_t(t) * kL + gs) * 2];
+      T im = coef[(size_t(t) * kL + gs) * 2 + 1];
+      tcoef[(at * kL + g) * 2] = re;
+      tcoef[(at * kL + g) * 2 + 1] = gauge_conj(k0, g) ? -im : im;
+    }
+  }
+}
+
+template <typename T, int C>
+__global__ void k_tsort(int64_t Rq, const int* __restrict__ tc, uint8_t* __restrict__ tcnt,
+                        int32_t* __restrict__ trow, T* __restrict__ tcoef) {
+  int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (q >= Rq) return;
+  int n = tc[q];
+  tcnt[q] = uint8_t(n);
+  for (int a = 1; a < n; ++a) {
+    int32_t key = trow[size_t(a) * Rq + q];
+    T v[C];
+    for (int k = 0; k < C; ++k) v[k] = tcoef[(size_t(a) * Rq + q) * C + k];
+    int b = a - 1;
+    while (b >= 0 && trow[size_t(b) * Rq + q] > key) {
+      trow[size_t(b + 1) * Rq + q] = trow[size_t(b) * Rq + q];
+      for (int k = 0; k < C; ++k) tcoef[(size_t(b + 1) * Rq + q) * C + k] = tcoef[(size_t(b) * Rq + q) * C + k];
+      --b;
+    }
+    trow[size_t(b + 1) * Rq + q] = key;
+    for (int k = 0; k < C; ++k) tcoef[(size_t(b + 1) * Rq + q) * C + k] = v[k];
+  }
+}
+
+inline unsigned blocks_for(int64_t n, int threads) { return unsigned((n + threads - 1) / threads); }
+
+int device_max_u8(const uint8_t* a, int64_t n, hipStream_t s) {
+  DevArray<int> out(1);
+  out.zero(s);
+  k_max_u8<<<std::min<int64_t>(1024, blocks_for(n, 256)), 256, 0, s>>>(n, a, out.data());
+  int h = 0;
+  STL_HIP(hipMemcpyAsync(&h, out.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+  STL_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+int device_max_i32(const int* a, int64_t n, hipStream_t s) {
+  DevArray<int> out(1);
+  out.zero(s);
+  k_max_i32<<<std::min<int64_t>(1024, blocks_for(n, 256)), 256, 0, s>>>(n, a, out.data());
+  int h = 0;
+  STL_HIP(hipMemcpyAsync(&h, out.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+  STL_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+void check_err(DevArray<int>& err, hipStream_t s, const char* what) {
+  int h = 0;
+  STL_HIP(hipMemcpyAsync(&h, err.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+  STL_HIP(hipStreamSynchronize(s));
+  STL_CHECK(h == 0, std::string(what) + ": slot capacity exceeded");
+}
+
+// Transpose of the forward ELL (cnt, cell, coef) into the store's (tcnt, trow, tcoef).
+template <typename T, int C>
+void build_transpose(int64_t R, int S, const DevArray<uint8_t>& cnt, const DevArray<int32_t>& cell,
+                     const DevArray<T>& coef, int order_row, int order_col, DevArray<uint8_t>& tcnt,
+                     DevArray<int32_t>& trow, DevArray<T>& tcoef, int& Tn, hipStream_t s) {
+  int64_t Rq = 3 * (int64_t(1) << (2 * order_col));
+  DevArray<int> tc(Rq);
+  tc.zero(s);
+  int64_t total = int64_t(S) * R;
+  k_tcount<<<blocks_for(total, 256), 256, 0, s>>>(R, S, cnt.data(), cell.data(), order_col, tc.data());
+  STL_HIP(hipGetLastError());
+  Tn = device_max_i32(tc.data(), Rq, s);
+  STL_CHECK(Tn <= 255, "transposed operator: too many rows per cell");
+  trow.resize(size_t(Tn) * Rq);
+  tcoef.resize(size_t(Tn) * Rq * C);
+  trow.zero(s);
+  tcoef.zero(s);
+  tcnt.resize(Rq);
+  DevArray<int> fill(Rq);
+  fill.zero(s);
+  k_tfill<T, C><<<blocks_for(total, 256), 256, 0, s>>>(R, S, cnt.data(), cell.data(), coef.data(), order_row,
+                                                       order_col, Rq, fill.data(), trow.data(), tcoef.data());
+  STL_HIP(hipGetLastError());
+  k_tsort<T, C><<<blocks_for(Rq, 128), 128, 0, s>>>(Rq, tc.data(), tcnt.data(), trow.data(), tcoef.data());
+  STL_HIP(hipGetLastError());
+  STL_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+std::vector<double> wavelet_kernel_f32(int ksz) {
+  // wavelet.rs:121-145 then the f32 round trip of Kernel::shared.
+  const double pi = 3.14159265358979323846;
+  const double sigma = 1.0;
+  int P = ksz * ksz;
+  std::vector<double> re(P), im(P);
+  for (int i = 0; i < ksz; ++i)
+    for (int j = 0; j < ksz; ++j) {
+      double x = double(j) - (double(ksz) - 1.0) / 2.0;
+      double y = double(i) - (double(ksz) - 1.0) / 2.0;
+      double mother = std::exp(-(x * x + y * y) / (2.0 * sigma * sigma));
+      double angle = 0.5 * pi * x;
+      re[i * ksz + j] = std::cos(angle) * mother;
+      im[i * ksz + j] = std::sin(angle) * mother;
+    }
+  double sr = 0.0, si = 0.0;
+  for (int p = 0; p < P; ++p) { sr += re[p]; si += im[p]; }
+  double mr = sr / double(P), mi = si / double(P);
+  for (int p = 0; p < P; ++p) { re[p] -= mr; im[p] -= mi; }
+  double l1 = 0.0;
+  for (int p = 0; p < P; ++p) l1 += std::hypot(re[p], im[p]);
+  std::vector<double> out(2 * P);
+  for (int p = 0; p < P; ++p) {
+    out[p] = double(float(re[p] / l1));
+    out[P + p] = double(float(im[p] / l1));
+  }
+  return out;
+}
+
+void build_conv(int order, const std::vector<double>& ker, ConvStore<double>* c64, ConvStore<float>* c32,
+                hipStream_t s) {
+  STL_CHECK(int(ker.size()) == 2 * kTaps, "kernel 5 only");
+  int64_t nside = int64_t(1) << order;
+  int64_t npf = nside * nside;
+  int64_t R = 3 * npf;
+  DevArray<double> gridvec(kTaps * 3), dker(2 * kTaps);
+  k_gridvec<<<1, 64, 0, s>>>(5, nside, gridvec.data());
+  STL_HIP(hipMemcpyAsync(dker.data(), ker.data(), ker.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  int64_t chunk = std::min<int64_t>(R, int64_t(1) << 18);
+  DevArray<int32_t> tcell(size_t(chunk) * kEntries);
+  DevArray<double> tw(size_t(chunk) * kEntries);
+  DevArray<uint8_t> cnt(R);
+  DevArray<int> err(1);
+  err.zero(s);
+  // Pass 1: slot counts.
+  for (int64_t r0 = 0; r0 < R; r0 += chunk) {
+    int64_t nr = std::min(chunk, R - r0);
+    k_taps<<<blocks_for(nr * kL, 128), 128, 0, s>>>(order, r0, nr, gridvec.data(), tcell.data(), tw.data());
+    k_fold<false><<<blocks_for(nr, 4), 256, 0, s>>>(r0, nr, R, tcell.data(), tw.data(), dker.data(), cnt.data(),
+                                                    nullptr, nullptr, nullptr, err.data());
+    STL_HIP(hipGetLastError());
+  }
+  check_err(err, s, "convolution geometry");
+  int S = device_max_u8(cnt.data(), R, s);
+  DevArray<int32_t> cell(size_t(S) * R);
+  DevArray<double> coef64;
+  DevArray<float> coef32;
+  cell.zero(s);
+  if (c64) { coef64.resize(size_t(S) * R * 2 * kL); coef64.zero(s); }
+  if (c32) { coef32.resize(size_t(S) * R * 2 * kL); coef32.zero(s); }
+  // Pass 2: cells and coefficients.
+  for (int64_t r0 = 0; r0 < R; r0 += chunk) {
+    int64_t nr = std::min(chunk, R - r0);
+    k_taps<<<blocks_for(nr * kL, 128), 128, 0, s>>>(order, r0, nr, gridvec.data(), tcell.data(), tw.data());
+    k_fold<true><<<blocks_for(nr, 4), 256, 0, s>>>(r0, nr, R, tcell.data(), tw.data(), dker.data(), cnt.data(),
+                                                   cell.data(), c64 ? coef64.data() : nullptr,
+                                                   c32 ? coef32.data() : nullptr, err.data());
+    STL_HIP(hipGetLastError());
+  }
+  check_err(err, s, "convolution geometry");
+  tcell.release();
+  tw.release();
+  auto finish = [&](auto* store, auto& coef) {
+    using T = typename std::remove_reference<decltype(coef)>::type;
+    store->order = order;
+    store->S = S;
+    build_transpose<typename std::remove_pointer<decltype(coef.data())>::type, 2 * kL>(
+        R, S, cnt, cell, coef, order, order, store->tcnt, store->trow, store->tcoef, store->Tn, s);
+    (void)sizeof(T);
+  };
+  if (c64) finish(c64, coef64);
+  if (c32) finish(c32, coef32);
+  // Forward arrays: shared ids, owned per store.
+  if (c64 && c32) {
+    c32->cnt.resize(R);
+    c32->cell.resize(cell.size());
+    STL_HIP(hipMemcpyAsync(c32->cnt.data(), cnt.data(), cnt.bytes(), hipMemcpyDeviceToDevice, s));
+    STL_HIP(hipMemcpyAsync(c32->cell.data(), cell.data(), cell.bytes(), hipMemcpyDeviceToDevice, s));
+    c32->coef = std::move(coef32);
+    c64->cnt = std::move(cnt);
+    c64->cell = std::move(cell);
+    c64->coef = std::move(coef64);
+  } else if (c64) {
+    c64->cnt = std::move(cnt);
+    c64->cell = std::move(cell);
+    c64->coef = std::move(coef64);
+  } else if (c32) {
+    c32->cnt = std::move(cnt);
+    c32->cell = std::move(cell);
+    c32->coef = std::move(coef32);
+  }
+  STL_HIP(hipStreamSynchronize(s));
+}
+
+void build_dec(int order_f, DecStore<double>* d64, DecStore<float>* d32, hipStream_t s) {
+  STL_CHECK(order_f >= 1, "decimation needs nside >= 2");
+  const double pi = 3.14159265358979323846;
+  int64_t nside_f = int64_t(1) << order_f;
+  int64_t Rc = 3 * (int64_t(1) << (2 * (order_f - 1)));
+  // decimation.rs:478-484: radius of 3 fine pixels (CPython degrees/radians), sigma = radius / 2.
+  double pix_deg = std::sqrt(4.0 * pi / double(12 * nside_f * nside_f)) * (180.0 / pi);
+  double radius = (3.0 * pix_deg) * (pi / 180.0);
+  double sigma = radius / 2.0;
+  DevArray<uint8_t> cnt(Rc);
+  DevArray<int> err(1);
+  err.zero(s);
+  k_dec_rows<false><<<blocks_for(Rc, 4), 256, 0, s>>>(order_f, Rc, radius, sigma, cnt.data(), nullptr, nullptr,
+                                                      nullptr, 255, err.data());
+  STL_HIP(hipGetLastError());
+  check_err(err, s, "decimation");
+  int S = device_max_u8(cnt.data(), Rc, s);
+  DevArray<int32_t> col(size_t(S) * Rc);
+  DevArray<double> v64;
+  DevArray<float> v32;
+  col.zero(s);
+  if (d64) { v64.resize(size_t(S) * Rc); v64.zero(s); }
+  if (d32) { v32.resize(size_t(S) * Rc); v32.zero(s); }
+  k_dec_rows<true><<<blocks_for(Rc, 4), 256, 0, s>>>(order_f, Rc, radius, sigma, cnt.data(), col.data(),
+                                                     d64 ? v64.data() : nullptr, d32 ? v32.data() : nullptr, S,
+                                                     err.data());
+  STL_HIP(hipGetLastError());
+  check_err(err, s, "decimation");
+  if (d64) {
+    d64->order_f = order_f;
+    d64->S = S;
+    build_transpose<double, 1>(Rc, S, cnt, col, v64, order_f - 1, order_f, d64->tcnt, d64->trow, d64->tval,
+                               d64->Tn, s);
+  }
+  if (d32) {
+    d32->order_f = order_f;
+    d32->S = S;
+    build_transpose<float, 1>(Rc, S, cnt, col, v32, order_f - 1, order_f, d32->tcnt, d32->trow, d32->tval, d32->Tn,
+                              s);
+  }
+  if (d64 && d32) {
+    d32->cnt.resize(Rc);
+    d32->col.resize(col.size());
+    STL_HIP(hipMemcpyAsync(d32->cnt.data(), cnt.data(), cnt.bytes(), hipMemcpyDeviceToDevice, s));
+    STL_HIP(hipMemcpyAsync(d32->col.data(), col.data(), col.bytes(), hipMemcpyDeviceToDevice, s));
+    d32->val = std::move(v32);
+    d64->cnt = std::move(cnt);
+    d64->col = std::move(col);
+    d64->val = std::move(v64);
+  } else if (d64) {
+    d64->cnt = std::move(cnt);
+    d64->col = std::move(col);
+    d64->val = std::move(v64);
+  } else if (d32) {
+    d32->cnt = std::move(cnt);
+    d32->col = std::move(col);
+    d32->val = std::move(v32);
+  }
+  STL_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace stl

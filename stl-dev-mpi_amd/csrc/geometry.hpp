@@ -1,0 +1,74 @@
+// Device-built pyramid operators: the wavelet convolution of every level
+// (folded "kernel x bilinear weight" coefficients per distinct cell) and the
+// Gaussian-cone decimation between levels, each with its transpose for the
+// gather-form adjoint.  Built once per engine, stored for the rep faces only.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+#include "device_buffer.hpp"
+
+namespace stl {
+
+template <typename T>
+struct ConvStore {
+  int order = 0;
+  int S = 0, Tn = 0;
+  DevArray<uint8_t> cnt, tcnt;
+  DevArray<int32_t> cell, trow;
+  DevArray<T> coef, tcoef;
+  ConvOp<T> view() const {
+    ConvOp<T> v;
+    v.order = order;
+    v.npf = int64_t(1) << (2 * order);
+    v.R = 3 * v.npf;
+    v.S = S;
+    v.Tn = Tn;
+    v.cnt = cnt.data(); v.cell = cell.data(); v.coef = coef.data();
+    v.tcnt = tcnt.data(); v.trow = trow.data(); v.tcoef = tcoef.data();
+    return v;
+  }
+  size_t bytes() const {
+    return cnt.bytes() + tcnt.bytes() + cell.bytes() + trow.bytes() + coef.bytes() + tcoef.bytes();
+  }
+};
+
+template <typename T>
+struct DecStore {
+  int order_f = 0;
+  int S = 0, Tn = 0;
+  DevArray<uint8_t> cnt, tcnt;
+  DevArray<int32_t> col, trow;
+  DevArray<T> val, tval;
+  DecOp<T> view() const {
+    DecOp<T> v;
+    v.order_f = order_f;
+    v.order_c = order_f - 1;
+    v.npf_f = int64_t(1) << (2 * order_f);
+    v.npf_c = int64_t(1) << (2 * (order_f - 1));
+    v.Rf = 3 * v.npf_f;
+    v.Rc = 3 * v.npf_c;
+    v.S = S;
+    v.Tn = Tn;
+    v.cnt = cnt.data(); v.col = col.data(); v.val = val.data();
+    v.tcnt = tcnt.data(); v.trow = trow.data(); v.tval = tval.data();
+    return v;
+  }
+  size_t bytes() const { return cnt.bytes() + tcnt.bytes() + col.bytes() + trow.bytes() + val.bytes() + tval.bytes(); }
+};
+
+// Zero-mean l1-normalised Gabor kernel of the wavelet (5 x 5, sigma 1),
+// rounded through float like the reference (wavelet.rs:11-35,
+// conv.rs:166-172): [re taps..., im taps...], 2 * ksz^2 values.
+std::vector<double> wavelet_kernel_f32(int ksz);
+
+// Builds the convolution of the level of `order` (kernel 5, kL gauges,
+// gauge cosmo) into the stores whose pointer is non-null.
+void build_conv(int order, const std::vector<double>& ker, ConvStore<double>* c64, ConvStore<float>* c32,
+                hipStream_t stream);
+
+// Builds the decimation from the level of `order_f` to order_f - 1.
+void build_dec(int order_f, DecStore<double>* d64, DecStore<float>* d32, hipStream_t stream);
+
+}  // namespace stl

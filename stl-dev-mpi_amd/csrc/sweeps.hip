@@ -1,0 +1,605 @@
+// Forward-sweep and VJP kernels of the ScatCov pyramid.  Every kernel is a
+// gather (no floating-point atomics), so results are deterministic; sums over
+// the sky are FP64 with a fixed order (per-thread, then a fixed LDS tree or
+// group order, then blocks in order).
+//
+// Threads of the convolution / decimation kernels are (rep row, face shift k):
+// the 4 lanes of a rep row share its coefficients (one fetch per wavefront),
+// each applies them to its own face through the face-rotation symmetry
+// (gauge g reads the rep coefficient of gauge g ^ 2(k & 1), conjugated per
+// gauge_conj).
+#include "sweeps.hpp"
+
+#include <algorithm>
+
+namespace stl {
+
+namespace {
+
+inline unsigned nblocks(int64_t n, int t) { return unsigned((n + t - 1) / t); }
+
+// Gram entries per thread when there are more entries than threads.
+constexpr int kGramM = 5;
+
+template <typename T>
+struct Load;
+
+template <>
+struct Load<double> {
+  __device__ static inline void coef8(const double* p, double c[8]) {
+    const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double2 v = q[i];
+      c[2 * i] = v.x;
+      c[2 * i + 1] = v.y;
+    }
+  }
+  template <int NI>
+  __device__ static inline void vec(const double* p, double v[NI]) {
+    if constexpr (NI == 4) {
+      const double2* q = reinterpret_cast<const double2*>(p);
+      double2 a = q[0], b = q[1];
+      v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    } else {
+#pragma unroll
+      for (int j = 0; j < NI; ++j) v[j] = p[j];
+    }
+  }
+};
+
+template <>
+struct Load<float> {
+  __device__ static inline void coef8(const float* p, float c[8]) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+    float4 a = q[0], b = q[1];
+    c[0] = a.x; c[1] = a.y; c[2] = a.z; c[3] = a.w;
+    c[4] = b.x; c[5] = b.y; c[6] = b.z; c[7] = b.w;
+  }
+  template <int NI>
+  __device__ static inline void vec(const float* p, float v[NI]) {
+    if constexpr (NI == 4) {
+      float4 a = *reinterpret_cast<const float4*>(p);
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < NI; ++j) v[j] = p[j];
+    }
+  }
+};
+
+template <typename T>
+__device__ inline cplx_t<T> make_c(T re, T im) {
+  cplx_t<T> v;
+  v.x = re;
+  v.y = im;
+  return v;
+}
+
+// Coefficients of face shift k from the rep coefficients c[g][o].
+template <typename T>
+__device__ inline void permute(int k, const T c[8], T a[8]) {
+  bool swap = k & 1;
+#pragma unroll
+  for (int g = 0; g < kL; ++g) {
+    int gs = swap ? (g ^ 2) : g;
+    T re = swap ? c[(g ^ 2) * 2] : c[g * 2];
+    T im = swap ? c[(g ^ 2) * 2 + 1] : c[g * 2 + 1];
+    (void)gs;
+    a[g * 2] = re;
+    a[g * 2 + 1] = gauge_conj(k, g) ? -im : im;
+  }
+}
+
+// y[row][g][p][j] = sum_s A[p, cell_s][g] x[row][cell_s][j]
+template <typename T, int NI>
+__global__ __launch_bounds__(256) void k_conv_fwd(ConvOp<T> op, const T* __restrict__ x,
+                                                  cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= op.R * 4) return;
+  int64_t r = t >> 2;
+  int k = int(t & 3);
+  int row = blockIdx.y;
+  int64_t N = 12 * op.npf;
+  const T* xr = x + size_t(row) * N * NI;
+  T acc[NI][kL][2];
+#pragma unroll
+  for (int j = 0; j < NI; ++j)
+#pragma unroll
+    for (int g = 0; g < kL; ++g) acc[j][g][0] = acc[j][g][1] = T(0);
+  int cnt = op.cnt[r];
+  for (int s = 0; s < cnt; ++s) {
+    size_t at = size_t(s) * op.R + r;
+    int64_t ck = rot_pix(op.cell[at], k, op.order);
+    T c[8], a[8];
+    Load<T>::coef8(op.coef + at * 8, c);
+    permute<T>(k, c, a);
+    T xv[NI];
+    Load<T>::template vec<NI>(xr + size_t(ck) * NI, xv);
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+#pragma unroll
+      for (int g = 0; g < kL; ++g) {
+        acc[j][g][0] += a[g * 2] * xv[j];
+        acc[j][g][1] += a[g * 2 + 1] * xv[j];
+      }
+  }
+  int64_t p = rot_pix(rep_pixel(r, op.order), k, op.order);
+#pragma unroll
+  for (int g = 0; g < kL; ++g)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(acc[j][g][0], acc[j][g][1]);
+  if (mod != nullptr && NI == 1) {
+#pragma unroll
+    for (int g = 0; g < kL; ++g) mod[(size_t(row) * N + p) * kL + g] = hypot(acc[0][g][0], acc[0][g][1]);
+  }
+}
+
+// gx[row][q][j] = sum_rows p reading q sum_g Re(conj-free pairing) A[p, q][g] . gy[row][g][p][j]
+template <typename T, int NI>
+__global__ __launch_bounds__(256) void k_conv_adj(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
+                                                  T* __restrict__ gx) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= op.R * 4) return;
+  int64_t q = t >> 2;
+  int k = int(t & 3);
+  int row = blockIdx.y;
+  int64_t N = 12 * op.npf;
+  const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
+  T acc[NI];
+#pragma unroll
+  for (int j = 0; j < NI; ++j) acc[j] = T(0);
+  int cnt = op.tcnt[q];
+  for (int s = 0; s < cnt; ++s) {
+    size_t at = size_t(s) * op.R + q;
+    int64_t pk = rot_pix(op.trow[at], k, op.order);
+    T c[8], a[8];
+    Load<T>::coef8(op.tcoef + at * 8, c);
+    permute<T>(k, c, a);
+#pragma unroll
+    for (int g = 0; g < kL; ++g) {
+      const cplx_t<T>* src = gr + (size_t(g) * N + pk) * NI;
+#pragma unroll
+      for (int j = 0; j < NI; ++j) {
+        cplx_t<T> v = src[j];
+        acc[j] += a[g * 2] * v.x + a[g * 2 + 1] * v.y;
+      }
+    }
+  }
+  int64_t qk = rot_pix(rep_pixel(q, op.order), k, op.order);
+#pragma unroll
+  for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + qk) * NI + j] = acc[j];
+}
+
+template <typename T, int NI>
+__global__ __launch_bounds__(256) void k_dec_fwd(DecOp<T> op, const T* __restrict__ x, T* __restrict__ y) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= op.Rc * 4) return;
+  int64_t r = t >> 2;
+  int k = int(t & 3);
+  int row = blockIdx.y;
+  int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
+  const T* xr = x + size_t(row) * Nf * NI;
+  T acc[NI];
+#pragma unroll
+  for (int j = 0; j < NI; ++j) acc[j] = T(0);
+  int cnt = op.cnt[r];
+  for (int s = 0; s < cnt; ++s) {
+    size_t at = size_t(s) * op.Rc + r;
+    int64_t fk = rot_pix(op.col[at], k, op.order_f);
+    T v = op.val[at];
+    T xv[NI];
+    Load<T>::template vec<NI>(xr + size_t(fk) * NI, xv);
+#pragma unroll
+    for (int j = 0; j < NI; ++j) acc[j] += v * xv[j];
+  }
+  int64_t p = rot_pix(rep_pixel(r, op.order_c), k, op.order_c);
+#pragma unroll
+  for (int j = 0; j < NI; ++j) y[(size_t(row) * Nc + p) * NI + j] = acc[j];
+}
+
+template <typename T, int NI>
+__global__ __launch_bounds__(256) void k_dec_adj(DecOp<T> op, const T* __restrict__ gy, const T* add,
+                                                 T* gx) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= op.Rf * 4) return;
+  int64_t q = t >> 2;
+  int k = int(t & 3);
+  int row = blockIdx.y;
+  int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
+  const T* gr = gy + size_t(row) * Nc * NI;
+  T acc[NI];
+#pragma unroll
+  for (int j = 0; j < NI; ++j) acc[j] = T(0);
+  int cnt = op.tcnt[q];
+  for (int s = 0; s < cnt; ++s) {
+    size_t at = size_t(s) * op.Rf + q;
+    int64_t ck = rot_pix(op.trow[at], k, op.order_c);
+    T v = op.tval[at];
+    T gv[NI];
+    Load<T>::template vec<NI>(gr + size_t(ck) * NI, gv);
+#pragma unroll
+    for (int j = 0; j < NI; ++j) acc[j] += v * gv[j];
+  }
+  int64_t qk = rot_pix(rep_pixel(q, op.order_f), k, op.order_f);
+  size_t o = (size_t(row) * Nf + qk) * NI;
+#pragma unroll
+  for (int j = 0; j < NI; ++j) gx[o + j] = add ? acc[j] + add[o + j] : acc[j];
+}
+
+template <typename T>
+__device__ inline double2 to_d2(cplx_t<T> v) {
+  return make_double2(double(v.x), double(v.y));
+}
+
+// Gram partial sums: block (tile stride, b * kL + l3).
+template <typename T, bool MOM>
+__global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
+                                              const cplx_t<T>* __restrict__ z, int TPX, double* __restrict__ part,
+                                              double* __restrict__ part2) {
+  extern __shared__ double2 smem[];
+  const int K = (n + 1) * kL + 1;
+  const int E = K * (K + 1) / 2;
+  double2* Vs = smem;
+  short* ei = reinterpret_cast<short*>(Vs + size_t(TPX) * K);
+  short* ej = ei + E;
+  int bl = blockIdx.y;
+  int b = bl / kL, l3 = bl % kL;
+  int tid = threadIdx.x;
+  for (int e = tid; e < E; e += blockDim.x) {
+    int i = 0, rem = e;
+    while (rem >= K - i) { rem -= K - i; ++i; }
+    ei[e] = short(i);
+    ej[e] = short(i + rem);
+  }
+  const bool wide = E >= 256;
+  const int G = wide ? 1 : 256 / E;
+  const int grp = wide ? 0 : tid / E;
+  const bool act = wide || grp < G;
+  const int e0 = wide ? tid : tid % E;
+  double2 acc[kGramM];
+  double acc2[kGramM];
+#pragma unroll
+  for (int m = 0; m < kGramM; ++m) { acc[m] = make_double2(0.0, 0.0); acc2[m] = 0.0; }
+  __syncthreads();
+  int64_t ntiles = (N + TPX - 1) / TPX;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    int64_t p0 = tile * TPX;
+    int tn = int(min<int64_t>(TPX, N - p0));
+    for (int d = 0; d <= n; ++d) {
+      const cplx_t<T>* src = br + ((size_t(d) * B + b) * kL + l3) * N * kL + size_t(p0) * kL;
+      for (int idx = tid; idx < tn * kL; idx += blockDim.x) {
+        int p = idx / kL, l1 = idx % kL;
+        Vs[p * K + d * kL + l1] = to_d2<T>(src[idx]);
+      }
+    }
+    const cplx_t<T>* zs = z + (size_t(b) * kL + l3) * N + p0;
+    for (int p = tid; p < tn; p += blockDim.x) Vs[p * K + K - 1] = to_d2<T>(zs[p]);
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < kGramM; ++m) {
+        int e = e0 + 256 * m;
+        if ((wide && e < E) || (!wide && m == 0)) {
+          int i = ei[e], j = ej[e];
+          double2 s = acc[m];
+          double s2 = acc2[m];
+          for (int p = grp; p < tn; p += G) {
+            double2 a = Vs[p * K + i], c = Vs[p * K + j];
+            double re = a.x * c.x + a.y * c.y;
+            double im = a.y * c.x - a.x * c.y;
+            s.x += re;
+            s.y += im;
+            if (MOM) s2 += re * re + im * im;
+          }
+          acc[m] = s;
+          acc2[m] = s2;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  size_t base = (size_t(blockIdx.x) * gridDim.y + bl) * E;
+  if (wide) {
+#pragma unroll
+    for (int m = 0; m < kGramM; ++m) {
+      int e = e0 + 256 * m;
+      if (e < E) {
+        part[(base + e) * 2] = acc[m].x;
+        part[(base + e) * 2 + 1] = acc[m].y;
+        if (MOM) part2[base + e] = acc2[m];
+      }
+    }
+  } else {
+    double2* red = Vs;
+    double* red2 = reinterpret_cast<double*>(Vs + 256);
+    red[tid] = acc[0];
+    red2[tid] = acc2[0];
+    __syncthreads();
+    if (tid < E) {
+      double2 s = make_double2(0.0, 0.0);
+      double s2 = 0.0;
+      for (int g = 0; g < G; ++g) {
+        s.x += red[g * E + tid].x;
+        s.y += red[g * E + tid].y;
+        s2 += red2[g * E + tid];
+      }
+      part[(base + tid) * 2] = s.x;
+      part[(base + tid) * 2 + 1] = s.y;
+      if (MOM) part2[base + tid] = s2;
+    }
+  }
+}
+
+// Fixed-order block tree of 256 values (4 per thread).
+__device__ inline void block_sum4(double v[4], double* sh) {
+  int tid = threadIdx.x;
+  for (int c = 0; c < 4; ++c) sh[c * 256 + tid] = v[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w)
+      for (int c = 0; c < 4; ++c) sh[c * 256 + tid] += sh[c * 256 + tid + w];
+    __syncthreads();
+  }
+  for (int c = 0; c < 4; ++c) v[c] = sh[c * 256];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_zstats(int64_t N, const cplx_t<T>* __restrict__ z, double* __restrict__ part) {
+  __shared__ double sh[4 * 256];
+  int bl = blockIdx.y;
+  const cplx_t<T>* zs = z + size_t(bl) * N;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < N; p += int64_t(gridDim.x) * blockDim.x) {
+    double re = double(zs[p].x), im = double(zs[p].y);
+    double h = hypot(re, im);
+    v[0] += h;
+    v[1] += h * h;
+    v[2] += re * re + im * im;
+    v[3] += (h * h) * (h * h);
+  }
+  block_sum4(v, sh);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 4; ++c) part[(size_t(blockIdx.x) * gridDim.y + bl) * 4 + c] = v[c];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_meanvar(int64_t N, const T* __restrict__ x, double* __restrict__ part) {
+  __shared__ double sh[4 * 256];
+  int b = blockIdx.y;
+  const T* xs = x + size_t(b) * N;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < N; p += int64_t(gridDim.x) * blockDim.x) {
+    double a = double(xs[p]);
+    double sq = fabs(a) * fabs(a);
+    v[0] += a;
+    v[1] += sq;
+    v[2] += a * a;
+    v[3] += sq * sq;
+  }
+  block_sum4(v, sh);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 4; ++c) part[(size_t(blockIdx.x) * gridDim.y + b) * 4 + c] = v[c];
+}
+
+__global__ void k_reduce_parts(const double* __restrict__ part, int nblk, int64_t m, double* __restrict__ out) {
+  int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[size_t(b) * m + i];
+  out[i] = s;
+}
+
+template <typename T, int NL>
+__global__ __launch_bounds__(256) void k_gv(int64_t N, int B, const cplx_t<T>* __restrict__ br,
+                                            const cplx_t<T>* __restrict__ z, const double2* __restrict__ H,
+                                            cplx_t<T>* __restrict__ gbr, cplx_t<T>* __restrict__ gzp) {
+  constexpr int K = NL * kL + 1;
+  __shared__ double2 Hs[K * K];
+  int bl = blockIdx.y;
+  int b = bl / kL, l3 = bl % kL;
+  for (int e = threadIdx.x; e < K * K; e += blockDim.x) Hs[e] = H[size_t(bl) * K * K + e];
+  __syncthreads();
+  int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (p >= N) return;
+  double2 V[K];
+#pragma unroll
+  for (int d = 0; d < NL; ++d) {
+    const cplx_t<T>* src = br + (((size_t(d) * B + b) * kL + l3) * N + p) * kL;
+#pragma unroll
+    for (int l1 = 0; l1 < kL; ++l1) V[d * kL + l1] = to_d2<T>(src[l1]);
+  }
+  V[K - 1] = to_d2<T>(z[(size_t(b) * kL + l3) * N + p]);
+#pragma unroll 1
+  for (int i = 0; i < K; ++i) {
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      double2 h = Hs[i * K + j];
+      re += h.x * V[j].x - h.y * V[j].y;
+      im += h.x * V[j].y + h.y * V[j].x;
+    }
+    if (i < K - 1) {
+      int d = i / kL, l1 = i % kL;
+      gbr[(((size_t(d) * B + b) * kL + l3) * N + p) * kL + l1] = make_c<T>(T(re), T(im));
+    } else {
+      gzp[(size_t(b) * kL + l3) * N + p] = make_c<T>(T(re), T(im));
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gz(int n, int64_t N, int B, const cplx_t<T>* __restrict__ z,
+                                            const cplx_t<T>* __restrict__ gzp, const T* __restrict__ chain,
+                                            const double* __restrict__ s1, const double* __restrict__ s2,
+                                            cplx_t<T>* __restrict__ gz) {
+  int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (p >= N) return;
+  int bl = blockIdx.y;
+  int b = bl / kL, l = bl % kL;
+  size_t at = size_t(bl) * N + p;
+  double zr = double(z[at].x), zi = double(z[at].y);
+  double mag = hypot(zr, zi);
+  double den = mag == 0.0 ? 1.0 : mag;
+  double h = double(chain[((size_t(n) * B + b) * N + p) * kL + l]);
+  double a = s1[bl], c = s2[bl];
+  double re = double(gzp[at].x) + (a * zr) / den + c * zr + (h * zr) / den;
+  double im = double(gzp[at].y) + (a * zi) / den + c * zi + (h * zi) / den;
+  gz[at] = make_c<T>(T(re), T(im));
+}
+
+template <typename T>
+__global__ void k_meanvar_grad(int64_t N, const T* __restrict__ x, const double* __restrict__ gmean,
+                               const double* __restrict__ gvar, T* __restrict__ g0) {
+  int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (p >= N) return;
+  int b = blockIdx.y;
+  size_t at = size_t(b) * N + p;
+  g0[at] = T(double(g0[at]) + gmean[b] + gvar[b] * double(x[at]));
+}
+
+template <typename TO, typename TI>
+__global__ void k_cast(int64_t n, const TI* __restrict__ in, TO* __restrict__ out) {
+  int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < n) out[i] = TO(in[i]);
+}
+
+}  // namespace
+
+template <typename T>
+void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s) {
+  dim3 grid(nblocks(op.R * 4, 256), rows);
+  if (NI == 1) k_conv_fwd<T, 1><<<grid, 256, 0, s>>>(op, x, y, mod);
+  else if (NI == kL) k_conv_fwd<T, kL><<<grid, 256, 0, s>>>(op, x, y, nullptr);
+  else throw std::runtime_error("conv: unsupported inner width");
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
+  dim3 grid(nblocks(op.R * 4, 256), rows);
+  if (NI == 1) k_conv_adj<T, 1><<<grid, 256, 0, s>>>(op, gy, gx);
+  else if (NI == kL) k_conv_adj<T, kL><<<grid, 256, 0, s>>>(op, gy, gx);
+  else throw std::runtime_error("conv adjoint: unsupported inner width");
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_dec_fwd(const DecOp<T>& op, int NI, const T* x, int rows, T* y, hipStream_t s) {
+  dim3 grid(nblocks(op.Rc * 4, 256), rows);
+  if (NI == 1) k_dec_fwd<T, 1><<<grid, 256, 0, s>>>(op, x, y);
+  else if (NI == kL) k_dec_fwd<T, kL><<<grid, 256, 0, s>>>(op, x, y);
+  else throw std::runtime_error("decimation: unsupported inner width");
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_dec_adj(const DecOp<T>& op, int NI, const T* gy, const T* add, int rows, T* gx, hipStream_t s) {
+  dim3 grid(nblocks(op.Rf * 4, 256), rows);
+  if (NI == 1) k_dec_adj<T, 1><<<grid, 256, 0, s>>>(op, gy, add, gx);
+  else if (NI == kL) k_dec_adj<T, kL><<<grid, 256, 0, s>>>(op, gy, add, gx);
+  else throw std::runtime_error("decimation adjoint: unsupported inner width");
+  STL_HIP(hipGetLastError());
+}
+
+int gram_tile(int n) {
+  int K = (n + 1) * kL + 1;
+  int tpx = 24576 / (16 * K);
+  return std::max(16, std::min(256, tpx));
+}
+
+size_t gram_lds(int n) {
+  int K = (n + 1) * kL + 1;
+  int E = K * (K + 1) / 2;
+  size_t v = size_t(gram_tile(n)) * K * sizeof(double2);
+  size_t red = 256 * (sizeof(double2) + sizeof(double));
+  return std::max(v, red) + 2 * size_t(E) * sizeof(short) + 16;
+}
+
+template <typename T>
+void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, bool moments, int nblk,
+                 double* part, double* part2, hipStream_t s) {
+  int K = (n + 1) * kL + 1;
+  STL_CHECK(K * (K + 1) / 2 <= 256 * kGramM, "gram: too many levels");
+  dim3 grid(nblk, B * kL);
+  size_t lds = gram_lds(n);
+  int tpx = gram_tile(n);
+  if (moments) k_gram<T, true><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
+  else k_gram<T, false><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_zstats(int64_t N, int B, const cplx_t<T>* z, int nblk, double* part, hipStream_t s) {
+  k_zstats<T><<<dim3(nblk, B * kL), 256, 0, s>>>(N, z, part);
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_meanvar(int64_t N, int B, const T* x, int nblk, double* part, hipStream_t s) {
+  k_meanvar<T><<<dim3(nblk, B), 256, 0, s>>>(N, x, part);
+  STL_HIP(hipGetLastError());
+}
+
+void launch_reduce_parts(const double* part, int nblk, int64_t m, double* out, hipStream_t s) {
+  k_reduce_parts<<<nblocks(m, 256), 256, 0, s>>>(part, nblk, m, out);
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_gv(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, const double2* H,
+               cplx_t<T>* gbranch, cplx_t<T>* gz_part, hipStream_t s) {
+  dim3 grid(nblocks(N, 256), B * kL);
+  switch (n + 1) {
+#define STL_GV(NL) \
+  case NL: k_gv<T, NL><<<grid, 256, 0, s>>>(N, B, branch, z, H, gbranch, gz_part); break;
+    STL_GV(1) STL_GV(2) STL_GV(3) STL_GV(4) STL_GV(5) STL_GV(6) STL_GV(7) STL_GV(8) STL_GV(9) STL_GV(10)
+    STL_GV(11) STL_GV(12)
+#undef STL_GV
+    default: throw std::runtime_error("gv: too many levels");
+  }
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_gz(int n, int64_t N, int B, const cplx_t<T>* z, const cplx_t<T>* gz_part, const T* chain,
+               const double* s1, const double* s2, cplx_t<T>* gz, hipStream_t s) {
+  k_gz<T><<<dim3(nblocks(N, 256), B * kL), 256, 0, s>>>(n, N, B, z, gz_part, chain, s1, s2, gz);
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, T* g0, hipStream_t s) {
+  k_meanvar_grad<T><<<dim3(nblocks(N, 256), B), 256, 0, s>>>(N, x, gmean, gvar, g0);
+  STL_HIP(hipGetLastError());
+}
+
+template <typename TO, typename TI>
+void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s) {
+  k_cast<TO, TI><<<nblocks(n, 256), 256, 0, s>>>(n, in, out);
+  STL_HIP(hipGetLastError());
+}
+
+#define STL_INSTANTIATE(T)                                                                                     \
+  template void launch_conv_fwd<T>(const ConvOp<T>&, int, const T*, int, cplx_t<T>*, T*, hipStream_t);        \
+  template void launch_conv_adj<T>(const ConvOp<T>&, int, const cplx_t<T>*, int, T*, hipStream_t);            \
+  template void launch_dec_fwd<T>(const DecOp<T>&, int, const T*, int, T*, hipStream_t);                      \
+  template void launch_dec_adj<T>(const DecOp<T>&, int, const T*, const T*, int, T*, hipStream_t);            \
+  template void launch_gram<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, bool, int, double*,     \
+                               double*, hipStream_t);                                                         \
+  template void launch_zstats<T>(int64_t, int, const cplx_t<T>*, int, double*, hipStream_t);                  \
+  template void launch_meanvar<T>(int64_t, int, const T*, int, double*, hipStream_t);                         \
+  template void launch_gv<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, const double2*,           \
+                             cplx_t<T>*, cplx_t<T>*, hipStream_t);                                            \
+  template void launch_gz<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, const T*, const double*,  \
+                             const double*, cplx_t<T>*, hipStream_t);                                         \
+  template void launch_meanvar_grad<T>(int64_t, int, const T*, const double*, const double*, T*, hipStream_t);
+
+STL_INSTANTIATE(double)
+STL_INSTANTIATE(float)
+#undef STL_INSTANTIATE
+
+template void launch_cast<double, float>(int64_t, const float*, double*, hipStream_t);
+template void launch_cast<float, double>(int64_t, const double*, float*, hipStream_t);
+
+}  // namespace stl

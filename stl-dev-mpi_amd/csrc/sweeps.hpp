@@ -1,0 +1,79 @@
+// Kernels of the forward sweep and its VJP on the pyramid (templated on the
+// bulk precision T = double or float; sky-wide sums always in FP64).
+//
+// Field layouts (row-major, N pixels of the level, NESTED order):
+//   map      [B][N]                    real
+//   z        [B][kL][N]                complex (z of gauge l at pixel p)
+//   modulus  [(d * B + b)][N][kL]      real, d = 0..n (row d = n holds |z_n|)
+//   branch   [(d * B + b)][kL(l3)][N][kL(l1)]  complex: second wavelet of gauge
+//            l3 of the modulus of orientation l1
+// A convolution maps real rows x [R][N][NI] to complex y [R][kL][N][NI]; a
+// decimation maps [R][N_f][NI] to [R][N_c][NI].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace stl {
+
+template <typename T>
+struct Complex2;
+template <>
+struct Complex2<double> {
+  using type = double2;
+};
+template <>
+struct Complex2<float> {
+  using type = float2;
+};
+template <typename T>
+using cplx_t = typename Complex2<T>::type;
+
+template <typename T>
+void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* modulus_out,
+                     hipStream_t s);
+template <typename T>
+void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s);
+template <typename T>
+void launch_dec_fwd(const DecOp<T>& op, int NI, const T* x, int rows, T* y, hipStream_t s);
+template <typename T>
+void launch_dec_adj(const DecOp<T>& op, int NI, const T* gy, const T* add, int rows, T* gx, hipStream_t s);
+
+// Pixels per LDS tile of the Gram kernel at level n.
+int gram_tile(int n);
+
+// Per-block partial sums of the Gram products V_i conj(V_j), i <= j, of
+// V = [branch(d)[l1][l3] for d <= n, l1 < kL; z[l3]] per (b, l3):
+// part [nblk][B * kL][E][2] (+ [E] second moments |.|^2 when moments).
+template <typename T>
+void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, bool moments, int nblk,
+                 double* part, double* part2, hipStream_t s);
+// S1/S2 sums of z: part [nblk][B * kL][4] = (sum |z|, sum |z|^2, sum re^2+im^2, sum |z|^4).
+template <typename T>
+void launch_zstats(int64_t N, int B, const cplx_t<T>* z, int nblk, double* part, hipStream_t s);
+// mean/var sums of the level-0 map: part [nblk][B][4] = (sum x, sum x^2, sum x^2, sum x^4).
+template <typename T>
+void launch_meanvar(int64_t N, int B, const T* x, int nblk, double* part, hipStream_t s);
+// Sums the partials over blocks in block order: part [nblk][m] -> out [m].
+void launch_reduce_parts(const double* part, int nblk, int64_t m, double* out, hipStream_t s);
+
+// Reverse of the Gram products: gV = H V per pixel and (b, l3), H [B * kL][K][K]
+// complex (double2); writes the branch cotangents [(d B + b)][l3][N][l1] and
+// the z part [B][kL][N].
+template <typename T>
+void launch_gv(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, const double2* H,
+               cplx_t<T>* gbranch, cplx_t<T>* gz_part, hipStream_t s);
+// gz = gz_part + s1[b,l] z/|z| + s2[b,l] z + (h z/|z|) with h = modulus
+// cotangent row d = n of the chain stack [(d B + b)][N][kL].
+template <typename T>
+void launch_gz(int n, int64_t N, int B, const cplx_t<T>* z, const cplx_t<T>* gz_part, const T* chain,
+               const double* s1, const double* s2, cplx_t<T>* gz, hipStream_t s);
+// g0 [B][N] += gmean[b] + gvar[b] x.
+template <typename T>
+void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, T* g0, hipStream_t s);
+// Element-wise precision casts (maps in, gradients out).
+template <typename TO, typename TI>
+void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s);
+
+}  // namespace stl

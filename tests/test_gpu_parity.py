@@ -1,0 +1,253 @@
+"""Parity of the HIP engine (through the C ABI) with the FP64 oracle and the
+reference's stats fixtures.
+
+Tolerances: the FP64 GPU path must match the oracle to 1e-10 relative
+(statistics: max |dS| / max |S| per family; gradient: max |dg| / max |g|);
+the reference's own fixture tolerances (stats.rs:183-201) for the fixtures.
+The mixed path (FP32 bulk, FP64 sums) is held to 1e-5 per family and 1e-4
+on the gradient here; its measured errors are reported by the benchmark.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as O
+from fixtures import cases
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+FAMILY_NAMES = ["mean", "var", "S1", "S2", "S3", "S4"]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+_ENGINES = {}
+
+
+def engine(nside, J, batch=1, precision="fp64"):
+    from stl_port import NativeEngine
+    key = (nside, J, batch, precision)
+    if key not in _ENGINES:
+        print(f"[gpu] NativeEngine nside={nside} J={J} batch={batch} precision={precision}", flush=True)
+        _ENGINES[key] = NativeEngine(nside, J, batch=batch, backend="gpu", precision=precision)
+    return _ENGINES[key]
+
+
+def family_errors(values, ref, plan):
+    """max |dS| / max |S| per family; the mean is scaled by the map RMS sqrt(var)
+    (a mean-subtracted map has a mean of rounding size)."""
+    fam = np.array([FAMILY_NAMES.index(c["family"]) for c in plan])
+    out = {}
+    for f, name in enumerate(FAMILY_NAMES):
+        sel = fam == f
+        if sel.any():
+            scale = np.abs(ref[..., sel]).max()
+            if name == "mean":
+                scale = np.sqrt(np.abs(ref[..., fam == 1]).max())
+            out[name] = float(np.abs(values[..., sel] - ref[..., sel]).max() / max(scale, 1e-300))
+    return out
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("nside", [2, 4, 8, 16, 64])
+def test_operators_match_oracle(nside):
+    J = int(np.log2(nside)) + 1
+    eng = engine(nside, J)
+    ref = O.Engine(nside, J)
+    rng = np.random.default_rng(nside)
+    for n in range(J):
+        N = 12 * (nside >> n) ** 2
+        print(f"[gpu] operators nside={nside} level={n}", flush=True)
+        x = rng.standard_normal((2, N))
+        y = torch.empty((2, 4, N, 2), dtype=torch.float64, device="cuda")
+        eng.apply_operator(0, n, torch.from_numpy(x).cuda(), y)
+        yc = y.cpu().numpy()
+        yc = yc[..., 0] + 1j * yc[..., 1]
+        assert rel(yc, ref.conv(n, x)) < 1e-12
+        gy = rng.standard_normal((2, 4, N)) + 1j * rng.standard_normal((2, 4, N))
+        gyt = torch.from_numpy(np.stack([gy.real, gy.imag], -1).copy()).cuda()
+        gx = torch.empty((2, N), dtype=torch.float64, device="cuda")
+        eng.apply_operator(1, n, gyt, gx)
+        assert rel(gx.cpu().numpy(), ref.conv_adjoint(n, gy)) < 1e-12
+        if n + 1 < J:
+            ns = nside >> n
+            indptr, cols, vals = O.decimation_rows(ns, 0, 3 * ns * ns)
+            D = sp.csr_matrix((vals, cols, indptr), shape=(3 * ns * ns, N))
+            yd = torch.empty((2, N // 4), dtype=torch.float64, device="cuda")
+            eng.apply_operator(2, n, torch.from_numpy(x).cuda(), yd)
+            assert rel(yd.cpu().numpy(), (D @ x.T).T) < 1e-13
+            g = rng.standard_normal((2, N // 4))
+            gd = torch.empty((2, N), dtype=torch.float64, device="cuda")
+            eng.apply_operator(3, n, torch.from_numpy(g).cuda(), gd)
+            assert rel(gd.cpu().numpy(), (D.T @ g.T).T) < 1e-13
+
+
+@pytest.mark.parametrize("case", [c for c in cases("stats", "stats")
+                                  if c["L"] == 4 and c["kernel_sz"] == 5 and c["channels"] == 1
+                                  and not c["fewer"] and c["j_to_dg"] == list(range(c["J"]))],
+                         ids=lambda c: c.name)
+def test_stats_fixtures(case):
+    """The reference's fixtures of the hot-path configuration, through the C ABI."""
+    eng = engine(case["nside"], case["J"], case["batch"])
+    x = case.arr("map0").reshape(case["batch"], -1)
+    print(f"[gpu] fixture {case.name}", flush=True)
+    values, _ = eng.forward(x)
+    ref = case.complex("values").reshape(values.shape)
+    flat = lambda z: np.concatenate([z.real.ravel(), z.imag.ravel()])
+    assert np.max(np.abs(flat(values) - flat(ref)) / (1e-11 + 1e-10 * np.abs(flat(ref)))) <= 1.0
+    grad = eng.vjp(case.complex("gradient").reshape(values.shape))
+    refg = case.arr("band_gradient0").reshape(grad.shape)
+    assert np.max(np.abs(grad - refg) / (1e-10 + 1e-9 * np.abs(refg))) <= 1.0
+
+
+@pytest.mark.parametrize("nside, J", [(32, 4), (64, 5)])
+def test_forward_vjp_vs_oracle(nside, J):
+    """Random N(0,1) maps (seed 0), cotangent 2 (S(x) - S(y)) (the plain-method workload)."""
+    eng = engine(nside, J)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(12 * nside * nside)
+    y = np.random.default_rng(1).standard_normal(12 * nside * nside)
+    ref = O.Engine(nside, J)
+    ref.load(y)
+    sy = ref.forward()
+    ref.load(x)
+    sx = ref.forward()
+    cot = 2.0 * (sx - sy)
+    g_ref = ref.gradient(cot)
+    print(f"[gpu] forward+vjp nside={nside} J={J}", flush=True)
+    values, _ = eng.forward(x, generation=5)
+    errs = family_errors(values, sx, eng.plan)
+    assert max(errs.values()) < 1e-10, errs
+    grad = eng.vjp(cot, generation=5)
+    assert rel(grad, g_ref) < 1e-10
+
+
+def test_mixed_precision_vs_oracle():
+    nside, J = 64, 5
+    eng = engine(nside, J, precision="mixed")
+    g = np.random.default_rng(7).standard_normal(12 * 16 * 16)
+    x = np.repeat(g, 16) + 0.1 * np.random.default_rng(8).standard_normal(12 * nside * nside)
+    x = np.exp(0.5 * x)
+    x -= x.mean()
+    ref = O.Engine(nside, J)
+    ref.load(x)
+    sx = ref.forward()
+    cot = np.random.default_rng(9).standard_normal(sx.shape) * np.abs(sx)
+    g_ref = ref.gradient(cot)
+    print("[gpu] mixed precision nside=64", flush=True)
+    values, _ = eng.forward(x)
+    errs = family_errors(values, sx, eng.plan)
+    print("mixed errors per family", errs)
+    assert max(errs.values()) < 1e-5, errs
+    grad = eng.vjp(cot)
+    gerr = rel(grad, g_ref)
+    print("mixed gradient error", gerr)
+    assert gerr < 1e-4
+
+
+def test_batch_equals_single_maps():
+    nside, J = 16, 3
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((2, 12 * nside * nside))
+    e2 = engine(nside, J, batch=2)
+    e1 = engine(nside, J, batch=1)
+    v2, _ = e2.forward(x)
+    cot = rng.standard_normal(v2.shape) + 1j * rng.standard_normal(v2.shape)
+    g2 = e2.vjp(cot)
+    for b in range(2):
+        v1, _ = e1.forward(x[b])
+        assert rel(v2[b], v1[0]) < 1e-14
+        g1 = e1.vjp(cot[b])
+        assert rel(g2[b], g1[0]) < 1e-14
+
+
+def test_moments_match_oracle():
+    nside, J = 16, 3
+    eng = engine(nside, J)
+    x = np.random.default_rng(4).standard_normal(12 * nside * nside)
+    ref = O.Engine(nside, J)
+    ref.load(x)
+    sx, second = ref.forward(moments=True)
+    var_ref = np.maximum(second - np.abs(sx) ** 2, 0.0)
+    values, var = eng.forward(x, moments=True)
+    assert rel(values, sx) < 1e-12
+    assert np.abs(var - var_ref).max() <= 1e-10 * np.abs(var_ref).max()
+
+
+def test_identity_is_checked():
+    nside, J = 16, 3
+    eng = engine(nside, J)
+    x = np.random.default_rng(5).standard_normal(12 * nside * nside)
+    values, _ = eng.forward(x, generation=11, member=2, objective="a")
+    with pytest.raises(RuntimeError):
+        eng.vjp(values, generation=12, member=2, objective="a")
+    with pytest.raises(RuntimeError):
+        eng.vjp(values, generation=11, member=2, objective="b")
+    lib = eng._lib
+    key = (11, 3, 0)
+    out = np.empty(eng.npix)
+    c = np.ascontiguousarray(values.astype(np.complex128))
+    assert lib.stlp_vjp(eng._handle, c.ctypes.data_as(lib.stlp_vjp.argtypes[1]), c.size, *key,
+                        out.ctypes.data_as(lib.stlp_vjp.argtypes[1])) == -1
+    assert "stale" in lib.stlp_last_error().decode()
+    eng.vjp(values, generation=11, member=2, objective="a")
+
+
+def test_checkpoint_restore():
+    nside, J = 16, 3
+    eng = engine(nside, J)
+    rng = np.random.default_rng(6)
+    x, y = rng.standard_normal((2, 12 * nside * nside))
+    vx, _ = eng.forward(x, generation=1, objective="x")
+    snap = eng.checkpoint()
+    assert snap.nbytes() > 0
+    vy, _ = eng.forward(y, generation=1, objective="y")
+    gy = eng.vjp(vy, generation=1, objective="y")
+    eng.restore(snap, generation=1, objective="x")
+    gx = eng.vjp(vx, generation=1, objective="x")
+    vx2, _ = eng.forward(x, generation=2, objective="x")
+    assert np.array_equal(vx, vx2)
+    assert np.array_equal(eng.vjp(vx2, generation=2, objective="x"), gx)
+    with pytest.raises(RuntimeError):
+        eng.restore(snap, generation=3, objective="x")
+    snap.close()
+    assert not np.array_equal(gx, gy)
+
+
+def test_torch_autograd_matches_vjp_and_finite_differences():
+    """torch.autograd through the engine equals vjp; central differences of the
+    pairing Re sum(conj(c) S(x)) along unit directions (eps 1e-4) match the
+    directional derivative (the reference's stats.rs:77-110 check)."""
+    from stl_port.torch_fn import scattering
+    nside, J = 16, 3
+    eng = engine(nside, J)
+    rng = np.random.default_rng(10)
+    x0 = torch.from_numpy(rng.standard_normal((1, 12 * nside * nside))).cuda()
+    c = rng.standard_normal(eng.ncoeff) + 1j * rng.standard_normal(eng.ncoeff)
+    ct = torch.from_numpy(c).cuda()
+
+    def pairing(x):
+        return (torch.conj(ct) * scattering(eng, x)).real.sum()
+
+    x = x0.clone().requires_grad_(True)
+    pairing(x).backward()
+    g = x.grad.detach()
+    eng.forward(x0.cpu().numpy(), generation=99)
+    assert rel(g.cpu().numpy(), eng.vjp(c, generation=99)) < 1e-13
+    eps = 1e-4
+    for _ in range(4):
+        v = torch.from_numpy(rng.standard_normal(x0.shape)).cuda()
+        v /= torch.linalg.norm(v)
+        with torch.no_grad():
+            fd = (pairing(x0 + eps * v) - pairing(x0 - eps * v)).item() / (2 * eps)
+        exact = float((g * v).sum())
+        assert abs(fd - exact) <= 1e-6 * abs(exact), (fd, exact)

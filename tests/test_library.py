@@ -1,0 +1,79 @@
+"""The C-ABI library loads, exports every symbol include/stl_bridge.h declares,
+and reports configuration errors through stlp_last_error (no GPU needed)."""
+import ctypes as C
+import json
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from stl_port import native
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "stl_bridge.h"
+
+
+def header_symbols():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(stl[px]_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(native.library_path())
+    names = header_symbols()
+    assert len(names) >= 55
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_table_covers_the_reference_binding():
+    # the reference binding (native.py:55-108) binds these eagerly; all must resolve
+    lib = native._library()
+    for name in native.SIGNATURES:
+        assert hasattr(lib, name), name
+
+
+@pytest.mark.parametrize("config, message", [
+    ("{", "valid JSON"),
+    ('{"nside": 12}', "power of two"),
+    ('{"nside": 16, "j": 9}', "invalid number of scales"),
+    ('{"nside": 16, "j": 3, "kernel": 3}', "kernel"),
+    ('{"nside": 16, "j": 3, "orientations": 2}', "orientations"),
+    ('{"nside": 16, "j": 3, "backend": "cpu"}', "backend cpu"),
+    ('{"nside": 16, "j": 3, "backend": "tpu"}', "backend must be"),
+    ('{"nside": 16, "j": 3, "cross": true}', "cross"),
+    ('{"nside": 16, "j": 3, "precision": "fp16"}', "precision"),
+    ('{"nside": 16, "j": 3, "batch": 0}', "batch"),
+])
+def test_bad_configuration_is_an_error(config, message):
+    lib = native._library()
+    assert not lib.stlp_new(config.encode())
+    assert message in lib.stlp_last_error().decode()
+
+
+def test_other_products_are_not_implemented():
+    lib = native._library()
+    assert not lib.stlp_patch_new(b"{}")
+    assert "not implemented" in lib.stlp_last_error().decode()
+    assert lib.stlp_unet_zero_grad(None) == -1
+    assert lib.stlp_spin_fields(None, None, 0, 0, None, None) == -1
+
+
+def test_identity_key_matches_the_reference_hash():
+    # native.py:122-129: first 8 bytes of sha256(objective), little endian
+    import hashlib
+    g, m, o = native._key(3, 7, "gaussian")
+    assert (g, m) == (3, 7)
+    assert o == int.from_bytes(hashlib.sha256(b"gaussian").digest()[:8], "little")
+    with pytest.raises(ValueError):
+        native._key(-1, 0, "x")
+    with pytest.raises(ValueError):
+        native._key(0, 0, "")
+
+
+def test_lbfgs_failure_value_without_callback_errors():
+    lib = native._library()
+    x = np.zeros(4)
+    assert lib.stlp_lbfgs(native._pointer(x), 4, b"{bad", native._CALLBACK(lambda *a: 0.0), None) == -1
