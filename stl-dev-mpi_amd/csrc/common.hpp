@@ -74,6 +74,18 @@ __host__ __device__ inline int64_t rep_pixel(int64_t r, int order) {
 // that consecutive rep rows read consecutive memory for one slot).
 // ---------------------------------------------------------------------------
 
+// Half-width of the stencil window in the extended face frame: every cell read
+// by a row, and every row reading a cell, lies within max-norm kWin of it at
+// nside >= 8 (kernel 5, L = 4).
+constexpr int kWin = 5;
+
+// Offset (dx, dy) packed as two int8 in an int16.
+__host__ __device__ inline int16_t pack_off(int dx, int dy) {
+  return int16_t((dx & 0xFF) | ((dy & 0xFF) << 8));
+}
+__host__ __device__ inline int off_dx(int16_t o) { return int(int8_t(o & 0xFF)); }
+__host__ __device__ inline int off_dy(int16_t o) { return int(int8_t((o >> 8) & 0xFF)); }
+
 // Wavelet convolution of one level, complex outputs of kL gauges.
 template <typename T>
 struct ConvOp {
@@ -82,12 +94,15 @@ struct ConvOp {
   int64_t R = 0;          // rep rows = 3 npf
   int S = 0;              // slot capacity of the forward rows
   int Tn = 0;             // slot capacity of the transposed rows
+  bool tiled = false;     // every offset within kWin and nside >= 8: LDS-tiled kernels
   const uint8_t* cnt = nullptr;   // [R] slots per rep row
   const int32_t* cell = nullptr;  // [S][R] NESTED cell read (rep row frame)
+  const int16_t* off = nullptr;   // [S][R] offset of that cell in the row's extended face frame
   const T* coef = nullptr;        // [S][R][kL][2]
   const uint8_t* tcnt = nullptr;  // [R] rows reading each rep cell
   const int32_t* trow = nullptr;  // [Tn][R] NESTED row id (any face), ascending
-  const T* tcoef = nullptr;       // [Tn][R][kL][2] that row's coefficient (its own gauges)
+  const int16_t* toff = nullptr;  // [Tn][R] offset of that row in the cell's extended face frame
+  const T* tcoef = nullptr;       // [kL][Tn][R][2] that row's coefficient (its own gauges), gauge planes
 };
 
 // Decimation from level n (fine) to level n + 1 (coarse), real weights.

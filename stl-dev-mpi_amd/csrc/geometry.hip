@@ -89,11 +89,12 @@ __global__ void k_taps(int order, int64_t row0, int64_t nrows, const double* __r
 // One wavefront per rep row: distinct cells with a nonzero weight, sorted, and
 // their folded coefficients.  WRITE = false only counts the slots.
 template <bool WRITE>
-__global__ __launch_bounds__(256) void k_fold(int64_t row0, int64_t nrows, int64_t R,
+__global__ __launch_bounds__(256) void k_fold(int order, int64_t row0, int64_t nrows, int64_t R,
                                               const int32_t* __restrict__ tcell, const double* __restrict__ tw,
                                               const double* __restrict__ ker, uint8_t* __restrict__ cnt,
-                                              int32_t* __restrict__ cell, double* __restrict__ coef64,
-                                              float* __restrict__ coef32, int* __restrict__ err) {
+                                              int32_t* __restrict__ cell, int16_t* __restrict__ off,
+                                              double* __restrict__ coef64, float* __restrict__ coef32,
+                                              int* __restrict__ err, int* __restrict__ untiled) {
   __shared__ int32_t ec[4][kEntries];
   __shared__ double ew[4][kEntries];
   __shared__ int32_t slots[4][kMaxSlots];
@@ -153,6 +154,13 @@ __global__ __launch_bounds__(256) void k_fold(int64_t row0, int64_t nrows, int64
   }
   size_t at = size_t(lane) * R + r;
   cell[at] = c;
+  int dx = 0, dy = 0;
+  bool ok = hp::rel_offset(rep_pixel(r, order), c, order, dx, dy);
+  if (!ok || abs(dx) > kWin || abs(dy) > kWin) {
+    atomicOr(untiled, 1);
+    dx = dy = 0;
+  }
+  off[at] = pack_off(dx, dy);
   for (int g = 0; g < kL; ++g)
     for (int o = 0; o < 2; ++o) {
       if (coef64) coef64[(at * kL + g) * 2 + o] = acc[g][o];
@@ -312,6 +320,72 @@ __global__ void k_tfill(int64_t R, int S, const uint8_t* __restrict__ cnt, const
   }
 }
 
+// Convolution transpose: row ids, their offsets in the cell's frame, and the
+// coefficients in gauge planes [kL][Tn][Rq][2].
+template <typename T>
+__global__ void k_conv_tfill(int64_t R, int S, const uint8_t* __restrict__ cnt, const int32_t* __restrict__ cell,
+                             const T* __restrict__ coef, int order, int Tn, int* __restrict__ fill,
+                             int32_t* __restrict__ trow, int16_t* __restrict__ toff, T* __restrict__ tcoef,
+                             int* __restrict__ untiled) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(S) * R) return;
+  int s = int(t / R);
+  int64_t r = t % R;
+  if (s >= cnt[r]) return;
+  int k0;
+  int64_t qr;
+  entry_target(cell[t], order, k0, qr);
+  int slot = atomicAdd(&fill[qr], 1);
+  size_t at = size_t(slot) * R + qr;
+  int64_t p0 = rot_pix(rep_pixel(r, order), k0, order);
+  int64_t q = rot_pix(cell[t], k0, order);
+  trow[at] = int32_t(p0);
+  int dx = 0, dy = 0;
+  bool ok = hp::rel_offset(q, p0, order, dx, dy);
+  if (!ok || abs(dx) > kWin || abs(dy) > kWin) {
+    atomicOr(untiled, 1);
+    dx = dy = 0;
+  }
+  toff[at] = pack_off(dx, dy);
+  for (int g = 0; g < kL; ++g) {
+    int gs = gauge_src(k0, g);
+    T re = coef[(size_t(t) * kL + gs) * 2];
+    T im = coef[(size_t(t) * kL + gs) * 2 + 1];
+    size_t o = ((size_t(g) * Tn + slot) * R + qr) * 2;
+    tcoef[o] = re;
+    tcoef[o + 1] = gauge_conj(k0, g) ? -im : im;
+  }
+}
+
+template <typename T>
+__global__ void k_conv_tsort(int64_t R, int Tn, const int* __restrict__ tc, uint8_t* __restrict__ tcnt,
+                             int32_t* __restrict__ trow, int16_t* __restrict__ toff, T* __restrict__ tcoef) {
+  int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (q >= R) return;
+  int n = tc[q];
+  tcnt[q] = uint8_t(n);
+  auto cidx = [&](int g, int slot, int o) { return ((size_t(g) * Tn + slot) * R + q) * 2 + o; };
+  for (int a = 1; a < n; ++a) {
+    int32_t key = trow[size_t(a) * R + q];
+    int16_t ko = toff[size_t(a) * R + q];
+    T v[2 * kL];
+    for (int g = 0; g < kL; ++g)
+      for (int o = 0; o < 2; ++o) v[g * 2 + o] = tcoef[cidx(g, a, o)];
+    int b = a - 1;
+    while (b >= 0 && trow[size_t(b) * R + q] > key) {
+      trow[size_t(b + 1) * R + q] = trow[size_t(b) * R + q];
+      toff[size_t(b + 1) * R + q] = toff[size_t(b) * R + q];
+      for (int g = 0; g < kL; ++g)
+        for (int o = 0; o < 2; ++o) tcoef[cidx(g, b + 1, o)] = tcoef[cidx(g, b, o)];
+      --b;
+    }
+    trow[size_t(b + 1) * R + q] = key;
+    toff[size_t(b + 1) * R + q] = ko;
+    for (int g = 0; g < kL; ++g)
+      for (int o = 0; o < 2; ++o) tcoef[cidx(g, b + 1, o)] = v[g * 2 + o];
+  }
+}
+
 template <typename T, int C>
 __global__ void k_tsort(int64_t Rq, const int* __restrict__ tc, uint8_t* __restrict__ tcnt,
                         int32_t* __restrict__ trow, T* __restrict__ tcoef) {
@@ -391,6 +465,36 @@ void build_transpose(int64_t R, int S, const DevArray<uint8_t>& cnt, const DevAr
   STL_HIP(hipStreamSynchronize(s));
 }
 
+template <typename T>
+void build_conv_transpose(int order, int64_t R, int S, const DevArray<uint8_t>& cnt, const DevArray<int32_t>& cell,
+                          const DevArray<T>& coef, ConvStore<T>& st, DevArray<int>& untiled, hipStream_t s) {
+  DevArray<int> tc(R);
+  tc.zero(s);
+  int64_t total = int64_t(S) * R;
+  k_tcount<<<blocks_for(total, 256), 256, 0, s>>>(R, S, cnt.data(), cell.data(), order, tc.data());
+  STL_HIP(hipGetLastError());
+  int Tn = device_max_i32(tc.data(), R, s);
+  STL_CHECK(Tn <= 255, "transposed operator: too many rows per cell");
+  st.Tn = Tn;
+  st.trow.resize(size_t(Tn) * R);
+  st.toff.resize(size_t(Tn) * R);
+  st.tcoef.resize(size_t(kL) * Tn * R * 2);
+  st.trow.zero(s);
+  st.toff.zero(s);
+  st.tcoef.zero(s);
+  st.tcnt.resize(R);
+  DevArray<int> fill(R);
+  fill.zero(s);
+  k_conv_tfill<T><<<blocks_for(total, 256), 256, 0, s>>>(R, S, cnt.data(), cell.data(), coef.data(), order, Tn,
+                                                         fill.data(), st.trow.data(), st.toff.data(), st.tcoef.data(),
+                                                         untiled.data());
+  STL_HIP(hipGetLastError());
+  k_conv_tsort<T><<<blocks_for(R, 128), 128, 0, s>>>(R, Tn, tc.data(), st.tcnt.data(), st.trow.data(),
+                                                      st.toff.data(), st.tcoef.data());
+  STL_HIP(hipGetLastError());
+  STL_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace
 
 std::vector<double> wavelet_kernel_f32(int ksz) {
@@ -435,63 +539,78 @@ void build_conv(int order, const std::vector<double>& ker, ConvStore<double>* c6
   DevArray<int32_t> tcell(size_t(chunk) * kEntries);
   DevArray<double> tw(size_t(chunk) * kEntries);
   DevArray<uint8_t> cnt(R);
-  DevArray<int> err(1);
+  DevArray<int> err(1), untiled(1);
   err.zero(s);
+  untiled.zero(s);
   // Pass 1: slot counts.
   for (int64_t r0 = 0; r0 < R; r0 += chunk) {
     int64_t nr = std::min(chunk, R - r0);
     k_taps<<<blocks_for(nr * kL, 128), 128, 0, s>>>(order, r0, nr, gridvec.data(), tcell.data(), tw.data());
-    k_fold<false><<<blocks_for(nr, 4), 256, 0, s>>>(r0, nr, R, tcell.data(), tw.data(), dker.data(), cnt.data(),
-                                                    nullptr, nullptr, nullptr, err.data());
+    k_fold<false><<<blocks_for(nr, 4), 256, 0, s>>>(order, r0, nr, R, tcell.data(), tw.data(), dker.data(),
+                                                    cnt.data(), nullptr, nullptr, nullptr, nullptr, err.data(),
+                                                    untiled.data());
     STL_HIP(hipGetLastError());
   }
   check_err(err, s, "convolution geometry");
   int S = device_max_u8(cnt.data(), R, s);
   DevArray<int32_t> cell(size_t(S) * R);
+  DevArray<int16_t> off(size_t(S) * R);
   DevArray<double> coef64;
   DevArray<float> coef32;
   cell.zero(s);
+  off.zero(s);
   if (c64) { coef64.resize(size_t(S) * R * 2 * kL); coef64.zero(s); }
   if (c32) { coef32.resize(size_t(S) * R * 2 * kL); coef32.zero(s); }
   // Pass 2: cells and coefficients.
   for (int64_t r0 = 0; r0 < R; r0 += chunk) {
     int64_t nr = std::min(chunk, R - r0);
     k_taps<<<blocks_for(nr * kL, 128), 128, 0, s>>>(order, r0, nr, gridvec.data(), tcell.data(), tw.data());
-    k_fold<true><<<blocks_for(nr, 4), 256, 0, s>>>(r0, nr, R, tcell.data(), tw.data(), dker.data(), cnt.data(),
-                                                   cell.data(), c64 ? coef64.data() : nullptr,
-                                                   c32 ? coef32.data() : nullptr, err.data());
+    k_fold<true><<<blocks_for(nr, 4), 256, 0, s>>>(order, r0, nr, R, tcell.data(), tw.data(), dker.data(),
+                                                   cnt.data(), cell.data(), off.data(),
+                                                   c64 ? coef64.data() : nullptr, c32 ? coef32.data() : nullptr,
+                                                   err.data(), untiled.data());
     STL_HIP(hipGetLastError());
   }
   check_err(err, s, "convolution geometry");
   tcell.release();
   tw.release();
-  auto finish = [&](auto* store, auto& coef) {
-    using T = typename std::remove_reference<decltype(coef)>::type;
-    store->order = order;
-    store->S = S;
-    build_transpose<typename std::remove_pointer<decltype(coef.data())>::type, 2 * kL>(
-        R, S, cnt, cell, coef, order, order, store->tcnt, store->trow, store->tcoef, store->Tn, s);
-    (void)sizeof(T);
-  };
-  if (c64) finish(c64, coef64);
-  if (c32) finish(c32, coef32);
+  if (c64) {
+    c64->order = order;
+    c64->S = S;
+    build_conv_transpose<double>(order, R, S, cnt, cell, coef64, *c64, untiled, s);
+  }
+  if (c32) {
+    c32->order = order;
+    c32->S = S;
+    build_conv_transpose<float>(order, R, S, cnt, cell, coef32, *c32, untiled, s);
+  }
+  int untiled_h = 0;
+  STL_HIP(hipMemcpy(&untiled_h, untiled.data(), sizeof(int), hipMemcpyDeviceToHost));
+  bool tiled = order >= 3 && untiled_h == 0;
+  if (c64) c64->tiled = tiled;
+  if (c32) c32->tiled = tiled;
   // Forward arrays: shared ids, owned per store.
   if (c64 && c32) {
     c32->cnt.resize(R);
     c32->cell.resize(cell.size());
+    c32->off.resize(off.size());
     STL_HIP(hipMemcpyAsync(c32->cnt.data(), cnt.data(), cnt.bytes(), hipMemcpyDeviceToDevice, s));
     STL_HIP(hipMemcpyAsync(c32->cell.data(), cell.data(), cell.bytes(), hipMemcpyDeviceToDevice, s));
+    STL_HIP(hipMemcpyAsync(c32->off.data(), off.data(), off.bytes(), hipMemcpyDeviceToDevice, s));
     c32->coef = std::move(coef32);
     c64->cnt = std::move(cnt);
     c64->cell = std::move(cell);
+    c64->off = std::move(off);
     c64->coef = std::move(coef64);
   } else if (c64) {
     c64->cnt = std::move(cnt);
     c64->cell = std::move(cell);
+    c64->off = std::move(off);
     c64->coef = std::move(coef64);
   } else if (c32) {
     c32->cnt = std::move(cnt);
     c32->cell = std::move(cell);
+    c32->off = std::move(off);
     c32->coef = std::move(coef32);
   }
   STL_HIP(hipStreamSynchronize(s));

@@ -15,8 +15,10 @@ template <typename T>
 struct ConvStore {
   int order = 0;
   int S = 0, Tn = 0;
+  bool tiled = false;
   DevArray<uint8_t> cnt, tcnt;
   DevArray<int32_t> cell, trow;
+  DevArray<int16_t> off, toff;
   DevArray<T> coef, tcoef;
   ConvOp<T> view() const {
     ConvOp<T> v;
@@ -25,12 +27,14 @@ struct ConvStore {
     v.R = 3 * v.npf;
     v.S = S;
     v.Tn = Tn;
-    v.cnt = cnt.data(); v.cell = cell.data(); v.coef = coef.data();
-    v.tcnt = tcnt.data(); v.trow = trow.data(); v.tcoef = tcoef.data();
+    v.tiled = tiled;
+    v.cnt = cnt.data(); v.cell = cell.data(); v.off = off.data(); v.coef = coef.data();
+    v.tcnt = tcnt.data(); v.trow = trow.data(); v.toff = toff.data(); v.tcoef = tcoef.data();
     return v;
   }
   size_t bytes() const {
-    return cnt.bytes() + tcnt.bytes() + cell.bytes() + trow.bytes() + coef.bytes() + tcoef.bytes();
+    return cnt.bytes() + tcnt.bytes() + cell.bytes() + trow.bytes() + off.bytes() + toff.bytes() + coef.bytes() +
+           tcoef.bytes();
   }
 };
 

@@ -290,7 +290,7 @@ __device__ inline void rotation_cosmo(double theta, double phi, int g, int G, do
 }
 
 // HEALPix C++ nb_facearray / nb_swaparray by direction 4 + dx + 3 dy (dia.rs:27-41).
-__constant__ const int8_t kNbFace[9][12] = {
+static __constant__ const int8_t kNbFace[9][12] = {
     {8, 9, 10, 11, -1, -1, -1, -1, 10, 11, 8, 9},
     {5, 6, 7, 4, 8, 9, 10, 11, 9, 10, 11, 8},
     {-1, -1, -1, -1, 5, 6, 7, 4, -1, -1, -1, -1},
@@ -301,7 +301,7 @@ __constant__ const int8_t kNbFace[9][12] = {
     {3, 0, 1, 2, 3, 0, 1, 2, 4, 5, 6, 7},
     {2, 3, 0, 1, -1, -1, -1, -1, 0, 1, 2, 3},
 };
-__constant__ const uint8_t kNbSwap[9][3] = {{0, 0, 3}, {0, 0, 6}, {0, 0, 0}, {0, 0, 5}, {0, 0, 0},
+static __constant__ const uint8_t kNbSwap[9][3] = {{0, 0, 3}, {0, 0, 6}, {0, 0, 0}, {0, 0, 5}, {0, 0, 0},
                                              {5, 0, 0}, {0, 0, 0}, {6, 0, 0}, {3, 0, 0}};
 
 // Pixel at extended coordinates of a face, -1 in the quadrant missing at a
@@ -321,6 +321,34 @@ __device__ inline int64_t ext_pix(int face, int64_t ex, int64_t ey, int order) {
   if (bits & 2) y = nside - y - 1;
   if (bits & 4) { int64_t t = x; x = y; y = t; }
   return xyf2nest(x, y, f, order);
+}
+
+// Offset (dx, dy) of `cell` from `row` in the extended frame of row's face;
+// false when cell lies on neither that face nor one of its neighbours (dia.rs:79-101).
+__device__ inline bool rel_offset(int64_t row, int64_t cell, int order, int& dx, int& dy) {
+  int64_t nside = int64_t(1) << order;
+  int64_t x, y, xq, yq;
+  int f, fq;
+  nest2xyf(row, order, x, y, f);
+  nest2xyf(cell, order, xq, yq, fq);
+  if (fq == f) {
+    dx = int(xq - x);
+    dy = int(yq - y);
+    return true;
+  }
+  int nb = -1;
+  for (int k = 0; k < 9; ++k)
+    if (k != 4 && kNbFace[k][f] == fq) { nb = k; break; }
+  if (nb < 0) return false;
+  uint8_t bits = kNbSwap[nb][f >> 2];
+  int64_t a = xq, b = yq;
+  if (bits & 4) { int64_t t = a; a = b; b = t; }
+  if (bits & 2) b = nside - b - 1;
+  if (bits & 1) a = nside - a - 1;
+  int64_t xdir = nb % 3 - 1, ydir = nb / 3 - 1;
+  dx = int(a + xdir * nside - x);
+  dy = int(b + ydir * nside - y);
+  return true;
 }
 
 // Haversine distance (decimation.rs:464-470).

@@ -154,7 +154,12 @@ __global__ __launch_bounds__(256) void k_conv_adj(ConvOp<T> op, const cplx_t<T>*
     size_t at = size_t(s) * op.R + q;
     int64_t pk = rot_pix(op.trow[at], k, op.order);
     T c[8], a[8];
-    Load<T>::coef8(op.tcoef + at * 8, c);
+#pragma unroll
+    for (int g = 0; g < kL; ++g) {
+      cplx_t<T> v = reinterpret_cast<const cplx_t<T>*>(op.tcoef)[(size_t(g) * op.Tn + s) * op.R + q];
+      c[2 * g] = v.x;
+      c[2 * g + 1] = v.y;
+    }
     permute<T>(k, c, a);
 #pragma unroll
     for (int g = 0; g < kL; ++g) {
@@ -169,6 +174,187 @@ __global__ __launch_bounds__(256) void k_conv_adj(ConvOp<T> op, const cplx_t<T>*
   int64_t qk = rot_pix(rep_pixel(q, op.order), k, op.order);
 #pragma unroll
   for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + qk) * NI + j] = acc[j];
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tiled convolution (levels with nside >= 8).  A workgroup is one TS x TS
+// tile of one face (an aligned range of TS^2 NESTED ids, block index
+// blockIdx.x / 4 over the rep faces, face shift blockIdx.x % 4); the inputs of
+// the tile plus a halo of kWin are staged in LDS from the face's extended
+// frame, and every slot reads LDS at its stored (dx, dy) offset.
+// ---------------------------------------------------------------------------
+
+__device__ inline int64_t compress_bits64(int64_t v) {
+  v &= 0x5555555555555555LL;
+  v = (v | (v >> 1)) & 0x3333333333333333LL;
+  v = (v | (v >> 2)) & 0x0F0F0F0F0F0F0F0FLL;
+  v = (v | (v >> 4)) & 0x00FF00FF00FF00FFLL;
+  v = (v | (v >> 8)) & 0x0000FFFF0000FFFFLL;
+  return (v | (v >> 16)) & 0x00000000FFFFFFFFLL;
+}
+
+__device__ inline int64_t spread_bits64(int64_t v) {
+  v &= 0x00000000FFFFFFFFLL;
+  v = (v | (v << 16)) & 0x0000FFFF0000FFFFLL;
+  v = (v | (v << 8)) & 0x00FF00FF00FF00FFLL;
+  v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0FLL;
+  v = (v | (v << 2)) & 0x3333333333333333LL;
+  return (v | (v << 1)) & 0x5555555555555555LL;
+}
+
+// HEALPix C++ neighbour faces / swaps by direction 4 + dx + 3 dy (dia.rs:27-41).
+static __constant__ const int8_t kNbFaceT[9][12] = {
+    {8, 9, 10, 11, -1, -1, -1, -1, 10, 11, 8, 9},
+    {5, 6, 7, 4, 8, 9, 10, 11, 9, 10, 11, 8},
+    {-1, -1, -1, -1, 5, 6, 7, 4, -1, -1, -1, -1},
+    {4, 5, 6, 7, 11, 8, 9, 10, 11, 8, 9, 10},
+    {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11},
+    {1, 2, 3, 0, 0, 1, 2, 3, 5, 6, 7, 4},
+    {-1, -1, -1, -1, 7, 4, 5, 6, -1, -1, -1, -1},
+    {3, 0, 1, 2, 3, 0, 1, 2, 4, 5, 6, 7},
+    {2, 3, 0, 1, -1, -1, -1, -1, 0, 1, 2, 3},
+};
+static __constant__ const uint8_t kNbSwapT[9][3] = {{0, 0, 3}, {0, 0, 6}, {0, 0, 0}, {0, 0, 5}, {0, 0, 0},
+                                                     {5, 0, 0}, {0, 0, 0}, {6, 0, 0}, {3, 0, 0}};
+
+// Pixel at extended coordinates of a face (-nside <= ex, ey < 2 nside), -1 in the
+// quadrant missing at a vertex of three faces.
+__device__ inline int64_t ext_pixel(int face, int ex, int ey, int order) {
+  int nside = 1 << order;
+  int x = ex, y = ey, nb = 4;
+  if (x < 0) { x += nside; nb -= 1; }
+  else if (x >= nside) { x -= nside; nb += 1; }
+  if (y < 0) { y += nside; nb -= 3; }
+  else if (y >= nside) { y -= nside; nb += 3; }
+  int f = kNbFaceT[nb][face];
+  if (f < 0) return -1;
+  uint8_t bits = kNbSwapT[nb][face >> 2];
+  if (bits & 1) x = nside - x - 1;
+  if (bits & 2) y = nside - y - 1;
+  if (bits & 4) { int t = x; x = y; y = t; }
+  return (int64_t(f) << (2 * order)) + spread_bits64(x) + (spread_bits64(y) << 1);
+}
+
+struct TileInfo {
+  int face, fr, k, x0, y0;
+  int64_t base;  // local NESTED index of the first pixel of the tile
+};
+
+template <int TS>
+__device__ inline TileInfo tile_info(int order) {
+  TileInfo t;
+  int64_t npf = int64_t(1) << (2 * order);
+  int64_t ntile = npf / (TS * TS);
+  int64_t tr = blockIdx.x >> 2;
+  t.k = int(blockIdx.x & 3);
+  t.fr = int(tr / ntile);
+  t.base = (tr % ntile) * (TS * TS);
+  t.face = 4 * t.fr + t.k;
+  t.x0 = int(compress_bits64(t.base));
+  t.y0 = int(compress_bits64(t.base >> 1));
+  return t;
+}
+
+template <typename T, int NI, int TS>
+__global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const T* __restrict__ x,
+                                                           cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
+  constexpr int SW = TS + 2 * kWin, SA = SW * SW;
+  __shared__ T xs[SA * NI];
+  const TileInfo ti = tile_info<TS>(op.order);
+  const int64_t N = 12 * op.npf;
+  const int row = blockIdx.y;
+  const T* xr = x + size_t(row) * N * NI;
+  for (int idx = threadIdx.x; idx < SA; idx += TS * TS) {
+    int64_t p = ext_pixel(ti.face, ti.x0 - kWin + idx % SW, ti.y0 - kWin + idx / SW, op.order);
+#pragma unroll
+    for (int j = 0; j < NI; ++j) xs[idx * NI + j] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
+  }
+  __syncthreads();
+  const int li = threadIdx.x;
+  const int pos = (int(compress_bits64(li >> 1)) + kWin) * SW + int(compress_bits64(li)) + kWin;
+  const int64_t r = int64_t(ti.fr) * op.npf + ti.base + li;
+  T acc[NI][kL][2];
+#pragma unroll
+  for (int j = 0; j < NI; ++j)
+#pragma unroll
+    for (int g = 0; g < kL; ++g) acc[j][g][0] = acc[j][g][1] = T(0);
+  const int cnt = op.cnt[r];
+  for (int s = 0; s < cnt; ++s) {
+    size_t at = size_t(s) * op.R + r;
+    int16_t o = op.off[at];
+    int src = (pos + off_dy(o) * SW + off_dx(o)) * NI;
+    T c[8], a[8];
+    Load<T>::coef8(op.coef + at * 8, c);
+    permute<T>(ti.k, c, a);
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+      T xv = xs[src + j];
+#pragma unroll
+      for (int g = 0; g < kL; ++g) {
+        acc[j][g][0] += a[g * 2] * xv;
+        acc[j][g][1] += a[g * 2 + 1] * xv;
+      }
+    }
+  }
+  const int64_t p = int64_t(ti.face) * op.npf + ti.base + li;
+#pragma unroll
+  for (int g = 0; g < kL; ++g)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(acc[j][g][0], acc[j][g][1]);
+  if (mod != nullptr && NI == 1) {
+#pragma unroll
+    for (int g = 0; g < kL; ++g) mod[(size_t(row) * N + p) * kL + g] = hypot(acc[0][g][0], acc[0][g][1]);
+  }
+}
+
+// GP gauges of the cotangent staged per pass.
+template <typename T, int NI, int TS, int GP>
+__global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
+                                                           T* __restrict__ gx) {
+  constexpr int SW = TS + 2 * kWin, SA = SW * SW;
+  __shared__ cplx_t<T> gs[GP * SA * NI];
+  const TileInfo ti = tile_info<TS>(op.order);
+  const int64_t N = 12 * op.npf;
+  const int row = blockIdx.y;
+  const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
+  const int li = threadIdx.x;
+  const int pos = (int(compress_bits64(li >> 1)) + kWin) * SW + int(compress_bits64(li)) + kWin;
+  const int64_t q = int64_t(ti.fr) * op.npf + ti.base + li;
+  const int cnt = op.tcnt[q];
+  const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
+  T acc[NI];
+#pragma unroll
+  for (int j = 0; j < NI; ++j) acc[j] = T(0);
+  for (int g0 = 0; g0 < kL; g0 += GP) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < GP * SA; idx += TS * TS) {
+      int gg = idx / SA, pi = idx % SA;
+      int64_t p = ext_pixel(ti.face, ti.x0 - kWin + pi % SW, ti.y0 - kWin + pi / SW, op.order);
+#pragma unroll
+      for (int j = 0; j < NI; ++j)
+        gs[idx * NI + j] = p >= 0 ? gr[(size_t(g0 + gg) * N + p) * NI + j] : make_c<T>(T(0), T(0));
+    }
+    __syncthreads();
+    for (int s = 0; s < cnt; ++s) {
+      size_t at = size_t(s) * op.R + q;
+      int16_t o = op.toff[at];
+      int src = pos + off_dy(o) * SW + off_dx(o);
+#pragma unroll
+      for (int gg = 0; gg < GP; ++gg) {
+        const int g = g0 + gg;
+        cplx_t<T> a = tc[(size_t(gauge_src(ti.k, g)) * op.Tn + s) * op.R + q];
+        T ai = gauge_conj(ti.k, g) ? -a.y : a.y;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+          cplx_t<T> v = gs[(gg * SA + src) * NI + j];
+          acc[j] += a.x * v.x + ai * v.y;
+        }
+      }
+    }
+  }
+  const int64_t p = int64_t(ti.face) * op.npf + ti.base + li;
+#pragma unroll
+  for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + p) * NI + j] = acc[j];
 }
 
 template <typename T, int NI>
@@ -466,8 +652,30 @@ __global__ void k_cast(int64_t n, const TI* __restrict__ in, TO* __restrict__ ou
 
 }  // namespace
 
+template <typename T, int TS>
+void conv_fwd_tiled(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s) {
+  dim3 grid(unsigned(op.R / (TS * TS) * 4), rows);
+  if (NI == 1) k_conv_fwd_tiled<T, 1, TS><<<grid, TS * TS, 0, s>>>(op, x, y, mod);
+  else k_conv_fwd_tiled<T, kL, TS><<<grid, TS * TS, 0, s>>>(op, x, y, nullptr);
+}
+
+template <typename T, int TS>
+void conv_adj_tiled(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
+  dim3 grid(unsigned(op.R / (TS * TS) * 4), rows);
+  constexpr int GP4 = sizeof(T) == 8 ? 1 : 2;  // NI = kL: 43 KB of LDS per pass
+  if (NI == 1) k_conv_adj_tiled<T, 1, TS, 4><<<grid, TS * TS, 0, s>>>(op, gy, gx);
+  else k_conv_adj_tiled<T, kL, TS, GP4><<<grid, TS * TS, 0, s>>>(op, gy, gx);
+}
+
 template <typename T>
 void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s) {
+  STL_CHECK(NI == 1 || NI == kL, "conv: unsupported inner width");
+  if (op.tiled) {
+    if (op.order >= 4) conv_fwd_tiled<T, 16>(op, NI, x, rows, y, mod, s);
+    else conv_fwd_tiled<T, 8>(op, NI, x, rows, y, mod, s);
+    STL_HIP(hipGetLastError());
+    return;
+  }
   dim3 grid(nblocks(op.R * 4, 256), rows);
   if (NI == 1) k_conv_fwd<T, 1><<<grid, 256, 0, s>>>(op, x, y, mod);
   else if (NI == kL) k_conv_fwd<T, kL><<<grid, 256, 0, s>>>(op, x, y, nullptr);
@@ -477,6 +685,13 @@ void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T
 
 template <typename T>
 void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
+  STL_CHECK(NI == 1 || NI == kL, "conv adjoint: unsupported inner width");
+  if (op.tiled) {
+    if (op.order >= 4) conv_adj_tiled<T, 16>(op, NI, gy, rows, gx, s);
+    else conv_adj_tiled<T, 8>(op, NI, gy, rows, gx, s);
+    STL_HIP(hipGetLastError());
+    return;
+  }
   dim3 grid(nblocks(op.R * 4, 256), rows);
   if (NI == 1) k_conv_adj<T, 1><<<grid, 256, 0, s>>>(op, gy, gx);
   else if (NI == kL) k_conv_adj<T, kL><<<grid, 256, 0, s>>>(op, gy, gx);
