@@ -240,13 +240,35 @@ struct TileInfo {
   int64_t base;  // local NESTED index of the first pixel of the tile
 };
 
+// Workgroup -> (rep tile, face shift k, row).  The 4 * rows workgroups of one
+// rep tile read the same coefficients; workgroups are dispatched round-robin
+// over the 8 XCDs (each with its own L2), so they are numbered to land on the
+// same XCD: in each group of 8 * ways consecutive workgroups, workgroup b runs
+// tile group * 8 + b % 8 and way (b / 8) % ways.
+constexpr int kXcds = 8;
+
 template <int TS>
-__device__ inline TileInfo tile_info(int order) {
+__device__ inline TileInfo tile_info(int order, int rows, int& row) {
   TileInfo t;
-  int64_t npf = int64_t(1) << (2 * order);
-  int64_t ntile = npf / (TS * TS);
-  int64_t tr = blockIdx.x >> 2;
-  t.k = int(blockIdx.x & 3);
+  const int64_t npf = int64_t(1) << (2 * order);
+  const int64_t ntile = npf / (TS * TS);
+  const int64_t tiles = 3 * ntile;
+  const int ways = 4 * rows;
+  const int64_t full = (tiles / kXcds) * kXcds * ways;
+  const int64_t b = blockIdx.x;
+  int64_t tr;
+  int j;
+  if (b < full) {
+    int64_t rem = b % (int64_t(kXcds) * ways);
+    tr = (b / (int64_t(kXcds) * ways)) * kXcds + rem % kXcds;
+    j = int(rem / kXcds);
+  } else {
+    int64_t rem = b - full;
+    tr = (tiles / kXcds) * kXcds + rem / ways;
+    j = int(rem % ways);
+  }
+  t.k = j & 3;
+  row = j >> 2;
   t.fr = int(tr / ntile);
   t.base = (tr % ntile) * (TS * TS);
   t.face = 4 * t.fr + t.k;
@@ -256,13 +278,13 @@ __device__ inline TileInfo tile_info(int order) {
 }
 
 template <typename T, int NI, int TS>
-__global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const T* __restrict__ x,
+__global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const T* __restrict__ x, int rows,
                                                            cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
   constexpr int SW = TS + 2 * kWin, SA = SW * SW;
   __shared__ T xs[SA * NI];
-  const TileInfo ti = tile_info<TS>(op.order);
+  int row;
+  const TileInfo ti = tile_info<TS>(op.order, rows, row);
   const int64_t N = 12 * op.npf;
-  const int row = blockIdx.y;
   const T* xr = x + size_t(row) * N * NI;
   for (int idx = threadIdx.x; idx < SA; idx += TS * TS) {
     int64_t p = ext_pixel(ti.face, ti.x0 - kWin + idx % SW, ti.y0 - kWin + idx / SW, op.order);
@@ -279,6 +301,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const 
 #pragma unroll
     for (int g = 0; g < kL; ++g) acc[j][g][0] = acc[j][g][1] = T(0);
   const int cnt = op.cnt[r];
+#pragma unroll 4
   for (int s = 0; s < cnt; ++s) {
     size_t at = size_t(s) * op.R + r;
     int16_t o = op.off[at];
@@ -309,13 +332,13 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const 
 
 // GP gauges of the cotangent staged per pass.
 template <typename T, int NI, int TS, int GP>
-__global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
+__global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
                                                            T* __restrict__ gx) {
   constexpr int SW = TS + 2 * kWin, SA = SW * SW;
   __shared__ cplx_t<T> gs[GP * SA * NI];
-  const TileInfo ti = tile_info<TS>(op.order);
+  int row;
+  const TileInfo ti = tile_info<TS>(op.order, rows, row);
   const int64_t N = 12 * op.npf;
-  const int row = blockIdx.y;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
   const int li = threadIdx.x;
   const int pos = (int(compress_bits64(li >> 1)) + kWin) * SW + int(compress_bits64(li)) + kWin;
@@ -335,6 +358,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
         gs[idx * NI + j] = p >= 0 ? gr[(size_t(g0 + gg) * N + p) * NI + j] : make_c<T>(T(0), T(0));
     }
     __syncthreads();
+#pragma unroll 4
     for (int s = 0; s < cnt; ++s) {
       size_t at = size_t(s) * op.R + q;
       int16_t o = op.toff[at];
@@ -654,17 +678,17 @@ __global__ void k_cast(int64_t n, const TI* __restrict__ in, TO* __restrict__ ou
 
 template <typename T, int TS>
 void conv_fwd_tiled(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s) {
-  dim3 grid(unsigned(op.R / (TS * TS) * 4), rows);
-  if (NI == 1) k_conv_fwd_tiled<T, 1, TS><<<grid, TS * TS, 0, s>>>(op, x, y, mod);
-  else k_conv_fwd_tiled<T, kL, TS><<<grid, TS * TS, 0, s>>>(op, x, y, nullptr);
+  dim3 grid(unsigned(op.R / (TS * TS) * 4 * rows));
+  if (NI == 1) k_conv_fwd_tiled<T, 1, TS><<<grid, TS * TS, 0, s>>>(op, x, rows, y, mod);
+  else k_conv_fwd_tiled<T, kL, TS><<<grid, TS * TS, 0, s>>>(op, x, rows, y, nullptr);
 }
 
 template <typename T, int TS>
 void conv_adj_tiled(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
-  dim3 grid(unsigned(op.R / (TS * TS) * 4), rows);
+  dim3 grid(unsigned(op.R / (TS * TS) * 4 * rows));
   constexpr int GP4 = sizeof(T) == 8 ? 1 : 2;  // NI = kL: 43 KB of LDS per pass
-  if (NI == 1) k_conv_adj_tiled<T, 1, TS, 4><<<grid, TS * TS, 0, s>>>(op, gy, gx);
-  else k_conv_adj_tiled<T, kL, TS, GP4><<<grid, TS * TS, 0, s>>>(op, gy, gx);
+  if (NI == 1) k_conv_adj_tiled<T, 1, TS, 4><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
+  else k_conv_adj_tiled<T, kL, TS, GP4><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
 }
 
 template <typename T>
