@@ -139,6 +139,17 @@ void* stlp_train_lbfgs(void* h, const uint32_t* f, size_t n, const char* options
 int stlp_train_set_joint(void* h, const double* a, const double* b, uint32_t c, uint32_t d, const char* e, double w);
 int stlp_train_front(void* h, const double* x, size_t n, bool b, double* out);
 
+/* All-GPU synthesis (no counterpart symbol in the reference; it is the
+ * reference's stl-synth objective and L-BFGS, crates/stl-mpi/src/synthesis.rs
+ * :207-268 with crates/stl-core/src/lbfgs.rs:356-465, run on device vectors):
+ * L-BFGS of loss = sum_k w_k |S_k(x) - target_k|^2 (cotangent 2 w (S - target))
+ * from the device maps x_dev (float64, n = batch * 12 nside^2, updated in
+ * place), target interleaved (re, im) [batch, ncoef], weights [ncoef] or NULL
+ * (all 1), options as stlp_lbfgs.  The loss of every evaluation goes to
+ * loss_history (up to cap).  Returns the number of evaluations or -1. */
+int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* target, const double* weights,
+                        const char* options, double* loss_history, size_t cap);
+
 /* Test hooks of this library (no counterpart in the reference): one level's
  * operators of a handle applied to device arrays, for operator-level parity
  * tests.  kind: 0 conv forward (x [rows][N] -> y [rows][4][N] complex),

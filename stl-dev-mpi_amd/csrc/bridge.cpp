@@ -4,6 +4,7 @@
 
 #include <atomic>
 #include <cctype>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -15,6 +16,8 @@
 #include "../../include/stl_bridge.h"
 #include "device_buffer.hpp"
 #include "engine.hpp"
+#include "lbfgs.hpp"
+#include "vectors.hpp"
 
 namespace {
 
@@ -379,6 +382,83 @@ int stlp_set_masks(StlHandle* h, const double* weights, size_t k) {
 int stlp_spin_fields(StlHandle*, const double*, size_t, size_t, double*, double*) {
   g_error = "not implemented: spin 2";
   return -1;
+}
+
+static stl::LbfgsOptions lbfgs_options(const Json& v) {
+  stl::LbfgsOptions opt;
+  opt.lr = v.real("lr", 1.0);
+  opt.max_iter = int(v.integer("max_iter", 20));
+  opt.max_eval = v.has("max_eval") ? int(v.integer("max_eval", 0)) : -1;
+  opt.tolerance_grad = v.real("tolerance_grad", 1e-12);
+  opt.tolerance_change = v.real("tolerance_change", 1e-15);
+  opt.history_size = int(v.integer("history_size", 10));
+  if (!(opt.history_size > 0 && opt.lr > 0.0 && std::isfinite(opt.lr) && opt.max_iter >= 0))
+    fail("invalid L-BFGS options");
+  return opt;
+}
+
+int64_t stlp_lbfgs(double* x, size_t n, const char* options, stlp_callback callback, void* context) {
+  return guard<int64_t>(-1, [&]() -> int64_t {
+    Json v = Json::parse(options);
+    stl::LbfgsOptions opt = lbfgs_options(v);
+    if (!x || !callback) fail("null argument");
+    stl::HostBackend be;
+    std::vector<double> xv(x, x + n);
+    auto f = [&](const std::vector<double>& p, std::vector<double>& g) {
+      g.assign(n, 0.0);
+      double loss = callback(p.data(), n, g.data(), context);
+      bool ok = std::isfinite(loss);
+      for (size_t i = 0; ok && i < n; ++i) ok = std::isfinite(g[i]);
+      if (!ok) fail("non-finite objective or gradient (or Python callback exception)");
+      return loss;
+    };
+    stl::Lbfgs<stl::HostBackend> lb(be, opt);
+    size_t evals = lb.minimize(xv, f);
+    std::copy(xv.begin(), xv.end(), x);
+    return int64_t(evals);
+  });
+}
+
+int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* target, const double* weights,
+                        const char* options, double* loss_history, size_t cap) {
+  return guard<int64_t>(-1, [&]() -> int64_t {
+    if (!h || !x_dev || !target) fail("null argument");
+    if (n != h->input_len()) fail("input shape mismatch");
+    Json v = Json::parse(options);
+    stl::LbfgsOptions opt = lbfgs_options(v);
+    h->current.reset();
+    const size_t nc = h->ncoef(), nv = size_t(h->batch) * nc;
+    std::vector<stl::cplx> values(nv), cot(nv);
+    size_t evals = 0;
+    stl::VectorOps ops(int64_t(n), h->engine->stream());
+    {
+      stl::VectorOps::Vec x = ops.make();
+      STL_HIP(hipMemcpyAsync(x.p, reinterpret_cast<const double*>(x_dev), n * sizeof(double),
+                             hipMemcpyDeviceToDevice, h->engine->stream()));
+      auto f = [&](const stl::VectorOps::Vec& p, stl::VectorOps::Vec& g) {
+        h->engine->forward_device(p.p, false, values.data(), nullptr);
+        double loss = 0.0;
+        for (size_t k = 0; k < nv; ++k) {
+          double w = weights ? weights[k % nc] : 1.0;
+          stl::cplx r = values[k] - stl::cplx(target[2 * k], target[2 * k + 1]);
+          double a = std::abs(r);
+          loss += w * (a * a);
+          cot[k] = (2.0 * w) * r;
+        }
+        if (!std::isfinite(loss)) fail("non-finite loss");
+        h->engine->vjp_device(cot.data(), g.p);
+        if (loss_history && evals < cap) loss_history[evals] = loss;
+        ++evals;
+        return loss;
+      };
+      stl::Lbfgs<stl::VectorOps> lb(ops, opt);
+      lb.minimize(x, f);
+      STL_HIP(hipMemcpyAsync(reinterpret_cast<double*>(x_dev), x.p, n * sizeof(double), hipMemcpyDeviceToDevice,
+                             h->engine->stream()));
+      STL_HIP(hipStreamSynchronize(h->engine->stream()));
+    }
+    return int64_t(evals);
+  });
 }
 
 int stlx_apply_operator(StlHandle* h, int kind, int level, int rows, uint64_t in_dev, uint64_t out_dev) {

@@ -91,6 +91,8 @@ SIGNATURES = {
     "stlp_train_set_joint": ([_VOID, _DOUBLE, _DOUBLE, C.c_uint32, C.c_uint32, C.c_char_p, C.c_double], C.c_int),
     "stlp_train_front": ([_VOID, _DOUBLE, C.c_size_t, C.c_bool, _DOUBLE], C.c_int),
     "stlx_apply_operator": ([_VOID, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64], C.c_int),
+    "stlx_synthesize": ([_VOID, C.c_uint64, C.c_size_t, _DOUBLE, _DOUBLE, C.c_char_p, _DOUBLE, C.c_size_t],
+                        C.c_int64),
 }
 
 
@@ -362,6 +364,31 @@ class NativeEngine:
                 raise _error(self._lib)
             self._current_key = key
 
+    def synthesize(self, x, target, weights=None, lr=1.0, max_iter=20, max_eval=None, tolerance_grad=1e-12,
+                   tolerance_change=1e-15, history_size=10):
+        """L-BFGS on the device of sum_k w_k |S_k(x) - target_k|^2 from the float64 device tensor x
+        [batch, npix] (updated in place); returns the loss of every evaluation."""
+        with self._lock:
+            self._open()
+            addr, size = _device_address(x, self.batch * self.npix)
+            t = np.ascontiguousarray(np.asarray(target, dtype=np.complex128).reshape(self._shape()))
+            w = None if weights is None else np.ascontiguousarray(np.asarray(weights, dtype=np.float64).reshape(-1))
+            if w is not None and w.size != self.ncoeff:
+                raise ValueError(f"expected {self.ncoeff} weights")
+            opts = dict(lr=lr, max_iter=max_iter, tolerance_grad=tolerance_grad, tolerance_change=tolerance_change,
+                        history_size=history_size)
+            if max_eval is not None:
+                opts["max_eval"] = max_eval
+            cap = (max_eval or max_iter * 5 // 4) + 64
+            hist = np.zeros(cap)
+            self._current_key = None
+            n = self._lib.stlx_synthesize(self._handle, addr, size, _pointer(t.view(np.float64)),
+                                          _pointer(w) if w is not None else None, json.dumps(opts).encode(),
+                                          _pointer(hist), cap)
+            if n < 0:
+                raise _error(self._lib)
+            return hist[:min(n, cap)].copy()
+
     def apply_operator(self, kind, level, x, out):
         """Test hook: one level's operator on float64 device tensors (see stlx_apply_operator)."""
         with self._lock:
@@ -385,6 +412,74 @@ class NativeEngine:
 
     def __exit__(self, *_):
         self.close()
+
+
+def ergodic_weights(native_pixels, variance):
+    """Ergodic weights 1 / (n sigma_i^2), sigma_i^2 = V_i / N_i (reference
+    crates/stl-mpi/src/synthesis.rs:296-314): V_i the spatial variance of the
+    terms of coefficient i on the target (forward(..., moments=True)), N_i the
+    pixels it averages (NativeEngine.native_pixels), n the number of
+    coefficients; a zero variance gets the largest finite weight."""
+    v = np.asarray(variance, dtype=np.float64).reshape(-1)
+    n_i = np.asarray(native_pixels, dtype=np.float64).reshape(-1)
+    if v.size != n_i.size:
+        raise ValueError("one variance per coefficient expected")
+    w = np.full(v.shape, np.inf)
+    pos = v > 0
+    w[pos] = n_i[pos] / v[pos]
+    finite = np.isfinite(w)
+    w[~finite] = w[finite].max() if finite.any() else 0.0
+    return w / w.size
+
+
+def native_lbfgs(callback, x, options=None, library=None):
+    """One L-BFGS minimization (strong Wolfe) through stlp_lbfgs (reference
+    native.py:1007-1052): x (float64, updated in place when C-contiguous
+    float64) is the start; callback(x_view) -> (loss, gradient of x's shape)
+    gets a read-only view valid during the call.  Returns (x, report)."""
+    lib = _library(library)
+    resolved = dict(lr=1.0, max_iter=20, max_eval=None, tolerance_grad=1e-12, tolerance_change=1e-15,
+                    history_size=10)
+    if options:
+        unknown = set(options) - set(resolved)
+        if unknown:
+            raise ValueError(f"unknown native L-BFGS options: {sorted(unknown)}")
+        resolved.update(options)
+    for k in ("max_iter", "history_size"):
+        if isinstance(resolved[k], bool) or int(resolved[k]) != resolved[k] or resolved[k] < 1:
+            raise ValueError(f"{k} must be a positive integer")
+    if resolved["max_eval"] is not None and (int(resolved["max_eval"]) != resolved["max_eval"]
+                                             or resolved["max_eval"] < 1):
+        raise ValueError("max_eval must be a positive integer or None")
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    if x.size == 0 or not np.isfinite(x).all():
+        raise ValueError("finite nonempty initial ensemble required")
+    shape, history, errors = x.shape, [], []
+
+    @_CALLBACK
+    def evaluate(p, n, gp, _context):
+        try:
+            trial = np.ctypeslib.as_array(p, shape=(n,)).reshape(shape)
+            trial.flags.writeable = False
+            value, grad = callback(trial)
+            grad = np.asarray(grad, dtype=np.float64)
+            if grad.shape != shape:
+                raise ValueError(f"gradient shape {grad.shape} != {shape}")
+            np.ctypeslib.as_array(gp, shape=(n,))[:] = grad.ravel()
+            history.append(float(value))
+            return float(value)
+        except BaseException as exc:  # reported to the library as a non-finite loss
+            errors.append(exc)
+            return float("nan")
+
+    opts = {k: v for k, v in resolved.items() if v is not None}
+    count = lib.stlp_lbfgs(_pointer(x), x.size, json.dumps(opts).encode(), evaluate, None)
+    if errors:
+        raise errors[0]
+    if count < 0:
+        raise _error(lib)
+    return x, dict(evaluations=int(count), loss_history=history, options=resolved, optimizer="native L-BFGS",
+                   line_search="strong_wolfe")
 
 
 def _device_address(a, expected):

@@ -14,7 +14,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 
-# Cases kept per suite: every stats case; geometry and block cases that pin
+# Cases kept per suite: every stats and synthesis case; geometry and block cases that pin
 # each primitive at small and large nside while keeping the copy small.
 KEEP = {
     "stats": lambda name: True,
@@ -31,6 +31,7 @@ KEEP = {
     ),
     "block": lambda name: name.startswith(("wavelet_", "decimate_n16", "decimate_n32", "level_n16", "level_n32",
                                            "wave_n16", "wave_n32")),
+    "synthesis": lambda name: True,
 }
 
 
