@@ -92,6 +92,87 @@ def run_engine(precision, nside, J, x, t, steps, warmup, timing):
     return out
 
 
+ENSEMBLE = [(128, 8), (256, 4), (512, 2)]  # (nside, maps), J = log2(nside) - 1
+
+
+def run_ensemble(steps, warmup):
+    """Forward + VJP of 8 maps at nside 128, 4 at 256 and 2 at 512 (lognormal
+    recipe, per-map seeds), mixed precision: as one ensemble (one HIP graph
+    launch per forward and per VJP over all groups) and, for comparison, as
+    three standalone batched engines run one after the other."""
+    import torch
+    from stl_port import NativeEngine, NativeEnsemble
+
+    xs = [np.stack([lognormal_map(n, 100 + 10 * g + b) for b in range(m)]) for g, (n, m) in enumerate(ENSEMBLE)]
+    ts = [np.stack([lognormal_map(n, 200 + 10 * g + b) for b in range(m)]) for g, (n, m) in enumerate(ENSEMBLE)]
+    xd = [torch.from_numpy(x).cuda() for x in xs]
+    td = [torch.from_numpy(t).cuda() for t in ts]
+    out = [torch.empty_like(x) for x in xd]
+    res = {}
+    with NativeEnsemble(ENSEMBLE, precision="mixed") as ens:
+        target = ens.forward_device(td)
+
+        def step():
+            vals = ens.forward_device(xd)
+            ens.vjp_device([2.0 * (v - t) for v, t in zip(vals, target)], out)
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        res["ensemble_ms"] = 1e3 * (time.perf_counter() - t0) / steps
+    engines = [NativeEngine(n, n.bit_length() - 2, batch=m, precision="mixed") for n, m in ENSEMBLE]
+    targets = [e.forward_device(t, objective="target")[0] for e, t in zip(engines, td)]
+
+    def step_separate(gen):
+        for e, x, t, o in zip(engines, xd, targets, out):
+            v, _ = e.forward_device(x, generation=gen)
+            e.vjp_device(2.0 * (v - t), o, generation=gen)
+
+    for i in range(warmup):
+        step_separate(i + 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step_separate(warmup + i + 1)
+    torch.cuda.synchronize()
+    res["separate_engines_ms"] = 1e3 * (time.perf_counter() - t0) / steps
+    for e in engines:
+        e.close()
+    maps = sum(m for _, m in ENSEMBLE)
+    return {"groups": [{"nside": n, "maps": m, "J": n.bit_length() - 2} for n, m in ENSEMBLE],
+            "evals_per_s": round(1e3 / res["ensemble_ms"], 2), "maps_per_s": round(maps * 1e3 / res["ensemble_ms"], 1),
+            "ms_per_eval": round(res["ensemble_ms"], 3), "separate_engines_ms_per_eval": round(res["separate_engines_ms"], 3),
+            "precision": "mixed", "steps": steps}
+
+
+def run_synthesis(nside, J, iters):
+    """L-BFGS synthesis on the device (stlx_synthesize): target the lognormal
+    map (seed 1), start N(0, 1) (seed 2), ergodic weights (the loss is
+    sum |S(x) - S(t)|^2 / var up to the reference's normalisation), history 10,
+    mixed-precision forward + VJP, FP64 L-BFGS."""
+    import torch
+    from stl_port import NativeEngine, ergodic_weights
+
+    t = lognormal_map(nside, 1)
+    x0 = np.random.default_rng(2).standard_normal(12 * nside * nside)
+    with NativeEngine(nside, J, precision="mixed") as eng:
+        target, var = eng.forward_device(torch.from_numpy(t).cuda().reshape(1, -1), moments=True)
+        w = ergodic_weights(eng.native_pixels(), var[0])
+        x = torch.from_numpy(x0).cuda().reshape(1, -1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        losses = eng.synthesize(x, target, weights=w, max_iter=iters, history_size=10)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+    return {"nside": nside, "J": J, "iterations": iters, "evaluations": int(len(losses)), "seconds": round(sec, 3),
+            "ms_per_evaluation": round(1e3 * sec / len(losses), 3), "first_loss": float(losses[0]),
+            "last_loss": float(losses[-1]), "history": 10, "weights": "ergodic"}
+
+
 def roofline(timers, steps):
     """Dominant kernel (family at one level) by total time: algorithmic bytes
     per launch over its mean launch duration (HIP events on the engine stream)."""
@@ -153,6 +234,9 @@ def main():
     ap.add_argument("--J", type=int, default=9)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--synthesis-iters", type=int, default=20,
+                    help="L-BFGS iterations of the synthesis measurement at the bench nside (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the whole sky fits one "
@@ -162,12 +246,26 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
 
     x, t = lognormal_map(args.nside, 1), lognormal_map(args.nside, 2)
-    mixed = run_engine("mixed", args.nside, args.J, x, t, args.steps, args.warmup, timing=True)
+    # Throughput with the forward and the VJP replayed as HIP graphs; then the
+    # same steps launched eagerly with HIP events around every kernel, for the
+    # per-kernel durations of the roofline.
+    mixed = run_engine("mixed", args.nside, args.J, x, t, args.steps, args.warmup, timing=False)
     print(f"[bench] mixed: {args.steps / mixed['elapsed']:.2f} evals/s", file=sys.stderr, flush=True)
-    fp64 = None
+    tsteps = max(3, min(args.steps, 10))
+    mixed_t = run_engine("mixed", args.nside, args.J, x, t, tsteps, 1, timing=True)
+    print(f"[bench] mixed, per-kernel events: {tsteps / mixed_t['elapsed']:.2f} evals/s", file=sys.stderr,
+          flush=True)
+    fp64 = fp64_t = None
     if not args.no_fp64:
-        fp64 = run_engine("fp64", args.nside, args.J, x, t, args.steps, args.warmup, timing=True)
+        fp64 = run_engine("fp64", args.nside, args.J, x, t, args.steps, args.warmup, timing=False)
         print(f"[bench] fp64: {args.steps / fp64['elapsed']:.2f} evals/s", file=sys.stderr, flush=True)
+        fp64_t = run_engine("fp64", args.nside, args.J, x, t, tsteps, 1, timing=True)
+    ensemble = None if args.no_ensemble else run_ensemble(max(3, args.steps // 2), 2)
+    if ensemble:
+        print(f"[bench] ensemble: {ensemble['evals_per_s']} evals/s", file=sys.stderr, flush=True)
+    synthesis = run_synthesis(args.nside, args.J, args.synthesis_iters) if args.synthesis_iters > 0 else None
+    if synthesis:
+        print(f"[bench] synthesis: {synthesis['ms_per_evaluation']} ms/evaluation", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
 
     value = args.steps / mixed["elapsed"]
@@ -187,15 +285,19 @@ def main():
         "config": {"workload": f"ScatCov forward+VJP of sum|S(x)-S(t)|^2, HEALPix NESTED nside {args.nside}, "
                                f"J={args.J}, L=4, kernel 5, 1 map, {len(mixed['plan'])} coefficients",
                    "nside": args.nside, "J": args.J, "L": 4, "batch": 1, "precision": "mixed"},
-        "roofline": roofline(mixed["timers"], args.steps),
+        "roofline": roofline(mixed_t["timers"], tsteps),
         "cpu_baseline": cpu,
         "setup_s": round(mixed["setup_s"], 3),
         "device_bytes": mixed["device_bytes"],
+        "peak_device_bytes": mixed["peak_bytes"],
+        "ms_per_step_with_kernel_events": round(1e3 * mixed_t["elapsed"] / tsteps, 3),
+        "ensemble": ensemble,
+        "synthesis": synthesis,
     }
     if fp64 is not None:
         v64 = args.steps / fp64["elapsed"]
         line["fp64"] = {"value": round(v64, 3), "ms_per_step": round(1e3 * fp64["elapsed"] / args.steps, 3),
-                        "roofline": roofline(fp64["timers"], args.steps), "setup_s": round(fp64["setup_s"], 3),
+                        "roofline": roofline(fp64_t["timers"], tsteps), "setup_s": round(fp64["setup_s"], 3),
                         "device_bytes": fp64["device_bytes"]}
         line["mixed_over_fp64"] = round(value / v64, 3)
         line["mixed_error_vs_fp64"] = family_errors(mixed["values"], fp64["values"], mixed["plan"])

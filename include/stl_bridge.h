@@ -150,6 +150,26 @@ int stlp_train_front(void* h, const double* x, size_t n, bool b, double* out);
 int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* target, const double* weights,
                         const char* options, double* loss_history, size_t cap);
 
+/* Ensembles of maps of uneven nside (no counterpart in the reference; its
+ * ensemble driver loops over NativeEngine handles): groups g of batch[g] maps
+ * at nside[g] with j[g] scales, evaluated together, each forward and each VJP
+ * one HIP graph launch over all groups (operators shared between the groups'
+ * levels of equal nside).  options: precision ("mixed" | "fp64"), device. */
+typedef struct StlEnsemble StlEnsemble;
+StlEnsemble* stlx_ensemble_new(const int32_t* nside, const int32_t* j, const int32_t* batch, size_t groups,
+                               const char* options);
+void stlx_ensemble_free(StlEnsemble* e);
+/* Coefficients of each group's plan (groups size_t). */
+int stlx_ensemble_ncoef(StlEnsemble* e, size_t* out);
+/* x_dev[g]: float64 device maps [batch[g], 12 nside[g]^2]; out: interleaved
+ * (re, im) statistics concatenated over groups [g][batch][ncoef_g]; var the
+ * same without the pairs when moments.  0 or -1. */
+int stlx_ensemble_forward_device(StlEnsemble* e, const uint64_t* x_dev, size_t groups, bool moments, double* out,
+                                 double* var);
+/* Map cotangents of Re sum(conj(cot) values) after a forward: cot as out
+ * (n complex values), gradients to the device arrays grad_dev[g]. */
+int stlx_ensemble_vjp_device(StlEnsemble* e, const double* cot, size_t n, const uint64_t* grad_dev, size_t groups);
+
 /* Test hooks of this library (no counterpart in the reference): one level's
  * operators of a handle applied to device arrays, for operator-level parity
  * tests.  kind: 0 conv forward (x [rows][N] -> y [rows][4][N] complex),

@@ -16,6 +16,7 @@
 #include "../../include/stl_bridge.h"
 #include "device_buffer.hpp"
 #include "engine.hpp"
+#include "ensemble.hpp"
 #include "lbfgs.hpp"
 #include "vectors.hpp"
 
@@ -152,6 +153,11 @@ struct StlHandle {
 struct StlSnapshot {
   Identity identity;
   std::vector<char> state;
+};
+
+struct StlEnsemble {
+  std::unique_ptr<stl::Ensemble> ensemble;
+  bool forwarded = false;
 };
 
 extern "C" {
@@ -435,8 +441,14 @@ int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* ta
       stl::VectorOps::Vec x = ops.make();
       STL_HIP(hipMemcpyAsync(x.p, reinterpret_cast<const double*>(x_dev), n * sizeof(double),
                              hipMemcpyDeviceToDevice, h->engine->stream()));
+      // The engine replays graphs keyed by device pointer: evaluate at the
+      // handle's fixed staging buffers, not at the pooled vectors.
+      hipStream_t st = h->engine->stream();
+      h->xbuf.resize(n);
+      h->gbuf.resize(n);
       auto f = [&](const stl::VectorOps::Vec& p, stl::VectorOps::Vec& g) {
-        h->engine->forward_device(p.p, false, values.data(), nullptr);
+        STL_HIP(hipMemcpyAsync(h->xbuf.data(), p.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        h->engine->forward_device(h->xbuf.data(), false, values.data(), nullptr);
         double loss = 0.0;
         for (size_t k = 0; k < nv; ++k) {
           double w = weights ? weights[k % nc] : 1.0;
@@ -446,7 +458,8 @@ int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* ta
           cot[k] = (2.0 * w) * r;
         }
         if (!std::isfinite(loss)) fail("non-finite loss");
-        h->engine->vjp_device(cot.data(), g.p);
+        h->engine->vjp_device(cot.data(), h->gbuf.data());
+        STL_HIP(hipMemcpyAsync(g.p, h->gbuf.data(), n * sizeof(double), hipMemcpyDeviceToDevice, st));
         if (loss_history && evals < cap) loss_history[evals] = loss;
         ++evals;
         return loss;
@@ -458,6 +471,71 @@ int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* ta
       STL_HIP(hipStreamSynchronize(h->engine->stream()));
     }
     return int64_t(evals);
+  });
+}
+
+StlEnsemble* stlx_ensemble_new(const int32_t* nside, const int32_t* j, const int32_t* batch, size_t groups,
+                               const char* options) {
+  return guard<StlEnsemble*>(nullptr, [&]() -> StlEnsemble* {
+    if (!nside || !j || !batch || groups == 0) fail("an ensemble needs at least one group");
+    Json v = Json::parse(options ? options : "{}");
+    std::string precision = v.string("precision", "mixed");
+    if (precision != "fp64" && precision != "mixed") fail("precision must be fp64 or mixed");
+    std::vector<stl::EnsembleGroup> gs(groups);
+    for (size_t g = 0; g < groups; ++g) {
+      if (nside[g] < 1 || (nside[g] & (nside[g] - 1))) fail("nside must be a power of two");
+      int order = 0;
+      while ((1 << order) < nside[g]) ++order;
+      if (j[g] < 1 || j[g] > order + 1) fail("j must satisfy 1 <= j <= log2(nside) + 1");
+      if (batch[g] < 1) fail("batch must be positive");
+      gs[g] = stl::EnsembleGroup{nside[g], j[g], batch[g]};
+    }
+    auto e = std::make_unique<StlEnsemble>();
+    e->ensemble = std::make_unique<stl::Ensemble>(gs, precision == "fp64", int(v.integer("device", 0)));
+    return e.release();
+  });
+}
+
+void stlx_ensemble_free(StlEnsemble* e) { delete e; }
+
+int stlx_ensemble_ncoef(StlEnsemble* e, size_t* out) {
+  return guard<int>(-1, [&] {
+    if (!e || !out) fail("null argument");
+    for (size_t g = 0; g < e->ensemble->groups(); ++g) out[g] = e->ensemble->engine(g).plan().size();
+    return 0;
+  });
+}
+
+int stlx_ensemble_forward_device(StlEnsemble* e, const uint64_t* x_dev, size_t groups, bool moments, double* out,
+                                 double* var) {
+  return guard<int>(-1, [&] {
+    if (!e || !x_dev || !out || (moments && !var)) fail("null argument");
+    if (groups != e->ensemble->groups()) fail("one device map array per group expected");
+    std::vector<const double*> x(groups);
+    for (size_t g = 0; g < groups; ++g) {
+      if (!x_dev[g]) fail("null device map");
+      x[g] = reinterpret_cast<const double*>(x_dev[g]);
+    }
+    e->forwarded = false;
+    e->ensemble->forward(x.data(), moments, reinterpret_cast<stl::cplx*>(out), moments ? var : nullptr);
+    e->forwarded = true;
+    return 0;
+  });
+}
+
+int stlx_ensemble_vjp_device(StlEnsemble* e, const double* cot, size_t n, const uint64_t* grad_dev, size_t groups) {
+  return guard<int>(-1, [&] {
+    if (!e || !cot || !grad_dev) fail("null argument");
+    if (!e->forwarded) fail("no forward state: call stlx_ensemble_forward_device first");
+    if (n != e->ensemble->values()) fail("cotangent shape mismatch");
+    if (groups != e->ensemble->groups()) fail("one device gradient array per group expected");
+    std::vector<double*> g(groups);
+    for (size_t k = 0; k < groups; ++k) {
+      if (!grad_dev[k]) fail("null device gradient");
+      g[k] = reinterpret_cast<double*>(grad_dev[k]);
+    }
+    e->ensemble->vjp(reinterpret_cast<const stl::cplx*>(cot), g.data());
+    return 0;
   });
 }
 

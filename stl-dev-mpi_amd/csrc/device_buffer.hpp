@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstddef>
 #include <utility>
 
@@ -11,8 +12,8 @@
 namespace stl {
 
 // Bytes currently held by DevArray allocations of this process.
-size_t& device_bytes_live();
-size_t& device_bytes_peak();
+std::atomic<size_t>& device_bytes_live();
+std::atomic<size_t>& device_bytes_peak();
 
 template <typename T>
 class DevArray {
@@ -39,7 +40,8 @@ class DevArray {
       STL_HIP(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)));
       n_ = n;
       device_bytes_live() += bytes();
-      if (device_bytes_live() > device_bytes_peak()) device_bytes_peak() = device_bytes_live();
+      size_t live = device_bytes_live().load();
+      if (live > device_bytes_peak().load()) device_bytes_peak() = live;
     }
   }
 
@@ -56,6 +58,39 @@ class DevArray {
     if (n_) STL_HIP(hipMemsetAsync(p_, 0, bytes(), s));
   }
 
+  T* data() { return p_; }
+  const T* data() const { return p_; }
+  size_t size() const { return n_; }
+  size_t bytes() const { return n_ * sizeof(T); }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// Page-locked host allocation (hipHostMalloc): staging of the statistics and
+// cotangents, so that their copies are plain DMA transfers.
+template <typename T>
+class PinnedArray {
+ public:
+  PinnedArray() = default;
+  ~PinnedArray() { release(); }
+  PinnedArray(const PinnedArray&) = delete;
+  PinnedArray& operator=(const PinnedArray&) = delete;
+
+  void resize(size_t n) {
+    if (n == n_) return;
+    release();
+    if (n) {
+      STL_HIP(hipHostMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T), hipHostMallocDefault));
+      n_ = n;
+    }
+  }
+  void release() {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
   T* data() { return p_; }
   const T* data() const { return p_; }
   size_t size() const { return n_; }

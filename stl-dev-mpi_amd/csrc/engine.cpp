@@ -27,12 +27,12 @@
 
 namespace stl {
 
-size_t& device_bytes_live() {
-  static size_t v = 0;
+std::atomic<size_t>& device_bytes_live() {
+  static std::atomic<size_t> v{0};
   return v;
 }
-size_t& device_bytes_peak() {
-  static size_t v = 0;
+std::atomic<size_t>& device_bytes_peak() {
+  static std::atomic<size_t> v{0};
   return v;
 }
 
@@ -207,22 +207,11 @@ class EngineT final : public Engine {
     levels_ = opt.J;
     for (auto& c : plan_) slots_.push_back(slot_of(c));
     B_ = opt.batch;
-    std::vector<double> ker = wavelet_kernel_f32(5);
-    conv_.resize(levels_);
-    dec_.resize(levels_ > 1 ? levels_ - 1 : 0);
     for (int n = 0; n < levels_; ++n) {
       int order = order0 - n;
       N_.push_back(12 * (int64_t(1) << (2 * order)));
-      if constexpr (std::is_same<T, double>::value) build_conv(order, ker, &conv_[n], nullptr, s_);
-      else build_conv(order, ker, nullptr, &conv_[n], s_);
-      conv_slots_.push_back(count_slots(conv_[n].cnt, conv_[n].view().R));
-      if (n + 1 < levels_) {
-        if constexpr (std::is_same<T, double>::value) build_dec(order, &dec_[n], nullptr, s_);
-        else build_dec(order, nullptr, &dec_[n], s_);
-        dec_slots_.push_back(count_slots(dec_[n].cnt, dec_[n].view().Rc));
-        dec_tslots_.push_back(count_slots(dec_[n].tcnt, dec_[n].view().Rf));
-      }
-      conv_tslots_.push_back(count_slots(conv_[n].tcnt, conv_[n].view().R));
+      conv_.push_back(shared_conv<T>(opt.device, order, s_));
+      if (n + 1 < levels_) dec_.push_back(shared_dec<T>(opt.device, order, s_));
     }
     geometry_bytes_ = device_bytes_live();
     allocate();
@@ -231,6 +220,8 @@ class EngineT final : public Engine {
   }
 
   ~EngineT() override {
+    for (auto* cache : {&fwd_graphs_, &vjp_graphs_})
+      for (auto& e : *cache) (void)hipGraphExecDestroy(e.exec);
     if (s_) (void)hipStreamDestroy(s_);
   }
 
@@ -244,9 +235,22 @@ class EngineT final : public Engine {
   void reset_timers() override { prof_.reset(); }
 
   void forward_device(const double* x_dev, bool moments, cplx* values, double* variance) override {
-    load_maps(x_dev);
-    for (int n = 1; n < levels_; ++n) dec_maps(n);
-    sweep_forward(moments, values, variance);
+    run(fwd_graphs_, x_dev, moments, [&] { enqueue_forward(x_dev, moments); });
+    finish_forward(moments, values, variance);
+  }
+
+  hipGraph_t capture_forward(const double* x_dev, bool moments) override {
+    STL_CHECK(!prof_.on(), "graph capture with per-kernel timing");
+    return capture([&] { enqueue_forward(x_dev, moments); });
+  }
+
+  void stage_cotangent(const cplx* cot) override {
+    std::memcpy(hcot_.data(), cot, size_t(B_) * plan_.size() * sizeof(double2));
+  }
+
+  hipGraph_t capture_vjp(double* grad_dev) override {
+    STL_CHECK(!prof_.on(), "graph capture with per-kernel timing");
+    return capture([&] { sweep_reverse(grad_dev); });
   }
 
   void forward_levels_device(const double* x_dev, cplx* values) override {
@@ -256,56 +260,23 @@ class EngineT final : public Engine {
       cast_in(x_dev + at, maps_[n].data(), len);
       at += size_t(len);
     }
-    sweep_forward(false, values, nullptr);
+    sweep_forward(false);
+    STL_HIP(hipStreamSynchronize(s_));
+    prof_.collect();
+    finish_forward(false, values, nullptr);
   }
 
   void vjp_device(const cplx* cot, double* grad_dev) override {
-    size_t np = plan_.size();
+    stage_cotangent(cot);
+    run(vjp_graphs_, grad_dev, false, [&] { sweep_reverse(grad_dev); });
+  }
+
+  void sweep_reverse(double* grad_dev) {
+    const size_t nv = size_t(B_) * plan_.size();
+    STL_HIP(hipMemcpyAsync(dcot_.data(), hcot_.data(), nv * sizeof(double2), hipMemcpyHostToDevice, s_));
+    launch_scatter_cot(nrt_, B_, int(plan_.size()), drt_.data(), drk_.data(), dcot_.data(), ddst_.data(), s_);
     for (int n = levels_ - 1; n >= 0; --n) {
       const int64_t N = N_[n];
-      const int K = gram_k(n);
-      auto& H = hH_[n];
-      std::fill(H.begin(), H.end(), double2{0.0, 0.0});
-      auto& s1 = hs1_[n];
-      auto& s2 = hs2_[n];
-      std::fill(s1.begin(), s1.end(), 0.0);
-      std::fill(s2.begin(), s2.end(), 0.0);
-      if (n == 0) {
-        std::fill(hgm_.begin(), hgm_.end(), 0.0);
-        std::fill(hgv_.begin(), hgv_.end(), 0.0);
-      }
-      for (size_t i = 0; i < np; ++i) {
-        const SumSlot& sl = slots_[i];
-        if (sl.n != n) continue;
-        const double Nd = double(sl.kind == SumSlot::MEANVAR ? N_[0] : N);
-        for (int b = 0; b < B_; ++b) {
-          cplx g = cot[size_t(b) * np + i];
-          if (g == cplx(0.0, 0.0)) continue;
-          switch (sl.kind) {
-            case SumSlot::MEANVAR:
-              if (sl.idx == 0) hgm_[b] += g.real() / Nd;
-              else hgv_[b] += 2.0 * g.real() / Nd;
-              break;
-            case SumSlot::ZSTAT:
-              if (sl.idx == 0) s1[b * kL + sl.l] += g.real() / Nd;
-              else s2[b * kL + sl.l] += 2.0 * g.real() / Nd;
-              break;
-            case SumSlot::GRAM: {
-              double2* h = &H[size_t(b * kL + sl.l) * K * K];
-              cplx a = g / Nd, c = std::conj(g) / Nd;
-              h[sl.gi * K + sl.gj].x += a.real();
-              h[sl.gi * K + sl.gj].y += a.imag();
-              h[sl.gj * K + sl.gi].x += c.real();
-              h[sl.gj * K + sl.gi].y += c.imag();
-              break;
-            }
-          }
-        }
-      }
-      STL_HIP(hipMemcpyAsync(dH_[n].data(), H.data(), H.size() * sizeof(double2), hipMemcpyHostToDevice, s_));
-      STL_HIP(hipMemcpyAsync(ds1_[n].data(), s1.data(), s1.size() * sizeof(double), hipMemcpyHostToDevice, s_));
-      STL_HIP(hipMemcpyAsync(ds2_[n].data(), s2.data(), s2.size() * sizeof(double), hipMemcpyHostToDevice, s_));
-
       int id = prof_.start();
       launch_gv<T>(n, N, B_, br_[n].data(), z_[n].data(), dH_[n].data(), gbr_.data(), gzp_.data(), s_);
       prof_.stop(id, tag("gv", n), double((n + 2) * B_ * kL) * N * kL * 2 * sizeof(T) * 2);
@@ -324,12 +295,8 @@ class EngineT final : public Engine {
       conv_adj(n, 1, gz_.data(), B_, mg);
       if (n + 1 < levels_) dec_adj(n, 1, mg_prev, mg, B_, mg);
     }
-    STL_HIP(hipMemcpyAsync(dgm_.data(), hgm_.data(), hgm_.size() * sizeof(double), hipMemcpyHostToDevice, s_));
-    STL_HIP(hipMemcpyAsync(dgv_.data(), hgv_.data(), hgv_.size() * sizeof(double), hipMemcpyHostToDevice, s_));
     launch_meanvar_grad<T>(N_[0], B_, maps_[0].data(), dgm_.data(), dgv_.data(), mg_[0].data(), s_);
     cast_out(mg_[0].data(), grad_dev, int64_t(B_) * N_[0]);
-    STL_HIP(hipStreamSynchronize(s_));
-    prof_.collect();
   }
 
   void apply_operator(int kind, int n, int rows, const double* in, double* out) override {
@@ -340,21 +307,21 @@ class EngineT final : public Engine {
       DevArray<T> x(size_t(rows) * N);
       DevArray<C2> y(size_t(rows) * kL * N);
       cast_in(in, x.data(), int64_t(rows) * N);
-      launch_conv_fwd<T>(conv_[n].view(), 1, x.data(), rows, y.data(), nullptr, s_);
+      launch_conv_fwd<T>(conv_[n]->store.view(), 1, x.data(), rows, y.data(), nullptr, s_);
       cast_out(reinterpret_cast<const T*>(y.data()), out, int64_t(rows) * kL * N * 2);
     } else if (kind == 1) {
       DevArray<C2> gy(size_t(rows) * kL * N);
       DevArray<T> gx(size_t(rows) * N);
       cast_in(in, reinterpret_cast<T*>(gy.data()), int64_t(rows) * kL * N * 2);
-      launch_conv_adj<T>(conv_[n].view(), 1, gy.data(), rows, gx.data(), s_);
+      launch_conv_adj<T>(conv_[n]->store.view(), 1, gy.data(), rows, gx.data(), s_);
       cast_out(gx.data(), out, int64_t(rows) * N);
     } else if (kind == 2 || kind == 3) {
       STL_CHECK(n + 1 < levels_, "no coarser level");
       const int64_t Nc = N_[n + 1];
       DevArray<T> a(size_t(rows) * (kind == 2 ? N : Nc)), b(size_t(rows) * (kind == 2 ? Nc : N));
       cast_in(in, a.data(), int64_t(a.size()));
-      if (kind == 2) launch_dec_fwd<T>(dec_[n].view(), 1, a.data(), rows, b.data(), s_);
-      else launch_dec_adj<T>(dec_[n].view(), 1, a.data(), nullptr, rows, b.data(), s_);
+      if (kind == 2) launch_dec_fwd<T>(dec_[n]->store.view(), 1, a.data(), rows, b.data(), s_);
+      else launch_dec_adj<T>(dec_[n]->store.view(), 1, a.data(), nullptr, rows, b.data(), s_);
       cast_out(b.data(), out, int64_t(b.size()));
     } else {
       throw std::runtime_error("unknown operator kind");
@@ -395,14 +362,6 @@ class EngineT final : public Engine {
   }
 
  private:
-  double count_slots(const DevArray<uint8_t>& cnt, int64_t n) {
-    std::vector<uint8_t> h(static_cast<size_t>(n));
-    STL_HIP(hipMemcpy(h.data(), cnt.data(), size_t(n), hipMemcpyDeviceToHost));
-    double s = 0.0;
-    for (auto v : h) s += v;
-    return s;
-  }
-
   void allocate() {
     maps_.resize(levels_);
     z_.resize(levels_);
@@ -414,15 +373,9 @@ class EngineT final : public Engine {
     gsum2_.resize(levels_);
     zpart_.resize(levels_);
     zsum_.resize(levels_);
-    hgsum_.resize(levels_);
-    hgsum2_.resize(levels_);
-    hzsum_.resize(levels_);
     dH_.resize(levels_);
-    hH_.resize(levels_);
     ds1_.resize(levels_);
     ds2_.resize(levels_);
-    hs1_.resize(levels_);
-    hs2_.resize(levels_);
     for (int n = 0; n < levels_; ++n) {
       int64_t N = N_[n];
       maps_[n].resize(size_t(B_) * N);
@@ -437,24 +390,17 @@ class EngineT final : public Engine {
       gpart2_[n].resize(size_t(nb) * B_ * kL * E);
       gsum_[n].resize(size_t(B_) * kL * E * 2);
       gsum2_[n].resize(size_t(B_) * kL * E);
-      hgsum_[n].resize(gsum_[n].size());
-      hgsum2_[n].resize(gsum2_[n].size());
       int nz = int(std::min<int64_t>((N + 255) / 256, 256));
       nblk_z_.push_back(nz);
       zpart_[n].resize(size_t(nz) * B_ * kL * 4);
       zsum_[n].resize(size_t(B_) * kL * 4);
-      hzsum_[n].resize(zsum_[n].size());
       dH_[n].resize(size_t(B_) * kL * K * K);
-      hH_[n].resize(size_t(B_) * kL * K * K);
       ds1_[n].resize(size_t(B_) * kL);
       ds2_[n].resize(size_t(B_) * kL);
-      hs1_[n].resize(size_t(B_) * kL);
-      hs2_[n].resize(size_t(B_) * kL);
     }
     nblk_mv_ = int(std::min<int64_t>((N_[0] + 255) / 256, 256));
     mvpart_.resize(size_t(nblk_mv_) * B_ * 4);
     mvsum_.resize(size_t(B_) * 4);
-    hmvsum_.resize(mvsum_.size());
     gbr_.resize(br_[0].size());
     for (int n = 1; n < levels_; ++n) gbr_.resize(std::max(gbr_.size(), br_[n].size()));
     gzp_.resize(z_[0].size());
@@ -465,8 +411,7 @@ class EngineT final : public Engine {
     mg_[1].resize(maps_[0].size());
     dgm_.resize(B_);
     dgv_.resize(B_);
-    hgm_.assign(B_, 0.0);
-    hgv_.assign(B_, 0.0);
+    build_tables();
   }
 
   void cast_in(const double* x, T* out, int64_t n) {
@@ -487,12 +432,12 @@ class EngineT final : public Engine {
 
   void dec_maps(int n) {
     int id = prof_.start();
-    launch_dec_fwd<T>(dec_[n - 1].view(), 1, maps_[n - 1].data(), B_, maps_[n].data(), s_);
+    launch_dec_fwd<T>(dec_[n - 1]->store.view(), 1, maps_[n - 1].data(), B_, maps_[n].data(), s_);
     prof_.stop(id, tag("dec_fwd", n), dec_bytes(n - 1, 1, B_));
   }
 
   double conv_bytes(int n, int NI, int rows, bool adj) const {
-    double slots = adj ? conv_tslots_[n] : conv_slots_[n];
+    double slots = adj ? conv_[n]->tslots : conv_[n]->slots;
     double coef = slots * (2 * kL * sizeof(T) + 4);
     double in = adj ? double(rows) * kL * N_[n] * NI * 2 * sizeof(T) : double(rows) * N_[n] * NI * sizeof(T);
     double out = adj ? double(rows) * N_[n] * NI * sizeof(T) : double(rows) * kL * N_[n] * NI * 2 * sizeof(T);
@@ -500,36 +445,36 @@ class EngineT final : public Engine {
   }
 
   double dec_bytes(int n, int NI, int rows, bool adj = false) const {
-    double slots = adj ? dec_tslots_[n] : dec_slots_[n];
+    double slots = adj ? dec_[n]->tslots : dec_[n]->slots;
     return slots * (sizeof(T) + 4) + double(rows) * (N_[n] + N_[n + 1]) * NI * sizeof(T) * (adj ? 1.5 : 1.0);
   }
 
   void conv_adj(int n, int NI, const C2* gy, int rows, T* gx) {
     int id = prof_.start();
-    launch_conv_adj<T>(conv_[n].view(), NI, gy, rows, gx, s_);
+    launch_conv_adj<T>(conv_[n]->store.view(), NI, gy, rows, gx, s_);
     prof_.stop(id, tag(NI == 1 ? "conv_adj_z" : "conv_adj_branch", n), conv_bytes(n, NI, rows, true));
   }
 
   void dec_adj(int n, int NI, const T* gy, const T* add, int rows, T* gx) {
     int id = prof_.start();
-    launch_dec_adj<T>(dec_[n].view(), NI, gy, add, rows, gx, s_);
+    launch_dec_adj<T>(dec_[n]->store.view(), NI, gy, add, rows, gx, s_);
     prof_.stop(id, tag("dec_adj", n), dec_bytes(n, NI, rows, true));
   }
 
-  void sweep_forward(bool moments, cplx* values, double* variance) {
+  void sweep_forward(bool moments) {
     for (int n = 0; n < levels_; ++n) {
       const int64_t N = N_[n];
       int id = prof_.start();
-      launch_conv_fwd<T>(conv_[n].view(), 1, maps_[n].data(), B_, z_[n].data(),
+      launch_conv_fwd<T>(conv_[n]->store.view(), 1, maps_[n].data(), B_, z_[n].data(),
                          M_[n].data() + size_t(n) * B_ * N * kL, s_);
       prof_.stop(id, tag("conv_fwd_z", n), conv_bytes(n, 1, B_, false) + double(B_) * N * kL * sizeof(T));
       if (n > 0) {
         id = prof_.start();
-        launch_dec_fwd<T>(dec_[n - 1].view(), kL, M_[n - 1].data(), n * B_, M_[n].data(), s_);
+        launch_dec_fwd<T>(dec_[n - 1]->store.view(), kL, M_[n - 1].data(), n * B_, M_[n].data(), s_);
         prof_.stop(id, tag("dec_fwd", n), dec_bytes(n - 1, kL, n * B_));
       }
       id = prof_.start();
-      launch_conv_fwd<T>(conv_[n].view(), kL, M_[n].data(), (n + 1) * B_, br_[n].data(), nullptr, s_);
+      launch_conv_fwd<T>(conv_[n]->store.view(), kL, M_[n].data(), (n + 1) * B_, br_[n].data(), nullptr, s_);
       prof_.stop(id, tag("conv_fwd_branch", n), conv_bytes(n, kL, (n + 1) * B_, false));
       id = prof_.start();
       int nb = nblk_gram_[n];
@@ -546,59 +491,175 @@ class EngineT final : public Engine {
         launch_reduce_parts(mvpart_.data(), nblk_mv_, int64_t(B_) * 4, mvsum_.data(), s_);
       }
       prof_.stop(id, tag("zstats", n), double(B_) * kL * N * 2 * sizeof(T));
-      STL_HIP(hipMemcpyAsync(hgsum_[n].data(), gsum_[n].data(), gsum_[n].bytes(), hipMemcpyDeviceToHost, s_));
-      if (moments)
-        STL_HIP(hipMemcpyAsync(hgsum2_[n].data(), gsum2_[n].data(), gsum2_[n].bytes(), hipMemcpyDeviceToHost, s_));
-      STL_HIP(hipMemcpyAsync(hzsum_[n].data(), zsum_[n].data(), zsum_[n].bytes(), hipMemcpyDeviceToHost, s_));
     }
-    STL_HIP(hipMemcpyAsync(hmvsum_.data(), mvsum_.data(), mvsum_.bytes(), hipMemcpyDeviceToHost, s_));
-    STL_HIP(hipStreamSynchronize(s_));
-    prof_.collect();
-    size_t np = plan_.size();
-    for (int b = 0; b < B_; ++b)
-      for (size_t i = 0; i < np; ++i) {
-        const SumSlot& sl = slots_[i];
-        cplx v;
-        double second = 0.0;
-        switch (sl.kind) {
-          case SumSlot::MEANVAR: {
-            double Nd = double(N_[0]);
-            v = cplx(hmvsum_[b * 4 + sl.idx] / Nd, 0.0);
-            second = hmvsum_[b * 4 + 2 + sl.idx] / Nd;
-            break;
-          }
-          case SumSlot::ZSTAT: {
-            double Nd = double(N_[sl.n]);
-            const double* z = &hzsum_[sl.n][size_t(b * kL + sl.l) * 4];
-            v = cplx(z[sl.idx] / Nd, 0.0);
-            second = z[2 + sl.idx] / Nd;
-            break;
-          }
-          case SumSlot::GRAM: {
-            double Nd = double(N_[sl.n]);
-            int E = gram_e(sl.n);
-            size_t e = size_t(b * kL + sl.l) * E + sl.idx;
-            double re = hgsum_[sl.n][2 * e], im = hgsum_[sl.n][2 * e + 1];
-            v = cplx(re / Nd, (sl.conj ? -im : im) / Nd);
-            if (moments) second = hgsum2_[sl.n][e] / Nd;
-            break;
-          }
-        }
-        values[b * np + i] = v;
-        if (moments && variance) variance[b * np + i] = std::max(second - std::norm(v), 0.0);
+    const size_t nv = size_t(B_) * plan_.size();
+    launch_assemble(int(plan_.size()), B_, dstat_.data(), dsrc_.data(), moments, dvalues_.data(), dvar_.data(), s_);
+    STL_HIP(hipMemcpyAsync(hvalues_.data(), dvalues_.data(), nv * sizeof(double2), hipMemcpyDeviceToHost, s_));
+    if (moments)
+      STL_HIP(hipMemcpyAsync(hvar_.data(), dvar_.data(), nv * sizeof(double), hipMemcpyDeviceToHost, s_));
+  }
+
+  void enqueue_forward(const double* x_dev, bool moments) {
+    load_maps(x_dev);
+    for (int n = 1; n < levels_; ++n) dec_maps(n);
+    sweep_forward(moments);
+  }
+
+  // After the forward's stream work has completed: statistics out of the pinned staging.
+  void finish_forward(bool moments, cplx* values, double* variance) override {
+    const size_t nv = size_t(B_) * plan_.size();
+    std::memcpy(values, hvalues_.data(), nv * sizeof(double2));
+    if (moments && variance) std::memcpy(variance, hvar_.data(), nv * sizeof(double));
+  }
+
+  // Runs body's stream work: eagerly with per-kernel timers, else as a HIP
+  // graph captured at the first call with these arguments (kernel arguments
+  // are fixed in a graph, so it is keyed by the device pointer and the flag)
+  // and replayed afterwards; returns after the stream has completed.
+  struct GraphEntry {
+    const void* ptr;
+    bool flag;
+    hipGraphExec_t exec;
+  };
+  static constexpr size_t kMaxGraphs = 8;
+
+  template <typename Body>
+  hipGraph_t capture(Body&& body) {
+    hipGraph_t graph = nullptr;
+    STL_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+    try {
+      body();
+    } catch (...) {
+      if (hipStreamEndCapture(s_, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    STL_HIP(hipStreamEndCapture(s_, &graph));
+    return graph;
+  }
+
+  template <typename Body>
+  void run(std::vector<GraphEntry>& cache, const void* ptr, bool flag, Body&& body) {
+    if (prof_.on()) {
+      body();
+      STL_HIP(hipStreamSynchronize(s_));
+      prof_.collect();
+      return;
+    }
+    hipGraphExec_t exec = nullptr;
+    for (auto& e : cache)
+      if (e.ptr == ptr && e.flag == flag) exec = e.exec;
+    if (!exec) {
+      hipGraph_t graph = capture(body);
+      hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      STL_HIP(err);
+      if (cache.size() == kMaxGraphs) {
+        (void)hipGraphExecDestroy(cache.front().exec);
+        cache.erase(cache.begin());
       }
+      cache.push_back(GraphEntry{ptr, flag, exec});
+    }
+    STL_HIP(hipGraphLaunch(exec, s_));
+    STL_HIP(hipStreamSynchronize(s_));
+  }
+
+  // Device tables of the statistics assembly and of the cotangent scatter.
+  void build_tables() {
+    const int np = int(plan_.size());
+    std::vector<const double*> src{mvsum_.data()};
+    std::vector<double*> dst{dgm_.data(), dgv_.data()};
+    for (int n = 0; n < levels_; ++n) {
+      src.push_back(gsum_[n].data());
+      src.push_back(gsum2_[n].data());
+      src.push_back(zsum_[n].data());
+      dst.push_back(reinterpret_cast<double*>(dH_[n].data()));
+      dst.push_back(ds1_[n].data());
+      dst.push_back(ds2_[n].data());
+    }
+    std::vector<StatEntry> st(np);
+    std::vector<RevTarget> targets;
+    std::vector<int> hbase(levels_), s1base(levels_), s2base(levels_);
+    targets.push_back(RevTarget{0, 0, 0, 0, 0, 1});
+    targets.push_back(RevTarget{1, 0, 0, 0, 0, 1});
+    for (int n = 0; n < levels_; ++n) {
+      const int K = gram_k(n);
+      hbase[n] = int(targets.size());
+      for (int e = 0; e < kL * K * K; ++e) targets.push_back(RevTarget{2 + 3 * n, 1, 0, 0, 2 * e, 2 * kL * K * K});
+      s1base[n] = int(targets.size());
+      for (int l = 0; l < kL; ++l) targets.push_back(RevTarget{3 + 3 * n, 0, 0, 0, l, kL});
+      s2base[n] = int(targets.size());
+      for (int l = 0; l < kL; ++l) targets.push_back(RevTarget{4 + 3 * n, 0, 0, 0, l, kL});
+    }
+    std::vector<std::vector<RevTerm>> lists(targets.size());
+    for (int i = 0; i < np; ++i) {
+      const SumSlot& sl = slots_[i];
+      const int n = sl.n;
+      StatEntry& e = st[i];
+      e = StatEntry{};
+      switch (sl.kind) {
+        case SumSlot::MEANVAR: {
+          const double Nd = double(N_[0]);
+          e.arr = 0; e.re = sl.idx; e.im = -1; e.bs = 4;
+          e.sq_arr = 0; e.sq = 2 + sl.idx; e.sbs = 4; e.Nd = Nd;
+          lists[sl.idx].push_back(RevTerm{i, 0, sl.idx == 0 ? 1.0 : 2.0, Nd});
+          break;
+        }
+        case SumSlot::ZSTAT: {
+          const double Nd = double(N_[n]);
+          e.arr = 3 + 3 * n; e.re = sl.l * 4 + sl.idx; e.im = -1; e.bs = kL * 4;
+          e.sq_arr = 3 + 3 * n; e.sq = sl.l * 4 + 2 + sl.idx; e.sbs = kL * 4; e.Nd = Nd;
+          lists[(sl.idx == 0 ? s1base[n] : s2base[n]) + sl.l].push_back(RevTerm{i, 0, sl.idx == 0 ? 1.0 : 2.0, Nd});
+          break;
+        }
+        case SumSlot::GRAM: {
+          const double Nd = double(N_[n]);
+          const int K = gram_k(n), E = gram_e(n);
+          const int64_t el = int64_t(sl.l) * E + sl.idx;
+          e.arr = 1 + 3 * n; e.re = 2 * el; e.im = 2 * el + 1; e.bs = int64_t(kL) * E * 2; e.conj = sl.conj;
+          e.sq_arr = 2 + 3 * n; e.sq = el; e.sbs = int64_t(kL) * E; e.Nd = Nd;
+          lists[hbase[n] + (sl.l * K + sl.gi) * K + sl.gj].push_back(RevTerm{i, 0, 1.0, Nd});
+          lists[hbase[n] + (sl.l * K + sl.gj) * K + sl.gi].push_back(RevTerm{i, 1, 1.0, Nd});
+          break;
+        }
+      }
+    }
+    std::vector<RevTerm> terms;
+    for (size_t t = 0; t < targets.size(); ++t) {
+      targets[t].first = int32_t(terms.size());
+      targets[t].count = int32_t(lists[t].size());
+      terms.insert(terms.end(), lists[t].begin(), lists[t].end());
+    }
+    nrt_ = int(targets.size());
+    upload(dstat_, st);
+    upload(dsrc_, src);
+    upload(drt_, targets);
+    upload(drk_, terms);
+    upload(ddst_, dst);
+    const size_t nv = size_t(B_) * np;
+    dvalues_.resize(nv);
+    dvar_.resize(nv);
+    dcot_.resize(nv);
+    hvalues_.resize(nv);
+    hvar_.resize(nv);
+    hcot_.resize(nv);
+  }
+
+  template <typename U>
+  void upload(DevArray<U>& d, const std::vector<U>& h) {
+    d.resize(std::max<size_t>(h.size(), 1));
+    if (!h.empty()) STL_HIP(hipMemcpy(d.data(), h.data(), h.size() * sizeof(U), hipMemcpyHostToDevice));
   }
 
   EngineOptions opt_;
   hipStream_t s_ = nullptr;
+  std::vector<GraphEntry> fwd_graphs_, vjp_graphs_;
   Profiler prof_;
   std::vector<Coefficient> plan_;
   std::vector<SumSlot> slots_;
   int levels_ = 0, B_ = 1;
   std::vector<int64_t> N_;
-  std::vector<ConvStore<T>> conv_;
-  std::vector<DecStore<T>> dec_;
-  std::vector<double> conv_slots_, conv_tslots_, dec_slots_, dec_tslots_;
+  std::vector<std::shared_ptr<const SharedConv<T>>> conv_;
+  std::vector<std::shared_ptr<const SharedDec<T>>> dec_;
   size_t geometry_bytes_ = 0;
   double setup_s_ = 0.0;
   // forward state
@@ -609,17 +670,23 @@ class EngineT final : public Engine {
   int nblk_mv_ = 1;
   std::vector<DevArray<double>> gpart_, gpart2_, gsum_, gsum2_, zpart_, zsum_;
   DevArray<double> mvpart_, mvsum_;
-  std::vector<std::vector<double>> hgsum_, hgsum2_, hzsum_;
-  std::vector<double> hmvsum_;
   // reverse
   DevArray<C2> gbr_, gzp_, gz_;
   DevArray<T> acc_[2], mg_[2];
   std::vector<DevArray<double2>> dH_;
-  std::vector<std::vector<double2>> hH_;
   std::vector<DevArray<double>> ds1_, ds2_;
-  std::vector<std::vector<double>> hs1_, hs2_;
   DevArray<double> dgm_, dgv_;
-  std::vector<double> hgm_, hgv_;
+  // device statistics assembly and cotangent scatter
+  int nrt_ = 0;
+  DevArray<StatEntry> dstat_;
+  DevArray<const double*> dsrc_;
+  DevArray<RevTarget> drt_;
+  DevArray<RevTerm> drk_;
+  DevArray<double*> ddst_;
+  DevArray<double2> dvalues_, dcot_;
+  DevArray<double> dvar_;
+  PinnedArray<double2> hvalues_, hcot_;
+  PinnedArray<double> hvar_;
 };
 
 }  // namespace

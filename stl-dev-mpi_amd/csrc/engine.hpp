@@ -59,6 +59,14 @@ class Engine {
   virtual void forward_levels_device(const double* x_dev, cplx* values) = 0;
   // Map cotangent [B][12 nside^2] (device, float64) of Re sum(conj(cot) values) after forward.
   virtual void vjp_device(const cplx* cot, double* grad_dev) = 0;
+  // The stream work of forward_device / vjp_device as an uninstantiated HIP
+  // graph (the caller owns it), for composition with other engines' graphs:
+  // run it, then finish_forward copies the statistics out; stage_cotangent
+  // must precede every run of a captured VJP.  Not with per-kernel timing.
+  virtual hipGraph_t capture_forward(const double* x_dev, bool moments) = 0;
+  virtual void finish_forward(bool moments, cplx* values, double* variance) = 0;
+  virtual void stage_cotangent(const cplx* cot) = 0;
+  virtual hipGraph_t capture_vjp(double* grad_dev) = 0;
   virtual hipStream_t stream() const = 0;
   virtual size_t device_bytes() const = 0;
   virtual double setup_seconds() const = 0;

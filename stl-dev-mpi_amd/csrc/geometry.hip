@@ -22,6 +22,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "healpix_device.hpp"
@@ -676,5 +680,65 @@ void build_dec(int order_f, DecStore<double>* d64, DecStore<float>* d32, hipStre
   }
   STL_HIP(hipStreamSynchronize(s));
 }
+
+namespace {
+
+double sum_u8(const DevArray<uint8_t>& cnt) {
+  std::vector<uint8_t> h(cnt.size());
+  STL_HIP(hipMemcpy(h.data(), cnt.data(), cnt.bytes(), hipMemcpyDeviceToHost));
+  double s = 0.0;
+  for (auto v : h) s += v;
+  return s;
+}
+
+std::mutex& cache_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+// (device, fp64, order) -> operators still held by some engine.
+template <typename Op>
+std::map<std::tuple<int, bool, int>, std::weak_ptr<const Op>>& cache() {
+  static std::map<std::tuple<int, bool, int>, std::weak_ptr<const Op>> c;
+  return c;
+}
+
+}  // namespace
+
+template <typename T>
+std::shared_ptr<const SharedConv<T>> shared_conv(int device, int order, hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(cache_mutex());
+  auto key = std::make_tuple(device, std::is_same<T, double>::value, order);
+  auto& c = cache<SharedConv<T>>();
+  if (auto hit = c[key].lock()) return hit;
+  auto op = std::make_shared<SharedConv<T>>();
+  static const std::vector<double> ker = wavelet_kernel_f32(5);
+  if constexpr (std::is_same<T, double>::value) build_conv(order, ker, &op->store, nullptr, stream);
+  else build_conv(order, ker, nullptr, &op->store, stream);
+  op->slots = sum_u8(op->store.cnt);
+  op->tslots = sum_u8(op->store.tcnt);
+  c[key] = op;
+  return op;
+}
+
+template <typename T>
+std::shared_ptr<const SharedDec<T>> shared_dec(int device, int order_f, hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(cache_mutex());
+  auto key = std::make_tuple(device, std::is_same<T, double>::value, order_f);
+  auto& c = cache<SharedDec<T>>();
+  if (auto hit = c[key].lock()) return hit;
+  auto op = std::make_shared<SharedDec<T>>();
+  if constexpr (std::is_same<T, double>::value) build_dec(order_f, &op->store, nullptr, stream);
+  else build_dec(order_f, nullptr, &op->store, stream);
+  op->slots = sum_u8(op->store.cnt);
+  op->tslots = sum_u8(op->store.tcnt);
+  c[key] = op;
+  return op;
+}
+
+template std::shared_ptr<const SharedConv<double>> shared_conv<double>(int, int, hipStream_t);
+template std::shared_ptr<const SharedConv<float>> shared_conv<float>(int, int, hipStream_t);
+template std::shared_ptr<const SharedDec<double>> shared_dec<double>(int, int, hipStream_t);
+template std::shared_ptr<const SharedDec<float>> shared_dec<float>(int, int, hipStream_t);
 
 }  // namespace stl

@@ -1,9 +1,11 @@
 // Device-built pyramid operators: the wavelet convolution of every level
 // (folded "kernel x bilinear weight" coefficients per distinct cell) and the
 // Gaussian-cone decimation between levels, each with its transpose for the
-// gather-form adjoint.  Built once per engine, stored for the rep faces only.
+// gather-form adjoint.  Built once per device and order, stored for the rep
+// faces only.
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "common.hpp"
@@ -74,5 +76,30 @@ void build_conv(int order, const std::vector<double>& ker, ConvStore<double>* c6
 
 // Builds the decimation from the level of `order_f` to order_f - 1.
 void build_dec(int order_f, DecStore<double>* d64, DecStore<float>* d32, hipStream_t stream);
+
+}  // namespace stl
+
+namespace stl {
+
+// Operators shared by every engine of the process on one device: the
+// operators of a level depend only on its order (and the precision), so
+// engines of different nside reuse each other's levels.  Built on the
+// caller's stream at the first request (and completed before returning);
+// freed when the last engine holding them goes.
+template <typename T>
+struct SharedConv {
+  ConvStore<T> store;
+  double slots = 0.0, tslots = 0.0;  // stored (transposed) slots summed over rep rows
+};
+template <typename T>
+struct SharedDec {
+  DecStore<T> store;
+  double slots = 0.0, tslots = 0.0;
+};
+
+template <typename T>
+std::shared_ptr<const SharedConv<T>> shared_conv(int device, int order, hipStream_t stream);
+template <typename T>
+std::shared_ptr<const SharedDec<T>> shared_dec(int device, int order_f, hipStream_t stream);
 
 }  // namespace stl

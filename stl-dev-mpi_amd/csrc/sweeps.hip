@@ -19,7 +19,6 @@ namespace {
 inline unsigned nblocks(int64_t n, int t) { return unsigned((n + t - 1) / t); }
 
 // Gram entries per thread when there are more entries than threads.
-constexpr int kGramM = 5;
 
 template <typename T>
 struct Load;
@@ -443,101 +442,108 @@ __device__ inline double2 to_d2(cplx_t<T> v) {
 }
 
 // Gram partial sums: block (tile stride, b * kL + l3).
+// Gram products of one (b, l3) per workgroup row.  A tile of V [TPX][KP]
+// (columns K..KP-1 zero) is staged in LDS; each thread owns a 4 x 4 block
+// (bi <= bj) of entries V_i conj(V_j) and a group of the tile's pixels, so
+// every LDS read feeds 4 products.  The values widen exactly to FP64; the
+// products and their sums are FP64.  The per-group sums are added in group
+// order: deterministic.
 template <typename T, bool MOM>
 __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
                                               const cplx_t<T>* __restrict__ z, int TPX, double* __restrict__ part,
                                               double* __restrict__ part2) {
   extern __shared__ double2 smem[];
   const int K = (n + 1) * kL + 1;
+  const int KP = (K + 3) & ~3, NR = KP / 4, NBLK = NR * (NR + 1) / 2;
   const int E = K * (K + 1) / 2;
   double2* Vs = smem;
-  short* ei = reinterpret_cast<short*>(Vs + size_t(TPX) * K);
-  short* ej = ei + E;
-  int bl = blockIdx.y;
-  int b = bl / kL, l3 = bl % kL;
-  int tid = threadIdx.x;
-  for (int e = tid; e < E; e += blockDim.x) {
-    int i = 0, rem = e;
-    while (rem >= K - i) { rem -= K - i; ++i; }
-    ei[e] = short(i);
-    ej[e] = short(i + rem);
-  }
-  const bool wide = E >= 256;
-  const int G = wide ? 1 : 256 / E;
-  const int grp = wide ? 0 : tid / E;
-  const bool act = wide || grp < G;
-  const int e0 = wide ? tid : tid % E;
-  double2 acc[kGramM];
-  double acc2[kGramM];
+  const int bl = blockIdx.y;
+  const int b = bl / kL, l3 = bl % kL;
+  const int tid = threadIdx.x;
+  const int G = 256 / NBLK;
+  const int blk = tid % NBLK, grp = tid / NBLK;
+  const bool act = grp < G;
+  int bi = 0, rem = blk;
+  while (rem >= NR - bi) { rem -= NR - bi; ++bi; }
+  const int bj = bi + rem;
+  double are[4][4], aim[4][4], asq[4][4];
 #pragma unroll
-  for (int m = 0; m < kGramM; ++m) { acc[m] = make_double2(0.0, 0.0); acc2[m] = 0.0; }
-  __syncthreads();
-  int64_t ntiles = (N + TPX - 1) / TPX;
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) are[r][c] = aim[r][c] = asq[r][c] = 0.0;
+  const int64_t ntiles = (N + TPX - 1) / TPX;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    int64_t p0 = tile * TPX;
-    int tn = int(min<int64_t>(TPX, N - p0));
+    const int64_t p0 = tile * TPX;
+    const int tn = int(min<int64_t>(TPX, N - p0));
     for (int d = 0; d <= n; ++d) {
       const cplx_t<T>* src = br + ((size_t(d) * B + b) * kL + l3) * N * kL + size_t(p0) * kL;
       for (int idx = tid; idx < tn * kL; idx += blockDim.x) {
         int p = idx / kL, l1 = idx % kL;
-        Vs[p * K + d * kL + l1] = to_d2<T>(src[idx]);
+        Vs[p * KP + d * kL + l1] = to_d2<T>(src[idx]);
       }
     }
     const cplx_t<T>* zs = z + (size_t(b) * kL + l3) * N + p0;
-    for (int p = tid; p < tn; p += blockDim.x) Vs[p * K + K - 1] = to_d2<T>(zs[p]);
+    for (int p = tid; p < tn; p += blockDim.x) Vs[p * KP + K - 1] = to_d2<T>(zs[p]);
+    const int pad = KP - K;
+    for (int idx = tid; idx < tn * pad; idx += blockDim.x) Vs[(idx / pad) * KP + K + idx % pad] = make_double2(0.0, 0.0);
     __syncthreads();
     if (act) {
+      for (int p = grp; p < tn; p += G) {
+        double2 a[4], c[4];
 #pragma unroll
-      for (int m = 0; m < kGramM; ++m) {
-        int e = e0 + 256 * m;
-        if ((wide && e < E) || (!wide && m == 0)) {
-          int i = ei[e], j = ej[e];
-          double2 s = acc[m];
-          double s2 = acc2[m];
-          for (int p = grp; p < tn; p += G) {
-            double2 a = Vs[p * K + i], c = Vs[p * K + j];
-            double re = a.x * c.x + a.y * c.y;
-            double im = a.y * c.x - a.x * c.y;
-            s.x += re;
-            s.y += im;
-            if (MOM) s2 += re * re + im * im;
-          }
-          acc[m] = s;
-          acc2[m] = s2;
+        for (int r = 0; r < 4; ++r) {
+          a[r] = Vs[p * KP + 4 * bi + r];
+          c[r] = Vs[p * KP + 4 * bj + r];
         }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            double re = a[r].x * c[q].x + a[r].y * c[q].y;
+            double im = a[r].y * c[q].x - a[r].x * c[q].y;
+            are[r][q] += re;
+            aim[r][q] += im;
+            if (MOM) asq[r][q] += re * re + im * im;
+          }
       }
     }
     __syncthreads();
   }
-  size_t base = (size_t(blockIdx.x) * gridDim.y + bl) * E;
-  if (wide) {
+  // Sum the pixel groups: one (r, q) entry of every block per round.
+  double2* red = smem;                                          // [256]
+  double* red2 = reinterpret_cast<double*>(smem + 256);         // [256]
+  double2* fin = smem + 256 + 128;                              // [NBLK][16]
+  double* fin2 = reinterpret_cast<double*>(fin + NBLK * 16);    // [NBLK][16]
 #pragma unroll
-    for (int m = 0; m < kGramM; ++m) {
-      int e = e0 + 256 * m;
-      if (e < E) {
-        part[(base + e) * 2] = acc[m].x;
-        part[(base + e) * 2 + 1] = acc[m].y;
-        if (MOM) part2[base + e] = acc2[m];
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      red[tid] = make_double2(are[r][q], aim[r][q]);
+      if (MOM) red2[tid] = asq[r][q];
+      __syncthreads();
+      if (tid < NBLK) {
+        double2 s = make_double2(0.0, 0.0);
+        double s2 = 0.0;
+        for (int g = 0; g < G; ++g) {
+          s.x += red[g * NBLK + tid].x;
+          s.y += red[g * NBLK + tid].y;
+          if (MOM) s2 += red2[g * NBLK + tid];
+        }
+        fin[tid * 16 + r * 4 + q] = s;
+        if (MOM) fin2[tid * 16 + r * 4 + q] = s2;
       }
+      __syncthreads();
     }
-  } else {
-    double2* red = Vs;
-    double* red2 = reinterpret_cast<double*>(Vs + 256);
-    red[tid] = acc[0];
-    red2[tid] = acc2[0];
-    __syncthreads();
-    if (tid < E) {
-      double2 s = make_double2(0.0, 0.0);
-      double s2 = 0.0;
-      for (int g = 0; g < G; ++g) {
-        s.x += red[g * E + tid].x;
-        s.y += red[g * E + tid].y;
-        s2 += red2[g * E + tid];
-      }
-      part[(base + tid) * 2] = s.x;
-      part[(base + tid) * 2 + 1] = s.y;
-      if (MOM) part2[base + tid] = s2;
-    }
+  const size_t base = (size_t(blockIdx.x) * gridDim.y + bl) * E;
+  for (int e = tid; e < E; e += blockDim.x) {
+    int i = 0, re = e;
+    while (re >= K - i) { re -= K - i; ++i; }
+    const int j = i + re;
+    const int ib = i / 4, jb = j / 4;
+    const int k = (ib * NR - ib * (ib - 1) / 2 + (jb - ib)) * 16 + (i % 4) * 4 + (j % 4);
+    part[(base + e) * 2] = fin[k].x;
+    part[(base + e) * 2 + 1] = fin[k].y;
+    if (MOM) part2[base + e] = fin2[k];
   }
 }
 
@@ -592,12 +598,63 @@ __global__ __launch_bounds__(256) void k_meanvar(int64_t N, const T* __restrict_
     for (int c = 0; c < 4; ++c) part[(size_t(blockIdx.x) * gridDim.y + b) * 4 + c] = v[c];
 }
 
-__global__ void k_reduce_parts(const double* __restrict__ part, int nblk, int64_t m, double* __restrict__ out) {
-  int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (i >= m) return;
+// part [nblk][m] -> out [m]: 64 outputs per workgroup, wave w adding the
+// blocks b = w, w + 16, ... in order, then the 16 wave sums in order.
+constexpr int kRedWaves = 16;
+__global__ __launch_bounds__(64 * kRedWaves) void k_reduce_parts(const double* __restrict__ part, int nblk,
+                                                                 int64_t m, double* __restrict__ out) {
+  __shared__ double sh[kRedWaves][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t i = blockIdx.x * int64_t(64) + lane;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += part[size_t(b) * m + i];
-  out[i] = s;
+  if (i < m) {
+#pragma unroll 8
+    for (int b = w; b < nblk; b += kRedWaves) s += part[size_t(b) * m + i];
+  }
+  sh[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && i < m) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < kRedWaves; ++k) t += sh[k][lane];
+    out[i] = t;
+  }
+}
+
+__global__ void k_assemble(int np, int B, const StatEntry* __restrict__ entries, const double* const* src,
+                           bool moments, double2* __restrict__ values, double* __restrict__ variance) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(np) * B) return;
+  const int b = int(t / np), i = int(t % np);
+  const StatEntry e = entries[i];
+  const double* a = src[e.arr];
+  double re = a[e.re + b * e.bs] / e.Nd;
+  double im = e.im < 0 ? 0.0 : a[e.im + b * e.bs] / e.Nd;
+  if (e.conj) im = -im;
+  values[t] = make_double2(re, im);
+  if (moments && variance != nullptr) {
+    double second = e.sq_arr < 0 ? 0.0 : src[e.sq_arr][e.sq + b * e.sbs] / e.Nd;
+    variance[t] = fmax(second - (re * re + im * im), 0.0);
+  }
+}
+
+__global__ void k_scatter_cot(int ntargets, int B, int np, const RevTarget* __restrict__ targets,
+                              const RevTerm* __restrict__ terms, const double2* __restrict__ cot,
+                              double* const* dst) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(ntargets) * B) return;
+  const int b = int(t / ntargets);
+  const RevTarget r = targets[t % ntargets];
+  double re = 0.0, im = 0.0;
+  for (int k = r.first; k < r.first + r.count; ++k) {
+    const RevTerm m = terms[k];
+    double2 g = cot[size_t(b) * np + m.coef];
+    re += (m.mult * g.x) / m.Nd;
+    im += (m.mult * (m.conj ? -g.y : g.y)) / m.Nd;
+  }
+  double* d = dst[r.arr] + r.off + b * r.bs;
+  d[0] = re;
+  if (r.cplx) d[1] = im;
 }
 
 template <typename T, int NL>
@@ -747,22 +804,16 @@ int gram_tile(int n) {
   return std::max(16, std::min(256, tpx));
 }
 
-size_t gram_lds(int n) {
-  int K = (n + 1) * kL + 1;
-  int E = K * (K + 1) / 2;
-  size_t v = size_t(gram_tile(n)) * K * sizeof(double2);
-  size_t red = 256 * (sizeof(double2) + sizeof(double));
-  return std::max(v, red) + 2 * size_t(E) * sizeof(short) + 16;
-}
 
 template <typename T>
 void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, bool moments, int nblk,
                  double* part, double* part2, hipStream_t s) {
-  int K = (n + 1) * kL + 1;
-  STL_CHECK(K * (K + 1) / 2 <= 256 * kGramM, "gram: too many levels");
+  const int K = (n + 1) * kL + 1, KP = (K + 3) & ~3, NR = KP / 4, NBLK = NR * (NR + 1) / 2;
+  STL_CHECK(NBLK <= 256, "gram: too many levels");
   dim3 grid(nblk, B * kL);
-  size_t lds = gram_lds(n);
-  int tpx = gram_tile(n);
+  const int tpx = gram_tile(n);
+  size_t lds = std::max(size_t(tpx) * KP * sizeof(double2),
+                        (256 + 128) * sizeof(double2) + size_t(NBLK) * 16 * (sizeof(double2) + sizeof(double)));
   if (moments) k_gram<T, true><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
   else k_gram<T, false><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
   STL_HIP(hipGetLastError());
@@ -780,8 +831,20 @@ void launch_meanvar(int64_t N, int B, const T* x, int nblk, double* part, hipStr
   STL_HIP(hipGetLastError());
 }
 
+void launch_assemble(int np, int B, const StatEntry* entries, const double* const* src, bool moments,
+                     double2* values, double* variance, hipStream_t s) {
+  k_assemble<<<nblocks(int64_t(np) * B, 256), 256, 0, s>>>(np, B, entries, src, moments, values, variance);
+  STL_HIP(hipGetLastError());
+}
+
+void launch_scatter_cot(int ntargets, int B, int np, const RevTarget* targets, const RevTerm* terms,
+                        const double2* cot, double* const* dst, hipStream_t s) {
+  k_scatter_cot<<<nblocks(int64_t(ntargets) * B, 256), 256, 0, s>>>(ntargets, B, np, targets, terms, cot, dst);
+  STL_HIP(hipGetLastError());
+}
+
 void launch_reduce_parts(const double* part, int nblk, int64_t m, double* out, hipStream_t s) {
-  k_reduce_parts<<<nblocks(m, 256), 256, 0, s>>>(part, nblk, m, out);
+  k_reduce_parts<<<nblocks(m, 64), 64 * kRedWaves, 0, s>>>(part, nblk, m, out);
   STL_HIP(hipGetLastError());
 }
 

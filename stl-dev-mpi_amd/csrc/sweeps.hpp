@@ -58,6 +58,34 @@ void launch_meanvar(int64_t N, int B, const T* x, int nblk, double* part, hipStr
 // Sums the partials over blocks in block order: part [nblk][m] -> out [m].
 void launch_reduce_parts(const double* part, int nblk, int64_t m, double* out, hipStream_t s);
 
+// Statistics from the sky-wide sums, on the device: value i of map b is
+// (src[arr][re + b bs] + i src[arr][im + b bs]) / Nd (imaginary part 0 when
+// im < 0, negated when conj), and with moments the variance of its terms
+// max(src[sq_arr][sq + b sbs] / Nd - |value|^2, 0).
+struct StatEntry {
+  int32_t arr, sq_arr, conj, pad;
+  int64_t re, im, bs, sq, sbs;
+  double Nd;
+};
+void launch_assemble(int np, int B, const StatEntry* entries, const double* const* src, bool moments,
+                     double2* values, double* variance, hipStream_t s);
+
+// Coefficient cotangents -> the per-level reverse operands (H, s1, s2 and the
+// mean / var factors), on the device, in a fixed order: target t of map b is
+// the sum over its terms k of (mult_k (conj_k ? conj(g) : g)) / Nd_k with
+// g = cot[b np + coef_k], in the terms' order; written to dst[arr][off + b bs]
+// (and off + 1, the imaginary part, when cplx).
+struct RevTarget {
+  int32_t arr, cplx, first, count;
+  int64_t off, bs;
+};
+struct RevTerm {
+  int32_t coef, conj;
+  double mult, Nd;
+};
+void launch_scatter_cot(int ntargets, int B, int np, const RevTarget* targets, const RevTerm* terms,
+                        const double2* cot, double* const* dst, hipStream_t s);
+
 // Reverse of the Gram products: gV = H V per pixel and (b, l3), H [B * kL][K][K]
 // complex (double2); writes the branch cotangents [(d B + b)][l3][N][l1] and
 // the z part [B][kL][N].

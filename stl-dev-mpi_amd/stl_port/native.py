@@ -20,6 +20,7 @@ import numpy as np
 _DOUBLE = C.POINTER(C.c_double)
 _FLOAT = C.POINTER(C.c_float)
 _UINT32 = C.POINTER(C.c_uint32)
+_INT32 = C.POINTER(C.c_int32)
 _UINT64 = C.POINTER(C.c_uint64)
 _INT64 = C.POINTER(C.c_int64)
 _SIZE = C.POINTER(C.c_size_t)
@@ -93,6 +94,11 @@ SIGNATURES = {
     "stlx_apply_operator": ([_VOID, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64], C.c_int),
     "stlx_synthesize": ([_VOID, C.c_uint64, C.c_size_t, _DOUBLE, _DOUBLE, C.c_char_p, _DOUBLE, C.c_size_t],
                         C.c_int64),
+    "stlx_ensemble_new": ([_INT32, _INT32, _INT32, C.c_size_t, C.c_char_p], _VOID),
+    "stlx_ensemble_free": ([_VOID], None),
+    "stlx_ensemble_ncoef": ([_VOID, _SIZE], C.c_int),
+    "stlx_ensemble_forward_device": ([_VOID, _UINT64, C.c_size_t, C.c_bool, _DOUBLE, _DOUBLE], C.c_int),
+    "stlx_ensemble_vjp_device": ([_VOID, _DOUBLE, C.c_size_t, _UINT64, C.c_size_t], C.c_int),
 }
 
 
@@ -402,6 +408,93 @@ class NativeEngine:
     def close(self):
         if getattr(self, "_handle", None):
             self._lib.stlp_free(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *_):
+        self.close()
+
+
+class NativeEnsemble:
+    """Maps of uneven nside evaluated together on one GPU: groups of
+    (nside, batch, j) (j defaults to log2(nside) - 1), each forward and each
+    VJP one HIP graph launch over all groups.  forward_device takes one float64
+    device tensor [batch, 12 nside^2] per group and returns one complex
+    statistics array [batch, ncoeff_g] per group; vjp_device takes the
+    cotangents in the same shapes and writes the gradients into `out`."""
+
+    def __init__(self, groups, precision="mixed", device=0, library=None):
+        self._lib = _library(library)
+        self._handle = None
+        self._lock = threading.RLock()
+        norm = []
+        for g in groups:
+            nside, batch = int(g[0]), int(g[1])
+            j = int(g[2]) if len(g) > 2 else nside.bit_length() - 2
+            norm.append((nside, batch, j))
+        if not norm:
+            raise ValueError("an ensemble needs at least one group")
+        self.groups = norm
+        arr = lambda k: np.ascontiguousarray([g[k] for g in norm], dtype=np.int32)
+        self._nside, self._batch, self._j = arr(0), arr(1), arr(2)
+        opts = json.dumps(dict(precision=precision, device=int(device))).encode()
+        self._handle = self._lib.stlx_ensemble_new(self._nside.ctypes.data_as(_INT32), self._j.ctypes.data_as(_INT32),
+                                                   self._batch.ctypes.data_as(_INT32), len(norm), opts)
+        if not self._handle:
+            raise _error(self._lib)
+        nc = np.zeros(len(norm), dtype=np.uintp)
+        if self._lib.stlx_ensemble_ncoef(self._handle, nc.ctypes.data_as(_SIZE)):
+            raise _error(self._lib)
+        self.ncoeff = [int(v) for v in nc]
+        self.npix = [12 * g[0] ** 2 for g in norm]
+        self._sizes = [g[1] * n for g, n in zip(norm, self.ncoeff)]
+
+    def _addresses(self, arrays):
+        if len(arrays) != len(self.groups):
+            raise ValueError(f"expected {len(self.groups)} arrays, one per group")
+        out = np.zeros(len(arrays), dtype=np.uint64)
+        for k, (a, g) in enumerate(zip(arrays, self.groups)):
+            out[k], _ = _device_address(a, g[1] * 12 * g[0] ** 2)
+        return out
+
+    def forward_device(self, maps, moments=False):
+        with self._lock:
+            if not self._handle:
+                raise RuntimeError("native ensemble is closed")
+            addr = self._addresses(maps)
+            n = sum(self._sizes)
+            values = np.empty(n, dtype=np.complex128)
+            var = np.empty(n if moments else 1, dtype=np.float64)
+            if self._lib.stlx_ensemble_forward_device(self._handle, addr.ctypes.data_as(_UINT64), len(addr), moments,
+                                                      _pointer(values.view(np.float64)), _pointer(var)):
+                raise _error(self._lib)
+            split = np.cumsum(self._sizes)[:-1]
+            vals = [v.reshape(g[1], -1) for v, g in zip(np.split(values, split), self.groups)]
+            if not moments:
+                return vals
+            return vals, [v.reshape(g[1], -1) for v, g in zip(np.split(var, split), self.groups)]
+
+    def vjp_device(self, cotangents, out):
+        with self._lock:
+            if not self._handle:
+                raise RuntimeError("native ensemble is closed")
+            c = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.complex128).ravel() for v in cotangents]))
+            if c.size != sum(self._sizes):
+                raise ValueError("cotangent shapes do not match the ensemble's statistics")
+            addr = self._addresses(out)
+            if self._lib.stlx_ensemble_vjp_device(self._handle, _pointer(c.view(np.float64)), c.size,
+                                                  addr.ctypes.data_as(_UINT64), len(addr)):
+                raise _error(self._lib)
+            return out
+
+    def close(self):
+        if getattr(self, "_handle", None):
+            self._lib.stlx_ensemble_free(self._handle)
             self._handle = None
 
     def __del__(self):

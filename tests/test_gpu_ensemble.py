@@ -1,0 +1,76 @@
+"""Ensembles of maps of uneven nside (stlx_ensemble_*): one HIP graph launch
+over all groups per forward and per VJP, with operators shared between the
+groups' levels of equal nside.  Every group's statistics and gradients equal,
+bit for bit, those of a standalone engine of the same nside, batch and
+precision (the same kernels on the same operators), over repeated graph
+replays with changing inputs; the FP64 groups also match the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+GROUPS = [(16, 3), (32, 2), (64, 1)]  # (nside, batch); j = log2(nside) - 1
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def maps(seed):
+    rng = np.random.default_rng(seed)
+    return [rng.standard_normal((b, 12 * n * n)) for n, b in GROUPS]
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_ensemble_equals_standalone_engines(precision):
+    from stl_port import NativeEngine, NativeEnsemble
+    print(f"[gpu] ensemble {GROUPS} precision={precision}", flush=True)
+    with NativeEnsemble(GROUPS, precision=precision) as ens:
+        singles = [NativeEngine(n, n.bit_length() - 2, batch=b, precision=precision) for n, b in GROUPS]
+        rng = np.random.default_rng(5)
+        for it in range(3):
+            xs = maps(it % 2)
+            xd = [torch.from_numpy(x).cuda() for x in xs]
+            vals = ens.forward_device(xd)
+            cots = [rng.standard_normal(v.shape) + 1j * rng.standard_normal(v.shape) for v in vals]
+            out = [torch.empty_like(x) for x in xd]
+            ens.vjp_device(cots, out)
+            torch.cuda.synchronize()
+            for g, eng in enumerate(singles):
+                v, _ = eng.forward(xs[g], generation=it)
+                np.testing.assert_array_equal(vals[g], v)
+                np.testing.assert_array_equal(out[g].cpu().numpy(), eng.vjp(cots[g], generation=it))
+        vals, var = ens.forward_device([torch.from_numpy(x).cuda() for x in maps(3)], moments=True)
+        for g, eng in enumerate(singles):
+            v, s = eng.forward(maps(3)[g], moments=True)
+            np.testing.assert_array_equal(vals[g], v)
+            np.testing.assert_array_equal(var[g], s)
+        for eng in singles:
+            eng.close()
+
+
+def test_ensemble_fp64_matches_oracle():
+    from stl_port import NativeEnsemble
+    xs = maps(7)
+    with NativeEnsemble(GROUPS, precision="fp64") as ens:
+        vals = ens.forward_device([torch.from_numpy(x).cuda() for x in xs])
+    for (n, b), x, v in zip(GROUPS, xs, vals):
+        ref = O.Engine(n, n.bit_length() - 2, B=b)
+        ref.load(x)
+        s = ref.forward()
+        assert np.abs(v - s).max() <= 1e-10 * np.abs(s).max()
+
+
+def test_ensemble_errors():
+    from stl_port import NativeEnsemble
+    with NativeEnsemble([(16, 1)]) as ens:
+        with pytest.raises(RuntimeError, match="no forward state"):
+            ens.vjp_device([np.zeros((1, ens.ncoeff[0]))], [torch.zeros((1, 12 * 256), dtype=torch.float64).cuda()])
+        with pytest.raises(ValueError):
+            ens.forward_device([torch.zeros((1, 12 * 255), dtype=torch.float64).cuda()])

@@ -251,3 +251,35 @@ def test_torch_autograd_matches_vjp_and_finite_differences():
             fd = (pairing(x0 + eps * v) - pairing(x0 - eps * v)).item() / (2 * eps)
         exact = float((g * v).sum())
         assert abs(fd - exact) <= 1e-6 * abs(exact), (fd, exact)
+
+
+def test_graph_replays_match_eager_launches():
+    """The forward and the VJP run as HIP graphs (captured per device pointer
+    and moments flag) unless per-kernel timing is on; replays, interleaved with
+    other work and other inputs, give the eager launches' bits."""
+    from stl_port import NativeEngine
+    nside, J = 32, 4
+    rng = np.random.default_rng(21)
+    xs = [rng.standard_normal((1, 12 * nside * nside)) for _ in range(2)]
+    with NativeEngine(nside, J, precision="mixed") as g, NativeEngine(nside, J, precision="mixed",
+                                                                      timing=True) as e:
+        cots = rng.standard_normal((1, g.ncoeff)) + 1j * rng.standard_normal((1, g.ncoeff))
+        xd = [torch.from_numpy(x).cuda() for x in xs]
+        out_g, out_e = torch.empty_like(xd[0]), torch.empty_like(xd[0])
+        for it, k in enumerate([0, 1, 0, 0, 1]):
+            moments = it % 2 == 1
+            vg, sg = g.forward(xs[k], generation=it, moments=moments)
+            ve, se = e.forward(xs[k], generation=it, moments=moments)
+            np.testing.assert_array_equal(vg, ve)
+            np.testing.assert_array_equal(sg, se)
+            np.testing.assert_array_equal(g.vjp(cots * (it + 1), generation=it),
+                                          e.vjp(cots * (it + 1), generation=it))
+            torch.cuda.synchronize()
+            _ = (xd[k] * 2).sum().item()  # other stream work between replays
+            vg, _ = g.forward_device(xd[k], generation=100 + it)
+            ve, _ = e.forward_device(xd[k], generation=100 + it)
+            np.testing.assert_array_equal(vg, ve)
+            g.vjp_device(cots, out_g, generation=100 + it)
+            e.vjp_device(cots, out_e, generation=100 + it)
+            torch.cuda.synchronize()
+            assert torch.equal(out_g, out_e)

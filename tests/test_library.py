@@ -77,3 +77,23 @@ def test_lbfgs_failure_value_without_callback_errors():
     lib = native._library()
     x = np.zeros(4)
     assert lib.stlp_lbfgs(native._pointer(x), 4, b"{bad", native._CALLBACK(lambda *a: 0.0), None) == -1
+
+
+@pytest.mark.parametrize("nside, j, batch, options, message", [
+    ([16, 12], [3, 3], [1, 1], "{}", "power of two"),
+    ([16], [6], [1], "{}", "j must satisfy"),
+    ([16], [3], [0], "{}", "batch"),
+    ([16], [3], [1], '{"precision": "fp16"}', "precision"),
+])
+def test_bad_ensemble_is_an_error(nside, j, batch, options, message):
+    # validated before any device work
+    from stl_port import NativeEnsemble
+    with pytest.raises(RuntimeError, match=message):
+        NativeEnsemble([(n, b, jj) for n, jj, b in zip(nside, j, batch)],
+                       precision=json.loads(options).get("precision", "mixed"))
+
+
+def test_ensemble_without_groups_is_an_error():
+    lib = native._library()
+    assert not lib.stlx_ensemble_new(None, None, None, 0, b"{}")
+    assert "at least one group" in lib.stlp_last_error().decode()
