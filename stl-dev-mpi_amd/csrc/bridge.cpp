@@ -194,6 +194,8 @@ StlHandle* stlp_new(const char* config) {
     opt.fp64 = precision == "fp64";
     opt.device = int(v.integer("device", 0));
     opt.timing = v.boolean("timing", false);
+    opt.conv_variant = int(v.integer("conv_variant", -1));
+    if (opt.conv_variant < -1 || opt.conv_variant > 3) fail("conv_variant must be -1, 0, 1, 2 or 3");
     auto h = std::make_unique<StlHandle>();
     h->engine = stl::Engine::create(opt);
     h->id = g_next_id.fetch_add(1);
@@ -221,6 +223,13 @@ char* stlp_info(StlHandle* h) {
     o << ", \"device_memory\": {\"total_bytes\": " << h->engine->device_bytes()
       << ", \"peak_bytes\": " << stl::device_bytes_peak() << "}";
     o << ", \"setup_seconds\": " << h->engine->setup_seconds();
+    o << ", \"conv_variants\": [";
+    {
+      const auto cv = h->engine->conv_variants();
+      for (size_t n = 0; n < cv.size(); ++n)
+        o << (n ? ", " : "") << "[" << cv[n][0] << ", " << cv[n][1] << ", " << cv[n][2] << ", " << cv[n][3] << "]";
+    }
+    o << "]";
     o << ", \"timers\": {";
     bool first = true;
     for (auto& t : h->engine->timers()) {

@@ -21,6 +21,8 @@
 #include <chrono>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <tuple>
 
 #include "geometry.hpp"
 #include "sweeps.hpp"
@@ -110,6 +112,17 @@ SumSlot slot_of(const Coefficient& c) {
     }
   }
   return s;
+}
+
+// Convolution variants chosen by timing: (device, fp64, order, kernel, rows).
+using TuneKey = std::tuple<int, bool, int, int, int>;
+std::map<TuneKey, int>& tune_cache() {
+  static std::map<TuneKey, int> c;
+  return c;
+}
+std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
 }
 
 // Timer name of a kernel family at a level.
@@ -215,6 +228,7 @@ class EngineT final : public Engine {
     }
     geometry_bytes_ = device_bytes_live();
     allocate();
+    tune();
     STL_HIP(hipStreamSynchronize(s_));
     setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -232,6 +246,7 @@ class EngineT final : public Engine {
   size_t device_bytes() const override { return device_bytes_live(); }
   double setup_seconds() const override { return setup_s_; }
   std::vector<KernelTimer> timers() const override { return prof_.timers(); }
+  std::vector<std::array<int, 4>> conv_variants() const override { return conv_var_; }
   void reset_timers() override { prof_.reset(); }
 
   void forward_device(const double* x_dev, bool moments, cplx* values, double* variance) override {
@@ -451,7 +466,7 @@ class EngineT final : public Engine {
 
   void conv_adj(int n, int NI, const C2* gy, int rows, T* gx) {
     int id = prof_.start();
-    launch_conv_adj<T>(conv_[n]->store.view(), NI, gy, rows, gx, s_);
+    launch_conv_adj<T>(conv_[n]->store.view(), NI, gy, rows, gx, s_, conv_var_[n][NI == 1 ? 2 : 3]);
     prof_.stop(id, tag(NI == 1 ? "conv_adj_z" : "conv_adj_branch", n), conv_bytes(n, NI, rows, true));
   }
 
@@ -466,7 +481,7 @@ class EngineT final : public Engine {
       const int64_t N = N_[n];
       int id = prof_.start();
       launch_conv_fwd<T>(conv_[n]->store.view(), 1, maps_[n].data(), B_, z_[n].data(),
-                         M_[n].data() + size_t(n) * B_ * N * kL, s_);
+                         M_[n].data() + size_t(n) * B_ * N * kL, s_, conv_var_[n][0]);
       prof_.stop(id, tag("conv_fwd_z", n), conv_bytes(n, 1, B_, false) + double(B_) * N * kL * sizeof(T));
       if (n > 0) {
         id = prof_.start();
@@ -474,7 +489,8 @@ class EngineT final : public Engine {
         prof_.stop(id, tag("dec_fwd", n), dec_bytes(n - 1, kL, n * B_));
       }
       id = prof_.start();
-      launch_conv_fwd<T>(conv_[n]->store.view(), kL, M_[n].data(), (n + 1) * B_, br_[n].data(), nullptr, s_);
+      launch_conv_fwd<T>(conv_[n]->store.view(), kL, M_[n].data(), (n + 1) * B_, br_[n].data(), nullptr, s_,
+                         conv_var_[n][1]);
       prof_.stop(id, tag("conv_fwd_branch", n), conv_bytes(n, kL, (n + 1) * B_, false));
       id = prof_.start();
       int nb = nblk_gram_[n];
@@ -561,6 +577,82 @@ class EngineT final : public Engine {
     }
     STL_HIP(hipGraphLaunch(exec, s_));
     STL_HIP(hipStreamSynchronize(s_));
+  }
+
+  // Convolution kernel variant per level and kernel (forward z, forward
+  // branch, adjoint z, adjoint branch): forced, or the fastest of the
+  // candidates timed on this level's operator and buffers (a candidate must
+  // beat the default by 3%), remembered for the process.
+  void tune() {
+    conv_var_.assign(levels_, {kConvTiled, kConvTiled, kConvTiled, kConvTiled});
+    for (int n = 0; n < levels_; ++n) {
+      ConvOp<T> op = conv_[n]->store.view();
+      if (!op.tiled) continue;
+      for (int kind = 0; kind < 4; ++kind) {
+        const bool adj = kind >= 2;
+        const int NI = kind % 2 == 0 ? 1 : kL, rows = kind % 2 == 0 ? B_ : (n + 1) * B_;
+        std::vector<int> cands{kConvTiled};
+        if (!adj || conv_adj_has_variant<T>(NI, kConvFaces)) cands.push_back(kConvFaces);
+        if (op.order >= 4) cands.push_back(kConvTiled8);
+        if (op.order >= 5) cands.push_back(kConvTiled32);
+        if (opt_.conv_variant >= 0) {
+          bool ok = std::find(cands.begin(), cands.end(), opt_.conv_variant) != cands.end();
+          conv_var_[n][kind] = ok ? opt_.conv_variant : kConvTiled;
+          continue;
+        }
+        const TuneKey key{opt_.device, sizeof(T) == 8, op.order, kind, rows};
+        {
+          std::lock_guard<std::mutex> lock(tune_mutex());
+          auto it = tune_cache().find(key);
+          if (it != tune_cache().end()) {
+            conv_var_[n][kind] = it->second;
+            continue;
+          }
+        }
+        auto timed = [&](int v) {
+          if (!adj) {
+            T* x = NI == 1 ? maps_[n].data() : M_[n].data();
+            C2* y = NI == 1 ? z_[n].data() : br_[n].data();
+            T* mod = NI == 1 ? M_[n].data() + size_t(n) * B_ * N_[n] * kL : nullptr;
+            return time_launch([&] { launch_conv_fwd<T>(op, NI, x, rows, y, mod, s_, v); });
+          }
+          const C2* gy = NI == 1 ? gz_.data() : gbr_.data();
+          T* gx = NI == 1 ? mg_[0].data() : acc_[0].data();
+          return time_launch([&] { launch_conv_adj<T>(op, NI, gy, rows, gx, s_, v); });
+        };
+        const double base = timed(kConvTiled);
+        double best = base;
+        int pick = kConvTiled;
+        for (size_t c = 1; c < cands.size(); ++c) {
+          const double ms = timed(cands[c]);
+          if (ms < 0.97 * base && ms < best) {
+            best = ms;
+            pick = cands[c];
+          }
+        }
+        conv_var_[n][kind] = pick;
+        std::lock_guard<std::mutex> lock(tune_mutex());
+        tune_cache()[key] = pick;
+      }
+    }
+  }
+
+  // Mean duration of 3 launches after one warm-up (HIP events on the engine stream).
+  template <typename F>
+  double time_launch(F&& launch) {
+    hipEvent_t a, b;
+    STL_HIP(hipEventCreate(&a));
+    STL_HIP(hipEventCreate(&b));
+    launch();
+    STL_HIP(hipEventRecord(a, s_));
+    for (int i = 0; i < 3; ++i) launch();
+    STL_HIP(hipEventRecord(b, s_));
+    STL_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    STL_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / 3.0;
   }
 
   // Device tables of the statistics assembly and of the cotangent scatter.
@@ -652,6 +744,7 @@ class EngineT final : public Engine {
 
   EngineOptions opt_;
   hipStream_t s_ = nullptr;
+  std::vector<std::array<int, 4>> conv_var_;
   std::vector<GraphEntry> fwd_graphs_, vjp_graphs_;
   Profiler prof_;
   std::vector<Coefficient> plan_;

@@ -5,6 +5,7 @@
 // gauge cosmo, kernel 5, L = 4).
 #pragma once
 
+#include <array>
 #include <complex>
 #include <memory>
 #include <string>
@@ -34,6 +35,10 @@ struct EngineOptions {
   bool fp64 = true;      // false: FP32 bulk fields and operators, FP64 sums
   int device = 0;
   bool timing = false;   // per-kernel HIP-event timers
+  // Convolution kernel variant of the LDS-tiled levels: -1 chooses per level
+  // and kernel by timing at creation (cached per device, precision, level and
+  // shape); 0, 1, 2 force one where the kernel has it (see sweeps.hpp).
+  int conv_variant = -1;
 };
 
 // Per-kernel-family timing (HIP events on the engine stream).
@@ -71,6 +76,9 @@ class Engine {
   virtual size_t device_bytes() const = 0;
   virtual double setup_seconds() const = 0;
   virtual std::vector<KernelTimer> timers() const = 0;
+  // Chosen convolution variant per level: [level][forward z, forward branch,
+  // adjoint z, adjoint branch].
+  virtual std::vector<std::array<int, 4>> conv_variants() const = 0;
   virtual void reset_timers() = 0;
   // One level's operator applied to float64 device arrays (kind 0 conv forward
   // [rows][N] -> [rows][4][N] complex, 1 its adjoint, 2 decimation to level + 1,

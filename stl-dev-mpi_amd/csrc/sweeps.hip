@@ -329,6 +329,103 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const 
   }
 }
 
+// Workgroup -> (rep tile, face group of FW faces, row) for the multi-face
+// kernels: the 4 / FW * rows workgroups of one rep tile are numbered to land
+// on one XCD as in tile_info.
+template <int TS, int FW>
+__device__ inline void tile_group(int order, int rows, int64_t& tr, int& fg, int& row) {
+  const int64_t npf = int64_t(1) << (2 * order);
+  const int64_t tiles = 3 * (npf / (TS * TS));
+  const int ways = (4 / FW) * rows;
+  const int64_t full = (tiles / kXcds) * kXcds * ways;
+  const int64_t b = blockIdx.x;
+  int j;
+  if (b < full) {
+    int64_t rem = b % (int64_t(kXcds) * ways);
+    tr = (b / (int64_t(kXcds) * ways)) * kXcds + rem % kXcds;
+    j = int(rem / kXcds);
+  } else {
+    int64_t rem = b - full;
+    tr = (tiles / kXcds) * kXcds + rem / ways;
+    j = int(rem % ways);
+  }
+  fg = j % (4 / FW);
+  row = j / (4 / FW);
+}
+
+// Forward convolution of FW rotated faces of one rep tile per workgroup:
+// every slot's coefficients are loaded once and applied to the FW faces'
+// staged inputs (faces k = fg * FW + f).
+template <typename T, int NI, int TS, int FW>
+__global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const T* __restrict__ x, int rows,
+                                                           cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
+  constexpr int SW = TS + 2 * kWin, SA = SW * SW;
+  __shared__ T xs[FW * SA * NI];
+  int64_t tr;
+  int fg, row;
+  tile_group<TS, FW>(op.order, rows, tr, fg, row);
+  const int64_t ntile = op.npf / (TS * TS);
+  const int fr = int(tr / ntile);
+  const int64_t base = (tr % ntile) * (TS * TS);
+  const int x0 = int(compress_bits64(base)), y0 = int(compress_bits64(base >> 1));
+  const int64_t N = 12 * op.npf;
+  const T* xr = x + size_t(row) * N * NI;
+  for (int idx = threadIdx.x; idx < FW * SA; idx += TS * TS) {
+    const int f = idx / SA, pi = idx % SA;
+    const int face = 4 * fr + fg * FW + f;
+    int64_t p = ext_pixel(face, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order);
+#pragma unroll
+    for (int j = 0; j < NI; ++j) xs[idx * NI + j] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
+  }
+  __syncthreads();
+  const int li = threadIdx.x;
+  const int pos = (int(compress_bits64(li >> 1)) + kWin) * SW + int(compress_bits64(li)) + kWin;
+  const int64_t r = int64_t(fr) * op.npf + base + li;
+  T acc[FW][NI][kL][2];
+#pragma unroll
+  for (int f = 0; f < FW; ++f)
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+#pragma unroll
+      for (int g = 0; g < kL; ++g) acc[f][j][g][0] = acc[f][j][g][1] = T(0);
+  const int cnt = op.cnt[r];
+#pragma unroll 2
+  for (int s = 0; s < cnt; ++s) {
+    size_t at = size_t(s) * op.R + r;
+    int16_t o = op.off[at];
+    int src = (pos + off_dy(o) * SW + off_dx(o)) * NI;
+    T c[8];
+    Load<T>::coef8(op.coef + at * 8, c);
+#pragma unroll
+    for (int f = 0; f < FW; ++f) {
+      T a[8];
+      permute<T>(fg * FW + f, c, a);
+#pragma unroll
+      for (int j = 0; j < NI; ++j) {
+        T xv = xs[f * SA * NI + src + j];
+#pragma unroll
+        for (int g = 0; g < kL; ++g) {
+          acc[f][j][g][0] += a[g * 2] * xv;
+          acc[f][j][g][1] += a[g * 2 + 1] * xv;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < FW; ++f) {
+    const int64_t p = int64_t(4 * fr + fg * FW + f) * op.npf + base + li;
+#pragma unroll
+    for (int g = 0; g < kL; ++g)
+#pragma unroll
+      for (int j = 0; j < NI; ++j)
+        y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(acc[f][j][g][0], acc[f][j][g][1]);
+    if (mod != nullptr && NI == 1) {
+#pragma unroll
+      for (int g = 0; g < kL; ++g) mod[(size_t(row) * N + p) * kL + g] = hypot(acc[f][0][g][0], acc[f][0][g][1]);
+    }
+  }
+}
+
 // GP gauges of the cotangent staged per pass.
 template <typename T, int NI, int TS, int GP>
 __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
@@ -347,7 +444,12 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
   T acc[NI];
 #pragma unroll
   for (int j = 0; j < NI; ++j) acc[j] = T(0);
-  for (int g0 = 0; g0 < kL; g0 += GP) {
+  for (int pass = 0; pass < kL / GP; ++pass) {
+    // A face of odd shift k reads gauge g from coefficient plane g ^ 2: it
+    // walks its gauges in that order, so that the 4 faces of a rep tile
+    // (consecutive workgroups on one XCD) read the same planes in the same
+    // pass and share them in L2.
+    const int g0 = GP == kL ? 0 : ((pass * GP) ^ (2 * (ti.k & 1)));
     __syncthreads();
     for (int idx = threadIdx.x; idx < GP * SA; idx += TS * TS) {
       int gg = idx / SA, pi = idx % SA;
@@ -380,12 +482,83 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
   for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + p) * NI + j] = acc[j];
 }
 
+// Adjoint convolution of FW rotated faces of one rep tile per workgroup.  In
+// pass P every face reads coefficient planes P * GP .. P * GP + GP - 1 (face k
+// takes gauge plane ^ 2 (k & 1) from them, conjugated by its shift), so each
+// plane entry is loaded once for the FW faces.
+template <typename T, int NI, int TS, int FW, int GP>
+__global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
+                                                           int rows, T* __restrict__ gx) {
+  constexpr int SW = TS + 2 * kWin, SA = SW * SW;
+  __shared__ cplx_t<T> gs[FW * GP * SA * NI];
+  int64_t tr;
+  int fg, row;
+  tile_group<TS, FW>(op.order, rows, tr, fg, row);
+  const int64_t ntile = op.npf / (TS * TS);
+  const int fr = int(tr / ntile);
+  const int64_t base = (tr % ntile) * (TS * TS);
+  const int x0 = int(compress_bits64(base)), y0 = int(compress_bits64(base >> 1));
+  const int64_t N = 12 * op.npf;
+  const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
+  const int li = threadIdx.x;
+  const int pos = (int(compress_bits64(li >> 1)) + kWin) * SW + int(compress_bits64(li)) + kWin;
+  const int64_t q = int64_t(fr) * op.npf + base + li;
+  const int cnt = op.tcnt[q];
+  const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
+  T acc[FW][NI];
+#pragma unroll
+  for (int f = 0; f < FW; ++f)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) acc[f][j] = T(0);
+  for (int pass = 0; pass < kL / GP; ++pass) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < FW * GP * SA; idx += TS * TS) {
+      const int f = idx / (GP * SA), rem = idx % (GP * SA), gg = rem / SA, pi = rem % SA;
+      const int k = fg * FW + f;
+      const int g = (pass * GP + gg) ^ (2 * (k & 1));
+      int64_t p = ext_pixel(4 * fr + k, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order);
+#pragma unroll
+      for (int j = 0; j < NI; ++j)
+        gs[idx * NI + j] = p >= 0 ? gr[(size_t(g) * N + p) * NI + j] : make_c<T>(T(0), T(0));
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int s = 0; s < cnt; ++s) {
+      size_t at = size_t(s) * op.R + q;
+      int16_t o = op.toff[at];
+      int src = pos + off_dy(o) * SW + off_dx(o);
+#pragma unroll
+      for (int gg = 0; gg < GP; ++gg) {
+        const int plane = pass * GP + gg;
+        const cplx_t<T> a = tc[(size_t(plane) * op.Tn + s) * op.R + q];
+#pragma unroll
+        for (int f = 0; f < FW; ++f) {
+          const int k = fg * FW + f;
+          const T ai = gauge_conj(k, plane ^ (2 * (k & 1))) ? -a.y : a.y;
+#pragma unroll
+          for (int j = 0; j < NI; ++j) {
+            cplx_t<T> v = gs[((f * GP + gg) * SA + src) * NI + j];
+            acc[f][j] += a.x * v.x + ai * v.y;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < FW; ++f) {
+    const int64_t p = int64_t(4 * fr + fg * FW + f) * op.npf + base + li;
+#pragma unroll
+    for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + p) * NI + j] = acc[f][j];
+  }
+}
+
 template <typename T, int NI>
 __global__ __launch_bounds__(256) void k_dec_fwd(DecOp<T> op, const T* __restrict__ x, T* __restrict__ y) {
-  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (t >= op.Rc * 4) return;
-  int64_t r = t >> 2;
-  int k = int(t & 3);
+  // Wave w of the workgroup = face shift w over 64 consecutive rep rows: a
+  // wave's gathers stay in one face, and the 4 waves share the rows' entries.
+  const int64_t r = blockIdx.x * int64_t(64) + (threadIdx.x & 63);
+  if (r >= op.Rc) return;
+  const int k = int(threadIdx.x >> 6);
   int row = blockIdx.y;
   int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
   const T* xr = x + size_t(row) * Nf * NI;
@@ -410,10 +583,9 @@ __global__ __launch_bounds__(256) void k_dec_fwd(DecOp<T> op, const T* __restric
 template <typename T, int NI>
 __global__ __launch_bounds__(256) void k_dec_adj(DecOp<T> op, const T* __restrict__ gy, const T* add,
                                                  T* gx) {
-  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (t >= op.Rf * 4) return;
-  int64_t q = t >> 2;
-  int k = int(t & 3);
+  const int64_t q = blockIdx.x * int64_t(64) + (threadIdx.x & 63);
+  if (q >= op.Rf) return;
+  const int k = int(threadIdx.x >> 6);
   int row = blockIdx.y;
   int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
   const T* gr = gy + size_t(row) * Nc * NI;
@@ -442,19 +614,95 @@ __device__ inline double2 to_d2(cplx_t<T> v) {
 }
 
 // Gram partial sums: block (tile stride, b * kL + l3).
+// Gram products at the levels of small K (all E = K (K + 1) / 2 entries of a
+// pixel fit in registers): each thread reads its pixels' V straight from HBM
+// (40 bytes per pixel at K = 5, coalesced) and accumulates every entry in
+// FP64; the workgroup then sums its threads by a fixed butterfly within each
+// wave and the 4 wave sums in order (deterministic).
+template <typename T, int KT, bool MOM>
+__global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
+                                                  const cplx_t<T>* __restrict__ z, double* __restrict__ part,
+                                                  double* __restrict__ part2) {
+  constexpr int E = KT * (KT + 1) / 2;
+  constexpr int ND = (KT - 1) / kL;  // branch rows d = 0..n
+  __shared__ double red[4][MOM ? 3 : 2][E];
+  const int bl = blockIdx.y;
+  const int b = bl / kL, l3 = bl % kL;
+  const int tid = threadIdx.x;
+  double are[E], aim[E], asq[MOM ? E : 1];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    are[e] = aim[e] = 0.0;
+    if (MOM) asq[e] = 0.0;
+  }
+  const cplx_t<T>* zr = z + (size_t(b) * kL + l3) * N;
+  for (int64_t p = blockIdx.x * int64_t(256) + tid; p < N; p += int64_t(gridDim.x) * 256) {
+    double2 v[KT];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const cplx_t<T>* src = br + ((size_t(d) * B + b) * kL + l3) * N * kL + size_t(p) * kL;
+#pragma unroll
+      for (int l1 = 0; l1 < kL; ++l1) v[d * kL + l1] = to_d2<T>(src[l1]);
+    }
+    v[KT - 1] = to_d2<T>(zr[p]);
+    int e = 0;
+#pragma unroll
+    for (int i = 0; i < KT; ++i)
+#pragma unroll
+      for (int j = i; j < KT; ++j, ++e) {
+        double re = v[i].x * v[j].x + v[i].y * v[j].y;
+        double im = v[i].y * v[j].x - v[i].x * v[j].y;
+        are[e] += re;
+        aim[e] += im;
+        if (MOM) asq[e] += re * re + im * im;
+      }
+  }
+  (void)n;
+  const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    double a = are[e], c = aim[e], q = MOM ? asq[e] : 0.0;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      a += __shfl_xor(a, off);
+      c += __shfl_xor(c, off);
+      if (MOM) q += __shfl_xor(q, off);
+    }
+    if (lane == 0) {
+      red[w][0][e] = a;
+      red[w][1][e] = c;
+      if (MOM) red[w][MOM ? 2 : 1][e] = q;
+    }
+  }
+  __syncthreads();
+  const size_t base = (size_t(blockIdx.x) * gridDim.y + bl) * E;
+  for (int e = tid; e < E; e += 256) {
+    double a = 0.0, c = 0.0, q = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a += red[k][0][e];
+      c += red[k][1][e];
+      if (MOM) q += red[k][MOM ? 2 : 1][e];
+    }
+    part[(base + e) * 2] = a;
+    part[(base + e) * 2 + 1] = c;
+    if (MOM) part2[base + e] = q;
+  }
+}
+
 // Gram products of one (b, l3) per workgroup row.  A tile of V [TPX][KP]
 // (columns K..KP-1 zero) is staged in LDS; each thread owns a 4 x 4 block
 // (bi <= bj) of entries V_i conj(V_j) and a group of the tile's pixels, so
 // every LDS read feeds 4 products.  The values widen exactly to FP64; the
 // products and their sums are FP64.  The per-group sums are added in group
 // order: deterministic.
-template <typename T, bool MOM>
+template <typename T, bool MOM, int BS>
 __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
                                               const cplx_t<T>* __restrict__ z, int TPX, double* __restrict__ part,
                                               double* __restrict__ part2) {
   extern __shared__ double2 smem[];
   const int K = (n + 1) * kL + 1;
-  const int KP = (K + 3) & ~3, NR = KP / 4, NBLK = NR * (NR + 1) / 2;
+  const int KP = (K + BS - 1) / BS * BS, NR = KP / BS, NBLK = NR * (NR + 1) / 2;
   const int E = K * (K + 1) / 2;
   double2* Vs = smem;
   const int bl = blockIdx.y;
@@ -466,11 +714,11 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
   int bi = 0, rem = blk;
   while (rem >= NR - bi) { rem -= NR - bi; ++bi; }
   const int bj = bi + rem;
-  double are[4][4], aim[4][4], asq[4][4];
+  double are[BS][BS], aim[BS][BS], asq[BS][BS];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int r = 0; r < BS; ++r)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) are[r][c] = aim[r][c] = asq[r][c] = 0.0;
+    for (int c = 0; c < BS; ++c) are[r][c] = aim[r][c] = asq[r][c] = 0.0;
   const int64_t ntiles = (N + TPX - 1) / TPX;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t p0 = tile * TPX;
@@ -489,16 +737,16 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
     __syncthreads();
     if (act) {
       for (int p = grp; p < tn; p += G) {
-        double2 a[4], c[4];
+        double2 a[BS], c[BS];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          a[r] = Vs[p * KP + 4 * bi + r];
-          c[r] = Vs[p * KP + 4 * bj + r];
+        for (int r = 0; r < BS; ++r) {
+          a[r] = Vs[p * KP + BS * bi + r];
+          c[r] = Vs[p * KP + BS * bj + r];
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < BS; ++r)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < BS; ++q) {
             double re = a[r].x * c[q].x + a[r].y * c[q].y;
             double im = a[r].y * c[q].x - a[r].x * c[q].y;
             are[r][q] += re;
@@ -512,12 +760,13 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
   // Sum the pixel groups: one (r, q) entry of every block per round.
   double2* red = smem;                                          // [256]
   double* red2 = reinterpret_cast<double*>(smem + 256);         // [256]
-  double2* fin = smem + 256 + 128;                              // [NBLK][16]
-  double* fin2 = reinterpret_cast<double*>(fin + NBLK * 16);    // [NBLK][16]
+  constexpr int BB = BS * BS;
+  double2* fin = smem + 256 + 128;                              // [NBLK][BB]
+  double* fin2 = reinterpret_cast<double*>(fin + NBLK * BB);    // [NBLK][BB]
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int r = 0; r < BS; ++r)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < BS; ++q) {
       red[tid] = make_double2(are[r][q], aim[r][q]);
       if (MOM) red2[tid] = asq[r][q];
       __syncthreads();
@@ -529,8 +778,8 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
           s.y += red[g * NBLK + tid].y;
           if (MOM) s2 += red2[g * NBLK + tid];
         }
-        fin[tid * 16 + r * 4 + q] = s;
-        if (MOM) fin2[tid * 16 + r * 4 + q] = s2;
+        fin[tid * BB + r * BS + q] = s;
+        if (MOM) fin2[tid * BB + r * BS + q] = s2;
       }
       __syncthreads();
     }
@@ -539,8 +788,8 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
     int i = 0, re = e;
     while (re >= K - i) { re -= K - i; ++i; }
     const int j = i + re;
-    const int ib = i / 4, jb = j / 4;
-    const int k = (ib * NR - ib * (ib - 1) / 2 + (jb - ib)) * 16 + (i % 4) * 4 + (j % 4);
+    const int ib = i / BS, jb = j / BS;
+    const int k = (ib * NR - ib * (ib - 1) / 2 + (jb - ib)) * BB + (i % BS) * BS + (j % BS);
     part[(base + e) * 2] = fin[k].x;
     part[(base + e) * 2 + 1] = fin[k].y;
     if (MOM) part2[base + e] = fin2[k];
@@ -743,16 +992,42 @@ void conv_fwd_tiled(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>
 template <typename T, int TS>
 void conv_adj_tiled(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
   dim3 grid(unsigned(op.R / (TS * TS) * 4 * rows));
-  constexpr int GP4 = sizeof(T) == 8 ? 1 : 2;  // NI = kL: 43 KB of LDS per pass
-  if (NI == 1) k_conv_adj_tiled<T, 1, TS, 4><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
-  else k_conv_adj_tiled<T, kL, TS, GP4><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
+  if constexpr (TS == 32) {
+    // 42 x 42 staged pixels: one gauge per pass for NI = kL (56 KB FP32,
+    // 113 KB FP64), 4 (FP32) or 2 (FP64) gauges for NI = 1.
+    constexpr int GP1 = sizeof(T) == 8 ? 2 : 4;
+    if (NI == 1) k_conv_adj_tiled<T, 1, TS, GP1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
+    else k_conv_adj_tiled<T, kL, TS, 1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
+  } else {
+    constexpr int GP4 = sizeof(T) == 8 ? 1 : 2;  // NI = kL: 43 KB of LDS per pass
+    if (NI == 1) k_conv_adj_tiled<T, 1, TS, 4><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
+    else k_conv_adj_tiled<T, kL, TS, GP4><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
+  }
+}
+
+template <typename T, int TS>
+void conv_fwd_faces(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s) {
+  // FP64 with NI = kL: 2 faces per workgroup (registers); otherwise 4.
+  constexpr int FW4 = sizeof(T) == 8 ? 2 : 4;
+  if (NI == 1) {
+    dim3 grid(unsigned(op.R / (TS * TS) * rows));
+    k_conv_fwd_faces<T, 1, TS, 4><<<grid, TS * TS, 0, s>>>(op, x, rows, y, mod);
+  } else {
+    dim3 grid(unsigned(op.R / (TS * TS) * (4 / FW4) * rows));
+    k_conv_fwd_faces<T, kL, TS, FW4><<<grid, TS * TS, 0, s>>>(op, x, rows, y, nullptr);
+  }
 }
 
 template <typename T>
-void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s) {
+void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s,
+                     int variant) {
   STL_CHECK(NI == 1 || NI == kL, "conv: unsupported inner width");
   if (op.tiled) {
-    if (op.order >= 4) conv_fwd_tiled<T, 16>(op, NI, x, rows, y, mod, s);
+    const int TS = op.order >= 4 ? 16 : 8;
+    if (variant == kConvTiled32 && op.order >= 5) conv_fwd_tiled<T, 32>(op, NI, x, rows, y, mod, s);
+    else if (variant == kConvFaces && TS == 16) conv_fwd_faces<T, 16>(op, NI, x, rows, y, mod, s);
+    else if (variant == kConvFaces) conv_fwd_faces<T, 8>(op, NI, x, rows, y, mod, s);
+    else if (TS == 16 && variant != kConvTiled8) conv_fwd_tiled<T, 16>(op, NI, x, rows, y, mod, s);
     else conv_fwd_tiled<T, 8>(op, NI, x, rows, y, mod, s);
     STL_HIP(hipGetLastError());
     return;
@@ -764,11 +1039,41 @@ void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T
   STL_HIP(hipGetLastError());
 }
 
+// Multi-face adjoint within 43 KB of LDS: FP32 NI = kL: 2 faces, 1 gauge per
+// pass; NI = 1: 4 faces, 2 (FP32) or 1 (FP64) gauges per pass.  FP64 with
+// NI = kL has no such variant (it would need 86 KB).
+template <typename T, int TS>
+bool conv_adj_faces(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
+  if (NI == 1) {
+    constexpr int GP1 = sizeof(T) == 8 ? 1 : 2;
+    dim3 grid(unsigned(op.R / (TS * TS) * rows));
+    k_conv_adj_faces<T, 1, TS, 4, GP1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
+    return true;
+  }
+  if constexpr (sizeof(T) == 4) {
+    dim3 grid(unsigned(op.R / (TS * TS) * 2 * rows));
+    k_conv_adj_faces<T, kL, TS, 2, 1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
+    return true;
+  }
+  return false;
+}
+
 template <typename T>
-void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
+bool conv_adj_has_variant(int NI, int variant) {
+  return variant != kConvFaces || NI == 1 || sizeof(T) == 4;
+}
+
+template <typename T>
+void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s,
+                     int variant) {
   STL_CHECK(NI == 1 || NI == kL, "conv adjoint: unsupported inner width");
+  STL_CHECK(conv_adj_has_variant<T>(NI, variant), "conv adjoint: no such variant");
   if (op.tiled) {
-    if (op.order >= 4) conv_adj_tiled<T, 16>(op, NI, gy, rows, gx, s);
+    const int TS = op.order >= 4 ? 16 : 8;
+    if (variant == kConvTiled32 && op.order >= 5) conv_adj_tiled<T, 32>(op, NI, gy, rows, gx, s);
+    else if (variant == kConvFaces && TS == 16) conv_adj_faces<T, 16>(op, NI, gy, rows, gx, s);
+    else if (variant == kConvFaces) conv_adj_faces<T, 8>(op, NI, gy, rows, gx, s);
+    else if (TS == 16 && variant != kConvTiled8) conv_adj_tiled<T, 16>(op, NI, gy, rows, gx, s);
     else conv_adj_tiled<T, 8>(op, NI, gy, rows, gx, s);
     STL_HIP(hipGetLastError());
     return;
@@ -808,14 +1113,34 @@ int gram_tile(int n) {
 template <typename T>
 void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, bool moments, int nblk,
                  double* part, double* part2, hipStream_t s) {
-  const int K = (n + 1) * kL + 1, KP = (K + 3) & ~3, NR = KP / 4, NBLK = NR * (NR + 1) / 2;
-  STL_CHECK(NBLK <= 256, "gram: too many levels");
+  const int K = (n + 1) * kL + 1;
   dim3 grid(nblk, B * kL);
+  if (K == 5) {
+    if (moments) k_gram_reg<T, 5, true><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2);
+    else k_gram_reg<T, 5, false><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2);
+    STL_HIP(hipGetLastError());
+    return;
+  }
+  if (K == 9 && !moments) {
+    k_gram_reg<T, 9, false><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2);
+    STL_HIP(hipGetLastError());
+    return;
+  }
+  // Small K: 2 x 2 blocks waste fewer padded products; larger K: 4 x 4 blocks
+  // halve the LDS reads per product.
+  const int BS = K <= 13 ? 2 : 4;
+  const int KP = (K + BS - 1) / BS * BS, NR = KP / BS, NBLK = NR * (NR + 1) / 2;
+  STL_CHECK(NBLK <= 256, "gram: too many levels");
   const int tpx = gram_tile(n);
   size_t lds = std::max(size_t(tpx) * KP * sizeof(double2),
-                        (256 + 128) * sizeof(double2) + size_t(NBLK) * 16 * (sizeof(double2) + sizeof(double)));
-  if (moments) k_gram<T, true><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
-  else k_gram<T, false><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
+                        (256 + 128) * sizeof(double2) + size_t(NBLK) * BS * BS * (sizeof(double2) + sizeof(double)));
+  if (BS == 2) {
+    if (moments) k_gram<T, true, 2><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
+    else k_gram<T, false, 2><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
+  } else {
+    if (moments) k_gram<T, true, 4><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
+    else k_gram<T, false, 4><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
+  }
   STL_HIP(hipGetLastError());
 }
 
@@ -883,8 +1208,9 @@ void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s) {
 }
 
 #define STL_INSTANTIATE(T)                                                                                     \
-  template void launch_conv_fwd<T>(const ConvOp<T>&, int, const T*, int, cplx_t<T>*, T*, hipStream_t);        \
-  template void launch_conv_adj<T>(const ConvOp<T>&, int, const cplx_t<T>*, int, T*, hipStream_t);            \
+  template void launch_conv_fwd<T>(const ConvOp<T>&, int, const T*, int, cplx_t<T>*, T*, hipStream_t, int);        \
+  template void launch_conv_adj<T>(const ConvOp<T>&, int, const cplx_t<T>*, int, T*, hipStream_t, int);     \
+  template bool conv_adj_has_variant<T>(int, int);            \
   template void launch_dec_fwd<T>(const DecOp<T>&, int, const T*, int, T*, hipStream_t);                      \
   template void launch_dec_adj<T>(const DecOp<T>&, int, const T*, const T*, int, T*, hipStream_t);            \
   template void launch_gram<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, bool, int, double*,     \

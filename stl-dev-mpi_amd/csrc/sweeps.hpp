@@ -30,11 +30,20 @@ struct Complex2<float> {
 template <typename T>
 using cplx_t = typename Complex2<T>::type;
 
+// Kernel variants of the LDS-tiled convolutions: 16 x 16 tiles with one face
+// per workgroup, several rotated faces per workgroup (one coefficient load
+// for all of them), 8 x 8 tiles, or 32 x 32 tiles (1024 threads, smaller
+// halo share; nside >= 32).
+constexpr int kConvTiled = 0, kConvFaces = 1, kConvTiled8 = 2, kConvTiled32 = 3;
 template <typename T>
 void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* modulus_out,
-                     hipStream_t s);
+                     hipStream_t s, int variant = kConvTiled);
 template <typename T>
-void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s);
+void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s,
+                     int variant = kConvTiled);
+// Whether the adjoint has this variant for the precision and inner width.
+template <typename T>
+bool conv_adj_has_variant(int NI, int variant);
 template <typename T>
 void launch_dec_fwd(const DecOp<T>& op, int NI, const T* x, int rows, T* y, hipStream_t s);
 template <typename T>

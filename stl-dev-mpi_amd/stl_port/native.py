@@ -186,7 +186,7 @@ class NativeEngine:
     def __init__(self, nside, j, batch=1, backend="gpu", kernel=5, orientations=4, block_pixels=4096,
                  cache_bytes=1 << 30, threads=1, cache="transposition", device=0, library=None, cross=False,
                  masks=None, spin=0, spin_kernel=5, spin_pyramid="scalar", spin_a=2, precision="fp64",
-                 timing=False):
+                 timing=False, conv_variant=None):
         self._lib = _library(library)
         self.library_sha256 = self._lib._stlp_binary_sha256
         self._handle = None
@@ -211,6 +211,8 @@ class NativeEngine:
                            orientations=int(orientations), block_pixels=int(block_pixels),
                            cache_bytes=int(cache_bytes), threads=int(threads), cache=cache, device=self.device,
                            cross=self.cross, spin=self.spin, precision=precision, timing=bool(timing))
+        if conv_variant is not None:
+            self.config["conv_variant"] = int(conv_variant)
         self._handle = self._lib.stlp_new(json.dumps(self.config).encode())
         if not self._handle:
             raise _error(self._lib)
