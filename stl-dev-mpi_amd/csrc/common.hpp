@@ -118,6 +118,16 @@ struct DecOp {
   const uint8_t* tcnt = nullptr;   // [Rf]
   const int32_t* trow = nullptr;   // [Tn][Rf] coarse row id (any face), ascending
   const T* tval = nullptr;         // [Tn][Rf]
+  // LDS-tiled forms: the fine cell's offset from the coarse row's first child
+  // (fine extended frame, within [kDecLo, kDecHi]) and the coarse row's offset
+  // from the fine cell's parent (coarse extended frame, within +-kDecT).
+  bool tiled = false, ttiled = false;
+  const int16_t* off = nullptr;    // [S][Rc]
+  const int16_t* toff = nullptr;   // [Tn][Rf]
 };
+
+// Offset bounds of the decimation entries used by the tiled kernels: the fine
+// candidates of a coarse row are the children of its +-2 coarse neighbours.
+constexpr int kDecLo = -4, kDecHi = 5, kDecT = 2;
 
 }  // namespace stl

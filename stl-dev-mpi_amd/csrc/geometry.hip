@@ -260,6 +260,48 @@ __global__ __launch_bounds__(256) void k_dec_rows(int order_f, int64_t Rc, doubl
   }
 }
 
+// Offset of every forward decimation entry from its coarse row's first child
+// (fine extended frame), and of every transposed entry's coarse row from the
+// fine cell's parent (coarse extended frame); out-of-bound entries clear the
+// tiled flag of their form.
+__global__ void k_dec_off(int64_t Rc, int S, int order_f, const uint8_t* __restrict__ cnt,
+                          const int32_t* __restrict__ col, int16_t* __restrict__ off, int* __restrict__ bad) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(S) * Rc) return;
+  int s = int(t / Rc);
+  int64_t r = t % Rc;
+  if (s >= cnt[r]) {
+    off[t] = 0;
+    return;
+  }
+  int dx = 0, dy = 0;
+  bool ok = hp::rel_offset(4 * rep_pixel(r, order_f - 1), col[t], order_f, dx, dy);
+  if (!ok || dx < kDecLo || dx > kDecHi || dy < kDecLo || dy > kDecHi) {
+    atomicOr(bad, 1);
+    dx = dy = 0;
+  }
+  off[t] = pack_off(dx, dy);
+}
+
+__global__ void k_dec_toff(int64_t Rf, int Tn, int order_f, const uint8_t* __restrict__ tcnt,
+                           const int32_t* __restrict__ trow, int16_t* __restrict__ toff, int* __restrict__ bad) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(Tn) * Rf) return;
+  int s = int(t / Rf);
+  int64_t q = t % Rf;
+  if (s >= tcnt[q]) {
+    toff[t] = 0;
+    return;
+  }
+  int dx = 0, dy = 0;
+  bool ok = hp::rel_offset(rep_pixel(q, order_f) >> 2, trow[t], order_f - 1, dx, dy);
+  if (!ok || dx < -kDecT || dx > kDecT || dy < -kDecT || dy > kDecT) {
+    atomicOr(bad, 2);
+    dx = dy = 0;
+  }
+  toff[t] = pack_off(dx, dy);
+}
+
 __global__ void k_max_u8(int64_t n, const uint8_t* __restrict__ a, int* __restrict__ out) {
   int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   int m = 0;
@@ -659,6 +701,46 @@ void build_dec(int order_f, DecStore<double>* d64, DecStore<float>* d32, hipStre
     d32->S = S;
     build_transpose<float, 1>(Rc, S, cnt, col, v32, order_f - 1, order_f, d32->tcnt, d32->trow, d32->tval, d32->Tn,
                               s);
+  }
+  // Offsets for the LDS-tiled kernels (coarse nside >= 8 forward, fine nside
+  // >= 16 adjoint), shared by both precisions.
+  DevArray<int16_t> off, toff;
+  int tiled_bad = 3;
+  const int Tn = d64 ? d64->Tn : d32->Tn;
+  const DevArray<uint8_t>& tcnt = d64 ? d64->tcnt : d32->tcnt;
+  const DevArray<int32_t>& trow = d64 ? d64->trow : d32->trow;
+  if (order_f - 1 >= 3) {
+    DevArray<int> bad(1);
+    bad.zero(s);
+    off.resize(size_t(S) * Rc);
+    k_dec_off<<<blocks_for(int64_t(S) * Rc, 256), 256, 0, s>>>(Rc, S, order_f, cnt.data(), col.data(), off.data(),
+                                                               bad.data());
+    STL_HIP(hipGetLastError());
+    int64_t Rf = 3 * (int64_t(1) << (2 * order_f));
+    toff.resize(size_t(Tn) * Rf);
+    k_dec_toff<<<blocks_for(int64_t(Tn) * Rf, 256), 256, 0, s>>>(Rf, Tn, order_f, tcnt.data(), trow.data(),
+                                                                 toff.data(), bad.data());
+    STL_HIP(hipGetLastError());
+    STL_HIP(hipMemcpyAsync(&tiled_bad, bad.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+    STL_HIP(hipStreamSynchronize(s));
+  }
+  for (auto* d : {d64 ? &d64->tiled : nullptr, d32 ? &d32->tiled : nullptr})
+    if (d) *d = !(tiled_bad & 1);
+  for (auto* d : {d64 ? &d64->ttiled : nullptr, d32 ? &d32->ttiled : nullptr})
+    if (d) *d = !(tiled_bad & 2) && order_f >= 4;
+  if (d64 && d32) {
+    d32->off.resize(off.size());
+    d32->toff.resize(toff.size());
+    if (off.size()) STL_HIP(hipMemcpyAsync(d32->off.data(), off.data(), off.bytes(), hipMemcpyDeviceToDevice, s));
+    if (toff.size()) STL_HIP(hipMemcpyAsync(d32->toff.data(), toff.data(), toff.bytes(), hipMemcpyDeviceToDevice, s));
+    d64->off = std::move(off);
+    d64->toff = std::move(toff);
+  } else if (d64) {
+    d64->off = std::move(off);
+    d64->toff = std::move(toff);
+  } else {
+    d32->off = std::move(off);
+    d32->toff = std::move(toff);
   }
   if (d64 && d32) {
     d32->cnt.resize(Rc);

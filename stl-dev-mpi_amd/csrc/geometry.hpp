@@ -44,8 +44,10 @@ template <typename T>
 struct DecStore {
   int order_f = 0;
   int S = 0, Tn = 0;
+  bool tiled = false, ttiled = false;
   DevArray<uint8_t> cnt, tcnt;
   DevArray<int32_t> col, trow;
+  DevArray<int16_t> off, toff;
   DevArray<T> val, tval;
   DecOp<T> view() const {
     DecOp<T> v;
@@ -59,9 +61,14 @@ struct DecStore {
     v.Tn = Tn;
     v.cnt = cnt.data(); v.col = col.data(); v.val = val.data();
     v.tcnt = tcnt.data(); v.trow = trow.data(); v.tval = tval.data();
+    v.tiled = tiled; v.ttiled = ttiled;
+    v.off = off.data(); v.toff = toff.data();
     return v;
   }
-  size_t bytes() const { return cnt.bytes() + tcnt.bytes() + col.bytes() + trow.bytes() + val.bytes() + tval.bytes(); }
+  size_t bytes() const {
+    return cnt.bytes() + tcnt.bytes() + col.bytes() + trow.bytes() + val.bytes() + tval.bytes() + off.bytes() +
+           toff.bytes();
+  }
 };
 
 // Zero-mean l1-normalised Gabor kernel of the wavelet (5 x 5, sigma 1),

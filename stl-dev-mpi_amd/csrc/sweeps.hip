@@ -552,6 +552,88 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
   }
 }
 
+// LDS-tiled decimation: a workgroup is a TC x TC tile of coarse pixels of one
+// face and one row; the fine window of their candidates, fine offsets
+// kDecLo..kDecHi around the tile's children ((2 TC + 8)^2 pixels), is staged
+// in LDS and every entry reads it at its stored offset.
+template <typename T, int NI, int TC>
+__global__ __launch_bounds__(TC * TC) void k_dec_fwd_tiled(DecOp<T> op, const T* __restrict__ x, int rows,
+                                                           T* __restrict__ y) {
+  constexpr int SW = 2 * TC + (kDecHi - kDecLo + 1) - 2, SA = SW * SW;
+  static_assert(SW == 2 * TC + 8, "fine window");
+  __shared__ T xs[SA * NI];
+  int row;
+  const TileInfo ti = tile_info<TC>(op.order_c, rows, row);
+  const int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
+  const T* xr = x + size_t(row) * Nf * NI;
+  const int fx0 = 2 * ti.x0 + kDecLo, fy0 = 2 * ti.y0 + kDecLo;
+  for (int idx = threadIdx.x; idx < SA; idx += TC * TC) {
+    int64_t p = ext_pixel(ti.face, fx0 + idx % SW, fy0 + idx / SW, op.order_f);
+#pragma unroll
+    for (int j = 0; j < NI; ++j) xs[idx * NI + j] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
+  }
+  __syncthreads();
+  const int li = threadIdx.x;
+  const int lx = int(compress_bits64(li)), ly = int(compress_bits64(li >> 1));
+  const int pos = (2 * ly - kDecLo) * SW + 2 * lx - kDecLo;
+  const int64_t r = int64_t(ti.fr) * op.npf_c + ti.base + li;
+  T acc[NI];
+#pragma unroll
+  for (int j = 0; j < NI; ++j) acc[j] = T(0);
+  const int cnt = op.cnt[r];
+  for (int s = 0; s < cnt; ++s) {
+    size_t at = size_t(s) * op.Rc + r;
+    int16_t o = op.off[at];
+    const T v = op.val[at];
+    const int src = (pos + off_dy(o) * SW + off_dx(o)) * NI;
+#pragma unroll
+    for (int j = 0; j < NI; ++j) acc[j] += v * xs[src + j];
+  }
+  const int64_t p = int64_t(ti.face) * op.npf_c + ti.base + li;
+#pragma unroll
+  for (int j = 0; j < NI; ++j) y[(size_t(row) * Nc + p) * NI + j] = acc[j];
+}
+
+// Its transpose: a TF x TF tile of fine cells gathers from the coarse window
+// of their parents +- kDecT ((TF / 2 + 2 kDecT)^2 pixels) staged in LDS.
+template <typename T, int NI, int TF>
+__global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T* __restrict__ gy, const T* add,
+                                                           int rows, T* gx) {
+  constexpr int SW = TF / 2 + 2 * kDecT, SA = SW * SW;
+  __shared__ T gs[SA * NI];
+  int row;
+  const TileInfo ti = tile_info<TF>(op.order_f, rows, row);
+  const int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
+  const T* gr = gy + size_t(row) * Nc * NI;
+  const int cx0 = ti.x0 / 2 - kDecT, cy0 = ti.y0 / 2 - kDecT;
+  for (int idx = threadIdx.x; idx < SA; idx += TF * TF) {
+    int64_t p = ext_pixel(ti.face, cx0 + idx % SW, cy0 + idx / SW, op.order_c);
+#pragma unroll
+    for (int j = 0; j < NI; ++j) gs[idx * NI + j] = p >= 0 ? gr[size_t(p) * NI + j] : T(0);
+  }
+  __syncthreads();
+  const int li = threadIdx.x;
+  const int lx = int(compress_bits64(li)), ly = int(compress_bits64(li >> 1));
+  const int pos = ((ly >> 1) + kDecT) * SW + (lx >> 1) + kDecT;
+  const int64_t q = int64_t(ti.fr) * op.npf_f + ti.base + li;
+  T acc[NI];
+#pragma unroll
+  for (int j = 0; j < NI; ++j) acc[j] = T(0);
+  const int cnt = op.tcnt[q];
+  for (int s = 0; s < cnt; ++s) {
+    size_t at = size_t(s) * op.Rf + q;
+    int16_t o = op.toff[at];
+    const T v = op.tval[at];
+    const int src = (pos + off_dy(o) * SW + off_dx(o)) * NI;
+#pragma unroll
+    for (int j = 0; j < NI; ++j) acc[j] += v * gs[src + j];
+  }
+  const int64_t p = int64_t(ti.face) * op.npf_f + ti.base + li;
+  const size_t o = (size_t(row) * Nf + p) * NI;
+#pragma unroll
+  for (int j = 0; j < NI; ++j) gx[o + j] = add ? acc[j] + add[o + j] : acc[j];
+}
+
 template <typename T, int NI>
 __global__ __launch_bounds__(256) void k_dec_fwd(DecOp<T> op, const T* __restrict__ x, T* __restrict__ y) {
   // Wave w of the workgroup = face shift w over 64 consecutive rep rows: a
@@ -1087,6 +1169,20 @@ void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows,
 
 template <typename T>
 void launch_dec_fwd(const DecOp<T>& op, int NI, const T* x, int rows, T* y, hipStream_t s) {
+  if (op.tiled) {
+    STL_CHECK(NI == 1 || NI == kL, "decimation: unsupported inner width");
+    if (op.order_c >= 4) {
+      dim3 grid(unsigned(op.Rc / 256 * 4 * rows));
+      if (NI == 1) k_dec_fwd_tiled<T, 1, 16><<<grid, 256, 0, s>>>(op, x, rows, y);
+      else k_dec_fwd_tiled<T, kL, 16><<<grid, 256, 0, s>>>(op, x, rows, y);
+    } else {
+      dim3 grid(unsigned(op.Rc / 64 * 4 * rows));
+      if (NI == 1) k_dec_fwd_tiled<T, 1, 8><<<grid, 64, 0, s>>>(op, x, rows, y);
+      else k_dec_fwd_tiled<T, kL, 8><<<grid, 64, 0, s>>>(op, x, rows, y);
+    }
+    STL_HIP(hipGetLastError());
+    return;
+  }
   dim3 grid(nblocks(op.Rc * 4, 256), rows);
   if (NI == 1) k_dec_fwd<T, 1><<<grid, 256, 0, s>>>(op, x, y);
   else if (NI == kL) k_dec_fwd<T, kL><<<grid, 256, 0, s>>>(op, x, y);
@@ -1096,6 +1192,14 @@ void launch_dec_fwd(const DecOp<T>& op, int NI, const T* x, int rows, T* y, hipS
 
 template <typename T>
 void launch_dec_adj(const DecOp<T>& op, int NI, const T* gy, const T* add, int rows, T* gx, hipStream_t s) {
+  if (op.ttiled) {
+    STL_CHECK(NI == 1 || NI == kL, "decimation adjoint: unsupported inner width");
+    dim3 grid(unsigned(op.Rf / 256 * 4 * rows));
+    if (NI == 1) k_dec_adj_tiled<T, 1, 16><<<grid, 256, 0, s>>>(op, gy, add, rows, gx);
+    else k_dec_adj_tiled<T, kL, 16><<<grid, 256, 0, s>>>(op, gy, add, rows, gx);
+    STL_HIP(hipGetLastError());
+    return;
+  }
   dim3 grid(nblocks(op.Rf * 4, 256), rows);
   if (NI == 1) k_dec_adj<T, 1><<<grid, 256, 0, s>>>(op, gy, add, gx);
   else if (NI == kL) k_dec_adj<T, kL><<<grid, 256, 0, s>>>(op, gy, add, gx);
