@@ -173,9 +173,11 @@ def run_synthesis(nside, J, iters):
             "last_loss": float(losses[-1]), "history": 10, "weights": "ergodic"}
 
 
-def roofline(timers, steps):
+def roofline(timers, steps, workload=None):
     """Dominant kernel (family at one level) by total time: algorithmic bytes
-    per launch over its mean launch duration (HIP events on the engine stream)."""
+    per launch over its mean launch duration (HIP events on the engine stream).
+    `traffic` is that kernel's HBM bytes per launch from profiles/traffic.json
+    (PMC counters of the same workload), else null."""
     name, t = max(timers.items(), key=lambda kv: kv[1]["ms"])
     per_launch_ms = t["ms"] / t["launches"]
     bytes_per_launch = t["bytes"] / t["launches"]
@@ -184,7 +186,8 @@ def roofline(timers, steps):
     prof = ROOT / "profiles" / "traffic.json"
     if prof.is_file():
         measured = json.loads(prof.read_text())
-        traffic = measured.get(name.split("@")[0], {}).get("bytes_per_launch")
+        if workload is not None and measured.get("workload") == workload:
+            traffic = measured.get(name, {}).get("bytes_per_launch")
     total_bytes = sum(v["bytes"] for v in timers.values()) / steps
     total_ms = sum(v["ms"] for v in timers.values()) / steps
     return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -285,7 +288,7 @@ def main():
         "config": {"workload": f"ScatCov forward+VJP of sum|S(x)-S(t)|^2, HEALPix NESTED nside {args.nside}, "
                                f"J={args.J}, L=4, kernel 5, 1 map, {len(mixed['plan'])} coefficients",
                    "nside": args.nside, "J": args.J, "L": 4, "batch": 1, "precision": "mixed"},
-        "roofline": roofline(mixed_t["timers"], tsteps),
+        "roofline": roofline(mixed_t["timers"], tsteps, {"nside": args.nside, "J": args.J, "precision": "mixed"}),
         "cpu_baseline": cpu,
         "setup_s": round(mixed["setup_s"], 3),
         "device_bytes": mixed["device_bytes"],

@@ -587,14 +587,14 @@ class EngineT final : public Engine {
     conv_var_.assign(levels_, {kConvTiled, kConvTiled, kConvTiled, kConvTiled});
     for (int n = 0; n < levels_; ++n) {
       ConvOp<T> op = conv_[n]->store.view();
-      if (!op.tiled) continue;
       for (int kind = 0; kind < 4; ++kind) {
         const bool adj = kind >= 2;
         const int NI = kind % 2 == 0 ? 1 : kL, rows = kind % 2 == 0 ? B_ : (n + 1) * B_;
         std::vector<int> cands{kConvTiled};
-        if (!adj || conv_adj_has_variant<T>(NI, kConvFaces)) cands.push_back(kConvFaces);
-        if (op.order >= 4) cands.push_back(kConvTiled8);
-        if (op.order >= 5) cands.push_back(kConvTiled32);
+        if (op.tiled && (!adj || conv_adj_has_variant<T>(NI, kConvFaces))) cands.push_back(kConvFaces);
+        if (op.tiled && op.order >= 4) cands.push_back(kConvTiled8);
+        if (op.tiled && op.order >= 5) cands.push_back(kConvTiled32);
+        if (op.order <= 6) cands.push_back(kConvSplit);
         if (opt_.conv_variant >= 0) {
           bool ok = std::find(cands.begin(), cands.end(), opt_.conv_variant) != cands.end();
           conv_var_[n][kind] = ok ? opt_.conv_variant : kConvTiled;

@@ -175,6 +175,110 @@ __global__ __launch_bounds__(256) void k_conv_adj(ConvOp<T> op, const cplx_t<T>*
   for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + qk) * NI + j] = acc[j];
 }
 
+// The same gathers with the slots of one output spread over LP consecutive
+// lanes, for levels too small to fill the GPU: each thread's chain of
+// dependent loads is LP times shorter.  The LP partial sums are added by a
+// fixed butterfly (deterministic).
+template <typename T, int NI, int LP>
+__global__ __launch_bounds__(256) void k_conv_fwd_split(ConvOp<T> op, const T* __restrict__ x,
+                                                        cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
+  const int64_t o = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) / LP;
+  const int lane = int(threadIdx.x % LP);
+  const bool active = o < op.R * 4;
+  const int64_t r = active ? o >> 2 : 0;
+  const int k = int(o & 3);
+  const int row = blockIdx.y;
+  const int64_t N = 12 * op.npf;
+  const T* xr = x + size_t(row) * N * NI;
+  T acc[NI][kL][2];
+#pragma unroll
+  for (int j = 0; j < NI; ++j)
+#pragma unroll
+    for (int g = 0; g < kL; ++g) acc[j][g][0] = acc[j][g][1] = T(0);
+  const int cnt = active ? op.cnt[r] : 0;
+  for (int s = lane; s < cnt; s += LP) {
+    size_t at = size_t(s) * op.R + r;
+    int64_t ck = rot_pix(op.cell[at], k, op.order);
+    T c[8], a[8];
+    Load<T>::coef8(op.coef + at * 8, c);
+    permute<T>(k, c, a);
+    T xv[NI];
+    Load<T>::template vec<NI>(xr + size_t(ck) * NI, xv);
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+#pragma unroll
+      for (int g = 0; g < kL; ++g) {
+        acc[j][g][0] += a[g * 2] * xv[j];
+        acc[j][g][1] += a[g * 2 + 1] * xv[j];
+      }
+  }
+#pragma unroll
+  for (int m = LP / 2; m >= 1; m >>= 1)
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+#pragma unroll
+      for (int g = 0; g < kL; ++g) {
+        acc[j][g][0] += __shfl_xor(acc[j][g][0], m);
+        acc[j][g][1] += __shfl_xor(acc[j][g][1], m);
+      }
+  if (!active || lane != 0) return;
+  int64_t p = rot_pix(rep_pixel(r, op.order), k, op.order);
+#pragma unroll
+  for (int g = 0; g < kL; ++g)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(acc[j][g][0], acc[j][g][1]);
+  if (mod != nullptr && NI == 1) {
+#pragma unroll
+    for (int g = 0; g < kL; ++g) mod[(size_t(row) * N + p) * kL + g] = hypot(acc[0][g][0], acc[0][g][1]);
+  }
+}
+
+template <typename T, int NI, int LP>
+__global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
+                                                        T* __restrict__ gx) {
+  const int64_t o = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) / LP;
+  const int lane = int(threadIdx.x % LP);
+  const bool active = o < op.R * 4;
+  const int64_t q = active ? o >> 2 : 0;
+  const int k = int(o & 3);
+  const int row = blockIdx.y;
+  const int64_t N = 12 * op.npf;
+  const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
+  T acc[NI];
+#pragma unroll
+  for (int j = 0; j < NI; ++j) acc[j] = T(0);
+  const int cnt = active ? op.tcnt[q] : 0;
+  for (int s = lane; s < cnt; s += LP) {
+    size_t at = size_t(s) * op.R + q;
+    int64_t pk = rot_pix(op.trow[at], k, op.order);
+    T c[8], a[8];
+#pragma unroll
+    for (int g = 0; g < kL; ++g) {
+      cplx_t<T> v = reinterpret_cast<const cplx_t<T>*>(op.tcoef)[(size_t(g) * op.Tn + s) * op.R + q];
+      c[2 * g] = v.x;
+      c[2 * g + 1] = v.y;
+    }
+    permute<T>(k, c, a);
+#pragma unroll
+    for (int g = 0; g < kL; ++g) {
+      const cplx_t<T>* src = gr + (size_t(g) * N + pk) * NI;
+#pragma unroll
+      for (int j = 0; j < NI; ++j) {
+        cplx_t<T> v = src[j];
+        acc[j] += a[g * 2] * v.x + a[g * 2 + 1] * v.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = LP / 2; m >= 1; m >>= 1)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) acc[j] += __shfl_xor(acc[j], m);
+  if (!active || lane != 0) return;
+  int64_t qk = rot_pix(rep_pixel(q, op.order), k, op.order);
+#pragma unroll
+  for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + qk) * NI + j] = acc[j];
+}
+
 // ---------------------------------------------------------------------------
 // LDS-tiled convolution (levels with nside >= 8).  A workgroup is one TS x TS
 // tile of one face (an aligned range of TS^2 NESTED ids, block index
@@ -280,7 +384,8 @@ template <typename T, int NI, int TS>
 __global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const T* __restrict__ x, int rows,
                                                            cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
   constexpr int SW = TS + 2 * kWin, SA = SW * SW;
-  __shared__ T xs[SA * NI];
+  // Planar [j][pixel]: consecutive pixels in consecutive banks.
+  __shared__ T xs[NI * SA];
   int row;
   const TileInfo ti = tile_info<TS>(op.order, rows, row);
   const int64_t N = 12 * op.npf;
@@ -288,12 +393,16 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const 
   for (int idx = threadIdx.x; idx < SA; idx += TS * TS) {
     int64_t p = ext_pixel(ti.face, ti.x0 - kWin + idx % SW, ti.y0 - kWin + idx / SW, op.order);
 #pragma unroll
-    for (int j = 0; j < NI; ++j) xs[idx * NI + j] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
+    for (int j = 0; j < NI; ++j) xs[j * SA + idx] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
   }
   __syncthreads();
   const int li = threadIdx.x;
   const int pos = (int(compress_bits64(li >> 1)) + kWin) * SW + int(compress_bits64(li)) + kWin;
   const int64_t r = int64_t(ti.fr) * op.npf + ti.base + li;
+  // FP32: rep-frame sums, permuted to the face's gauges at the end (the same
+  // products and additions); FP64 permutes each slot's coefficients (faster
+  // there: more loads in flight).
+  constexpr bool kRepSums = sizeof(T) == 4;
   T acc[NI][kL][2];
 #pragma unroll
   for (int j = 0; j < NI; ++j)
@@ -304,13 +413,18 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const 
   for (int s = 0; s < cnt; ++s) {
     size_t at = size_t(s) * op.R + r;
     int16_t o = op.off[at];
-    int src = (pos + off_dy(o) * SW + off_dx(o)) * NI;
+    int src = pos + off_dy(o) * SW + off_dx(o);
     T c[8], a[8];
     Load<T>::coef8(op.coef + at * 8, c);
-    permute<T>(ti.k, c, a);
+    if constexpr (kRepSums) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) a[i] = c[i];
+    } else {
+      permute<T>(ti.k, c, a);
+    }
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
-      T xv = xs[src + j];
+      T xv = xs[j * SA + src];
 #pragma unroll
       for (int g = 0; g < kL; ++g) {
         acc[j][g][0] += a[g * 2] * xv;
@@ -320,12 +434,21 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const 
   }
   const int64_t p = int64_t(ti.face) * op.npf + ti.base + li;
 #pragma unroll
-  for (int g = 0; g < kL; ++g)
+  for (int j = 0; j < NI; ++j) {
+    T sum[8], out[8];
 #pragma unroll
-    for (int j = 0; j < NI; ++j) y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(acc[j][g][0], acc[j][g][1]);
-  if (mod != nullptr && NI == 1) {
+    for (int g = 0; g < kL; ++g) { sum[g * 2] = acc[j][g][0]; sum[g * 2 + 1] = acc[j][g][1]; }
+    if constexpr (kRepSums) {
+      permute<T>(ti.k, sum, out);
+    } else {
 #pragma unroll
-    for (int g = 0; g < kL; ++g) mod[(size_t(row) * N + p) * kL + g] = hypot(acc[0][g][0], acc[0][g][1]);
+      for (int i = 0; i < 8; ++i) out[i] = sum[i];
+    }
+#pragma unroll
+    for (int g = 0; g < kL; ++g) {
+      y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(out[g * 2], out[g * 2 + 1]);
+      if (mod != nullptr && NI == 1) mod[(size_t(row) * N + p) * kL + g] = hypot(out[g * 2], out[g * 2 + 1]);
+    }
   }
 }
 
@@ -360,7 +483,8 @@ template <typename T, int NI, int TS, int FW>
 __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const T* __restrict__ x, int rows,
                                                            cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
   constexpr int SW = TS + 2 * kWin, SA = SW * SW;
-  __shared__ T xs[FW * SA * NI];
+  // Planar [face][j][pixel].
+  __shared__ T xs[FW * NI * SA];
   int64_t tr;
   int fg, row;
   tile_group<TS, FW>(op.order, rows, tr, fg, row);
@@ -375,7 +499,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
     const int face = 4 * fr + fg * FW + f;
     int64_t p = ext_pixel(face, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order);
 #pragma unroll
-    for (int j = 0; j < NI; ++j) xs[idx * NI + j] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
+    for (int j = 0; j < NI; ++j) xs[(f * NI + j) * SA + pi] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
   }
   __syncthreads();
   const int li = threadIdx.x;
@@ -393,16 +517,17 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
   for (int s = 0; s < cnt; ++s) {
     size_t at = size_t(s) * op.R + r;
     int16_t o = op.off[at];
-    int src = (pos + off_dy(o) * SW + off_dx(o)) * NI;
+    int src = pos + off_dy(o) * SW + off_dx(o);
     T c[8];
     Load<T>::coef8(op.coef + at * 8, c);
 #pragma unroll
     for (int f = 0; f < FW; ++f) {
       T a[8];
-      permute<T>(fg * FW + f, c, a);
+      // FW = 4: one face group, the shift is the (unrolled) face index.
+      permute<T>(FW == 4 ? f : fg * FW + f, c, a);
 #pragma unroll
       for (int j = 0; j < NI; ++j) {
-        T xv = xs[f * SA * NI + src + j];
+        T xv = xs[(f * NI + j) * SA + src];
 #pragma unroll
         for (int g = 0; g < kL; ++g) {
           acc[f][j][g][0] += a[g * 2] * xv;
@@ -431,7 +556,8 @@ template <typename T, int NI, int TS, int GP>
 __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
                                                            T* __restrict__ gx) {
   constexpr int SW = TS + 2 * kWin, SA = SW * SW;
-  __shared__ cplx_t<T> gs[GP * SA * NI];
+  // Planar [gg][j][pixel]: consecutive pixels in consecutive banks.
+  __shared__ cplx_t<T> gs[GP * NI * SA];
   int row;
   const TileInfo ti = tile_info<TS>(op.order, rows, row);
   const int64_t N = 12 * op.npf;
@@ -454,9 +580,14 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
     for (int idx = threadIdx.x; idx < GP * SA; idx += TS * TS) {
       int gg = idx / SA, pi = idx % SA;
       int64_t p = ext_pixel(ti.face, ti.x0 - kWin + pi % SW, ti.y0 - kWin + pi / SW, op.order);
+    // The face shift's conjugation of a gauge is applied to the staged
+    // cotangent: a.y (-v.y) instead of (-a.y) v.y, the same products.
 #pragma unroll
-      for (int j = 0; j < NI; ++j)
-        gs[idx * NI + j] = p >= 0 ? gr[(size_t(g0 + gg) * N + p) * NI + j] : make_c<T>(T(0), T(0));
+      for (int j = 0; j < NI; ++j) {
+        cplx_t<T> v = p >= 0 ? gr[(size_t(g0 + gg) * N + p) * NI + j] : make_c<T>(T(0), T(0));
+        if (gauge_conj(ti.k, g0 + gg)) v.y = -v.y;
+        gs[(gg * NI + j) * SA + pi] = v;
+      }
     }
     __syncthreads();
 #pragma unroll 4
@@ -468,11 +599,10 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
       for (int gg = 0; gg < GP; ++gg) {
         const int g = g0 + gg;
         cplx_t<T> a = tc[(size_t(gauge_src(ti.k, g)) * op.Tn + s) * op.R + q];
-        T ai = gauge_conj(ti.k, g) ? -a.y : a.y;
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
-          cplx_t<T> v = gs[(gg * SA + src) * NI + j];
-          acc[j] += a.x * v.x + ai * v.y;
+          cplx_t<T> v = gs[(gg * NI + j) * SA + src];
+          acc[j] += a.x * v.x + a.y * v.y;
         }
       }
     }
@@ -490,7 +620,8 @@ template <typename T, int NI, int TS, int FW, int GP>
 __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
                                                            int rows, T* __restrict__ gx) {
   constexpr int SW = TS + 2 * kWin, SA = SW * SW;
-  __shared__ cplx_t<T> gs[FW * GP * SA * NI];
+  // Planar [face][gg][j][pixel].
+  __shared__ cplx_t<T> gs[FW * GP * NI * SA];
   int64_t tr;
   int fg, row;
   tile_group<TS, FW>(op.order, rows, tr, fg, row);
@@ -517,9 +648,13 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
       const int k = fg * FW + f;
       const int g = (pass * GP + gg) ^ (2 * (k & 1));
       int64_t p = ext_pixel(4 * fr + k, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order);
+      const bool cj = gauge_conj(k, g);
 #pragma unroll
-      for (int j = 0; j < NI; ++j)
-        gs[idx * NI + j] = p >= 0 ? gr[(size_t(g) * N + p) * NI + j] : make_c<T>(T(0), T(0));
+      for (int j = 0; j < NI; ++j) {
+        cplx_t<T> v = p >= 0 ? gr[(size_t(g) * N + p) * NI + j] : make_c<T>(T(0), T(0));
+        if (cj) v.y = -v.y;
+        gs[((f * GP + gg) * NI + j) * SA + pi] = v;
+      }
     }
     __syncthreads();
 #pragma unroll 2
@@ -533,12 +668,10 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
         const cplx_t<T> a = tc[(size_t(plane) * op.Tn + s) * op.R + q];
 #pragma unroll
         for (int f = 0; f < FW; ++f) {
-          const int k = fg * FW + f;
-          const T ai = gauge_conj(k, plane ^ (2 * (k & 1))) ? -a.y : a.y;
 #pragma unroll
           for (int j = 0; j < NI; ++j) {
-            cplx_t<T> v = gs[((f * GP + gg) * SA + src) * NI + j];
-            acc[f][j] += a.x * v.x + ai * v.y;
+            cplx_t<T> v = gs[((f * GP + gg) * NI + j) * SA + src];
+            acc[f][j] += a.x * v.x + a.y * v.y;
           }
         }
       }
@@ -555,7 +688,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
 // LDS-tiled decimation: a workgroup is a TC x TC tile of coarse pixels of one
 // face and one row; the fine window of their candidates, fine offsets
 // kDecLo..kDecHi around the tile's children ((2 TC + 8)^2 pixels), is staged
-// in LDS and every entry reads it at its stored offset.
+// in LDS (planar [j][pixel]) and every entry reads it at its stored offset.
 template <typename T, int NI, int TC>
 __global__ __launch_bounds__(TC * TC) void k_dec_fwd_tiled(DecOp<T> op, const T* __restrict__ x, int rows,
                                                            T* __restrict__ y) {
@@ -570,7 +703,7 @@ __global__ __launch_bounds__(TC * TC) void k_dec_fwd_tiled(DecOp<T> op, const T*
   for (int idx = threadIdx.x; idx < SA; idx += TC * TC) {
     int64_t p = ext_pixel(ti.face, fx0 + idx % SW, fy0 + idx / SW, op.order_f);
 #pragma unroll
-    for (int j = 0; j < NI; ++j) xs[idx * NI + j] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
+    for (int j = 0; j < NI; ++j) xs[j * SA + idx] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
   }
   __syncthreads();
   const int li = threadIdx.x;
@@ -585,9 +718,9 @@ __global__ __launch_bounds__(TC * TC) void k_dec_fwd_tiled(DecOp<T> op, const T*
     size_t at = size_t(s) * op.Rc + r;
     int16_t o = op.off[at];
     const T v = op.val[at];
-    const int src = (pos + off_dy(o) * SW + off_dx(o)) * NI;
+    const int src = pos + off_dy(o) * SW + off_dx(o);
 #pragma unroll
-    for (int j = 0; j < NI; ++j) acc[j] += v * xs[src + j];
+    for (int j = 0; j < NI; ++j) acc[j] += v * xs[j * SA + src];
   }
   const int64_t p = int64_t(ti.face) * op.npf_c + ti.base + li;
 #pragma unroll
@@ -609,7 +742,7 @@ __global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T*
   for (int idx = threadIdx.x; idx < SA; idx += TF * TF) {
     int64_t p = ext_pixel(ti.face, cx0 + idx % SW, cy0 + idx / SW, op.order_c);
 #pragma unroll
-    for (int j = 0; j < NI; ++j) gs[idx * NI + j] = p >= 0 ? gr[size_t(p) * NI + j] : T(0);
+    for (int j = 0; j < NI; ++j) gs[j * SA + idx] = p >= 0 ? gr[size_t(p) * NI + j] : T(0);
   }
   __syncthreads();
   const int li = threadIdx.x;
@@ -624,9 +757,9 @@ __global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T*
     size_t at = size_t(s) * op.Rf + q;
     int16_t o = op.toff[at];
     const T v = op.tval[at];
-    const int src = (pos + off_dy(o) * SW + off_dx(o)) * NI;
+    const int src = pos + off_dy(o) * SW + off_dx(o);
 #pragma unroll
-    for (int j = 0; j < NI; ++j) acc[j] += v * gs[src + j];
+    for (int j = 0; j < NI; ++j) acc[j] += v * gs[j * SA + src];
   }
   const int64_t p = int64_t(ti.face) * op.npf_f + ti.base + li;
   const size_t o = (size_t(row) * Nf + p) * NI;
@@ -1104,6 +1237,14 @@ template <typename T>
 void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s,
                      int variant) {
   STL_CHECK(NI == 1 || NI == kL, "conv: unsupported inner width");
+  if (variant == kConvSplit) {
+    constexpr int LP = 8;
+    dim3 grid(nblocks(op.R * 4 * LP, 256), rows);
+    if (NI == 1) k_conv_fwd_split<T, 1, LP><<<grid, 256, 0, s>>>(op, x, y, mod);
+    else k_conv_fwd_split<T, kL, LP><<<grid, 256, 0, s>>>(op, x, y, nullptr);
+    STL_HIP(hipGetLastError());
+    return;
+  }
   if (op.tiled) {
     const int TS = op.order >= 4 ? 16 : 8;
     if (variant == kConvTiled32 && op.order >= 5) conv_fwd_tiled<T, 32>(op, NI, x, rows, y, mod, s);
@@ -1150,6 +1291,14 @@ void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows,
                      int variant) {
   STL_CHECK(NI == 1 || NI == kL, "conv adjoint: unsupported inner width");
   STL_CHECK(conv_adj_has_variant<T>(NI, variant), "conv adjoint: no such variant");
+  if (variant == kConvSplit) {
+    constexpr int LP = 8;
+    dim3 grid(nblocks(op.R * 4 * LP, 256), rows);
+    if (NI == 1) k_conv_adj_split<T, 1, LP><<<grid, 256, 0, s>>>(op, gy, gx);
+    else k_conv_adj_split<T, kL, LP><<<grid, 256, 0, s>>>(op, gy, gx);
+    STL_HIP(hipGetLastError());
+    return;
+  }
   if (op.tiled) {
     const int TS = op.order >= 4 ? 16 : 8;
     if (variant == kConvTiled32 && op.order >= 5) conv_adj_tiled<T, 32>(op, NI, gy, rows, gx, s);

@@ -288,22 +288,27 @@ def test_graph_replays_match_eager_launches():
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_forward_conv_variants_agree(precision):
     """The convolution kernel variants (16 x 16 tiles, several faces per
-    workgroup, 8 x 8 tiles, 32 x 32 tiles; chosen per level and kernel by
-    timing unless forced) add each pixel's slots in the same order: the same
-    statistics and gradients."""
+    workgroup, 8 x 8 tiles, 32 x 32 tiles, slots split over lanes; chosen per
+    level and kernel by timing unless forced) agree to rounding."""
     from stl_port import NativeEngine
     nside, J = 64, 5
     rng = np.random.default_rng(31)
     x = rng.standard_normal((1, 12 * nside * nside))
     out = []
-    for v in (0, 1, 2, 3, None):
+    for v in (0, 1, 2, 3, 4, None):
         print(f"[gpu] conv_variant={v} precision={precision}", flush=True)
         with NativeEngine(nside, J, precision=precision, conv_variant=v) as e:
             vals, _ = e.forward(x)
             cot = np.conj(vals) * 0.5 + 1.0
             out.append((vals, e.vjp(cot), e.info()["conv_variants"]))
+            plan = e.plan
     for vals, grad, variants in out[1:]:
-        np.testing.assert_allclose(vals, out[0][0], rtol=1e-12 if precision == "fp64" else 1e-6, atol=0)
+        if precision == "fp64":
+            np.testing.assert_allclose(vals, out[0][0], rtol=1e-12, atol=0)
+        else:  # FP32 sums in another order: per-family error, as for the mixed engine
+            assert max(family_errors(vals, out[0][0], plan).values()) <= 1e-6
         assert rel(grad, out[0][1]) <= (1e-12 if precision == "fp64" else 1e-5)
-    # levels of nside >= 8 are tiled; FP64 has no multi-face branch adjoint
+    # levels of nside >= 8 are tiled; FP64 has no multi-face branch adjoint;
+    # the split gather serves every level of nside <= 64
     assert all(row == [1, 1, 1, 1 if precision == "mixed" else 0] for row in out[1][2][:4])
+    assert all(row == [4, 4, 4, 4] for row in out[4][2])
