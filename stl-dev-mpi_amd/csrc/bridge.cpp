@@ -229,6 +229,13 @@ char* stlp_info(StlHandle* h) {
       for (size_t n = 0; n < cv.size(); ++n)
         o << (n ? ", " : "") << "[" << cv[n][0] << ", " << cv[n][1] << ", " << cv[n][2] << ", " << cv[n][3] << "]";
     }
+    o << "], \"conv_tuning\": [";
+    {
+      const auto tr = h->engine->tuning();
+      for (size_t i = 0; i < tr.size(); ++i)
+        o << (i ? ", " : "") << "[" << tr[i].level << ", " << tr[i].kernel << ", " << tr[i].variant << ", "
+          << tr[i].ms << "]";
+    }
     o << "]";
     o << ", \"timers\": {";
     bool first = true;

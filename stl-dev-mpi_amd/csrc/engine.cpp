@@ -247,6 +247,7 @@ class EngineT final : public Engine {
   double setup_seconds() const override { return setup_s_; }
   std::vector<KernelTimer> timers() const override { return prof_.timers(); }
   std::vector<std::array<int, 4>> conv_variants() const override { return conv_var_; }
+  std::vector<TuneRecord> tuning() const override { return tune_ms_; }
   void reset_timers() override { prof_.reset(); }
 
   void forward_device(const double* x_dev, bool moments, cplx* values, double* variance) override {
@@ -405,7 +406,7 @@ class EngineT final : public Engine {
       gpart2_[n].resize(size_t(nb) * B_ * kL * E);
       gsum_[n].resize(size_t(B_) * kL * E * 2);
       gsum2_[n].resize(size_t(B_) * kL * E);
-      int nz = int(std::min<int64_t>((N + 255) / 256, 256));
+      int nz = int(std::min<int64_t>((N + 1023) / 1024, 1024));  // ~4 pixels per thread
       nblk_z_.push_back(nz);
       zpart_[n].resize(size_t(nz) * B_ * kL * 4);
       zsum_[n].resize(size_t(B_) * kL * 4);
@@ -413,7 +414,7 @@ class EngineT final : public Engine {
       ds1_[n].resize(size_t(B_) * kL);
       ds2_[n].resize(size_t(B_) * kL);
     }
-    nblk_mv_ = int(std::min<int64_t>((N_[0] + 255) / 256, 256));
+    nblk_mv_ = int(std::min<int64_t>((N_[0] + 1023) / 1024, 1024));
     mvpart_.resize(size_t(nblk_mv_) * B_ * 4);
     mvsum_.resize(size_t(B_) * 4);
     gbr_.resize(br_[0].size());
@@ -620,24 +621,27 @@ class EngineT final : public Engine {
           T* gx = NI == 1 ? mg_[0].data() : acc_[0].data();
           return time_launch([&] { launch_conv_adj<T>(op, NI, gy, rows, gx, s_, v); });
         };
-        const double base = timed(kConvTiled);
-        double best = base;
+        // Two interleaved rounds, each candidate's best: no bias from the
+        // order of the candidates or from the first touches of the buffers.
+        std::vector<double> ms(cands.size(), 1e30);
+        for (int round = 0; round < 2; ++round)
+          for (size_t c = 0; c < cands.size(); ++c) ms[c] = std::min(ms[c], timed(cands[c]));
+        double best = ms[0];
         int pick = kConvTiled;
-        for (size_t c = 1; c < cands.size(); ++c) {
-          const double ms = timed(cands[c]);
-          if (ms < 0.97 * base && ms < best) {
-            best = ms;
+        for (size_t c = 1; c < cands.size(); ++c)
+          if (ms[c] < 0.97 * ms[0] && ms[c] < best) {
+            best = ms[c];
             pick = cands[c];
           }
-        }
         conv_var_[n][kind] = pick;
+        for (size_t c = 0; c < cands.size(); ++c) tune_ms_.push_back(TuneRecord{n, kind, cands[c], ms[c]});
         std::lock_guard<std::mutex> lock(tune_mutex());
         tune_cache()[key] = pick;
       }
     }
   }
 
-  // Mean duration of 3 launches after one warm-up (HIP events on the engine stream).
+  // Mean duration of 2 launches after one warm-up (HIP events on the engine stream).
   template <typename F>
   double time_launch(F&& launch) {
     hipEvent_t a, b;
@@ -645,14 +649,14 @@ class EngineT final : public Engine {
     STL_HIP(hipEventCreate(&b));
     launch();
     STL_HIP(hipEventRecord(a, s_));
-    for (int i = 0; i < 3; ++i) launch();
+    for (int i = 0; i < 2; ++i) launch();
     STL_HIP(hipEventRecord(b, s_));
     STL_HIP(hipEventSynchronize(b));
     float ms = 0.f;
     STL_HIP(hipEventElapsedTime(&ms, a, b));
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    return ms / 3.0;
+    return ms / 2.0;
   }
 
   // Device tables of the statistics assembly and of the cotangent scatter.
@@ -745,6 +749,7 @@ class EngineT final : public Engine {
   EngineOptions opt_;
   hipStream_t s_ = nullptr;
   std::vector<std::array<int, 4>> conv_var_;
+  std::vector<TuneRecord> tune_ms_;
   std::vector<GraphEntry> fwd_graphs_, vjp_graphs_;
   Profiler prof_;
   std::vector<Coefficient> plan_;

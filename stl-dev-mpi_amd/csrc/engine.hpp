@@ -41,6 +41,13 @@ struct EngineOptions {
   int conv_variant = -1;
 };
 
+// One timed candidate of the convolution-variant choice: level, kernel
+// (0 forward z, 1 forward branch, 2 adjoint z, 3 adjoint branch), variant, ms.
+struct TuneRecord {
+  int level, kernel, variant;
+  double ms;
+};
+
 // Per-kernel-family timing (HIP events on the engine stream).
 struct KernelTimer {
   std::string name;
@@ -79,6 +86,9 @@ class Engine {
   // Chosen convolution variant per level: [level][forward z, forward branch,
   // adjoint z, adjoint branch].
   virtual std::vector<std::array<int, 4>> conv_variants() const = 0;
+  // Candidate timings of the variant choice made by this engine (empty when
+  // forced or taken from an earlier engine of the process).
+  virtual std::vector<TuneRecord> tuning() const = 0;
   virtual void reset_timers() = 0;
   // One level's operator applied to float64 device arrays (kind 0 conv forward
   // [rows][N] -> [rows][4][N] complex, 1 its adjoint, 2 decimation to level + 1,

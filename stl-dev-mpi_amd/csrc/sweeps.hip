@@ -1030,6 +1030,7 @@ __global__ __launch_bounds__(256) void k_zstats(int64_t N, const cplx_t<T>* __re
   int bl = blockIdx.y;
   const cplx_t<T>* zs = z + size_t(bl) * N;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
   for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < N; p += int64_t(gridDim.x) * blockDim.x) {
     double re = double(zs[p].x), im = double(zs[p].y);
     double h = hypot(re, im);
@@ -1049,6 +1050,7 @@ __global__ __launch_bounds__(256) void k_meanvar(int64_t N, const T* __restrict_
   int b = blockIdx.y;
   const T* xs = x + size_t(b) * N;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
   for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < N; p += int64_t(gridDim.x) * blockDim.x) {
     double a = double(xs[p]);
     double sq = fabs(a) * fabs(a);
