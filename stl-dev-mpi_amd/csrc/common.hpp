@@ -40,7 +40,19 @@ constexpr int kMaxSlots = 64;
 // conjugates the folded coefficient.  For a shift of k faces:
 //   A[rot^k(p), rot^k(c)][g] = A[p, c][g ^ (2 (k & 1))], conjugated when
 //   k = 1 and g < 2, k = 2, or k = 3 and g >= 2.
-// Operators are therefore stored for the faces 0, 4 and 8 only ("rep" rows).
+// Operators are therefore built for the faces 0, 4 and 8 ("rep" rows); the
+// decimation is stored that way.
+//
+// North-south mirror.  z -> -z maps face f < 4 to f + 8 and the in-face
+// (x, y) to (nside - 1 - y, nside - 1 - x); it keeps the longitude, so every
+// gauge keeps its index, and it reverses orientations, so the folded
+// coefficient is conjugated:
+//   A[M(p), M(c)][g] = conj(A[p, c][g])   for every row p of the faces 0-3.
+// The convolution is therefore stored for the faces 0 and 4 only; rows of the
+// faces 8-11 are the mirrors of those of the faces 0-3.  The rows on the
+// equator ring (faces 4-7, x + y = nside - 1, z = 0) are their own mirrors but
+// not mirror-symmetric, so the columns of south cells they read are not the
+// mirrors of the north columns: those entries carry a correction list.
 // ---------------------------------------------------------------------------
 
 __host__ __device__ inline int rot_face(int f, int k) { return (f & ~3) | ((f + k) & 3); }
@@ -69,6 +81,20 @@ __host__ __device__ inline int64_t rep_pixel(int64_t r, int order) {
   return (r / npf) * 4 * npf + (r % npf);
 }
 
+// Mirror of a local NESTED index: (x, y) -> (nside - 1 - y, nside - 1 - x) is
+// the complement of every bit followed by the exchange of the x and y bits.
+__host__ __device__ inline int64_t mirror_loc(int64_t loc, int order) {
+  int64_t v = loc ^ ((int64_t(1) << (2 * order)) - 1);
+  return ((v & 0x5555555555555555LL) << 1) | ((v >> 1) & 0x5555555555555555LL);
+}
+
+__host__ __device__ inline int mirror_face(int f) { return f < 4 ? f + 8 : (f >= 8 ? f - 8 : f); }
+
+__host__ __device__ inline int64_t mirror_pix(int64_t pix, int order) {
+  int64_t npf = int64_t(1) << (2 * order);
+  return int64_t(mirror_face(int(pix >> (2 * order)))) * npf + mirror_loc(pix & (npf - 1), order);
+}
+
 // ---------------------------------------------------------------------------
 // Operator layouts (device memory, ELL by slot: entry (s, r) at s * R + r so
 // that consecutive rep rows read consecutive memory for one slot).
@@ -86,12 +112,15 @@ __host__ __device__ inline int16_t pack_off(int dx, int dy) {
 __host__ __device__ inline int off_dx(int16_t o) { return int(int8_t(o & 0xFF)); }
 __host__ __device__ inline int off_dy(int16_t o) { return int(int8_t((o >> 8) & 0xFF)); }
 
-// Wavelet convolution of one level, complex outputs of kL gauges.
+// Wavelet convolution of one level, complex outputs of kL gauges.  Rows and
+// columns are stored for the faces 0 and 4; the "virtual" rows 2 npf .. 3 npf
+// of the kernels are the mirrors of face 0's rows (the stored row r - 2 npf,
+// written to the face-8 pixel mirror_loc(r - 2 npf)).
 template <typename T>
 struct ConvOp {
   int order = 0;          // log2 nside of the level
   int64_t npf = 0;        // pixels per face
-  int64_t R = 0;          // rep rows = 3 npf
+  int64_t R = 0;          // stored rows = 2 npf
   int S = 0;              // slot capacity of the forward rows
   int Tn = 0;             // slot capacity of the transposed rows
   bool tiled = false;     // every offset within kWin and nside >= 8: LDS-tiled kernels
@@ -103,6 +132,14 @@ struct ConvOp {
   const int32_t* trow = nullptr;  // [Tn][R] NESTED row id (any face), ascending
   const int16_t* toff = nullptr;  // [Tn][R] offset of that row in the cell's extended face frame
   const T* tcoef = nullptr;       // [kL][Tn][R][2] that row's coefficient (its own gauges), gauge planes
+  // Adjoint correction of the mirrored columns: for each south cell read by an
+  // equator-ring row, the true coefficients minus the mirrored ones, summed
+  // after the gather (CSR by cell, rows ascending).
+  int nfix = 0;
+  const int32_t* fix_cell = nullptr;  // [nfix] NESTED cell (faces 8-11)
+  const int32_t* fix_ptr = nullptr;   // [nfix + 1]
+  const int32_t* fix_row = nullptr;   // [entries] NESTED row (faces 4-7)
+  const T* fix_coef = nullptr;        // [entries][kL][2] row's gauges
 };
 
 // Decimation from level n (fine) to level n + 1 (coarse), real weights.

@@ -1,5 +1,7 @@
-// One-off device build of the pyramid operators (FP64 geometry), stored for the
-// rep faces 0, 4, 8.  Compiled with -ffp-contract=off so that products and
+// One-off device build of the pyramid operators (FP64 geometry), built for the
+// rep faces 0, 4, 8; the convolution is then kept for the faces 0 and 4 (the
+// south faces are mirrors of the north ones) plus a correction list of the
+// mirrored columns.  Compiled with -ffp-contract=off so that products and
 // sums round like the reference's scalar code.
 //
 // Convolution (reference conv.rs:73-135 + dia.rs:213-238): per rep row and
@@ -21,11 +23,13 @@
 #include "geometry.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <map>
 #include <mutex>
 #include <tuple>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "healpix_device.hpp"
@@ -541,6 +545,123 @@ void build_conv_transpose(int order, int64_t R, int S, const DevArray<uint8_t>& 
   STL_HIP(hipStreamSynchronize(s));
 }
 
+// Rows of the equator ring of face 4 (local (i, nside - 1 - i), rep index
+// npf + local): slot counts, cells and coefficients, [i][S] and [i][S][8].
+template <typename T>
+__global__ void k_gather_ring(int order, int64_t R, int S, const uint8_t* __restrict__ cnt,
+                              const int32_t* __restrict__ cell, const T* __restrict__ coef, uint8_t* __restrict__ ocnt,
+                              int32_t* __restrict__ ocell, T* __restrict__ ocoef) {
+  const int64_t nside = int64_t(1) << order;
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= nside * S) return;
+  int64_t i = t / S;
+  int s = int(t % S);
+  int64_t r = nside * nside + hp::spread_bits(i) + (hp::spread_bits(nside - 1 - i) << 1);
+  if (s == 0) ocnt[i] = cnt[r];
+  size_t at = size_t(s) * R + r;
+  ocell[t] = cell[at];
+  for (int k = 0; k < 2 * kL; ++k) ocoef[size_t(t) * 2 * kL + k] = coef[at * 2 * kL + k];
+}
+
+// First R2 rows of every plane of a [planes][R3][width] array.
+template <typename E>
+void keep_rows(DevArray<E>& a, size_t planes, int64_t R3, int64_t R2, size_t width, hipStream_t s) {
+  DevArray<E> b(planes * size_t(R2) * width);
+  for (size_t p = 0; p < planes; ++p)
+    STL_HIP(hipMemcpyAsync(b.data() + p * size_t(R2) * width, a.data() + p * size_t(R3) * width,
+                           size_t(R2) * width * sizeof(E), hipMemcpyDeviceToDevice, s));
+  STL_HIP(hipStreamSynchronize(s));
+  a = std::move(b);
+}
+
+// Correction list of the mirrored columns (ConvOp::fix_*) from the equator-ring
+// rows of the faces 4-7: for each of their entries (row p, cell c) with c in a
+// south face, + A[p, c]; with c in a north face, - conj(A[p, c]) at M(c) (what
+// the mirrored column of M(c) holds for p, its own mirror).  Entries of one
+// (cell, row) are summed; exact zeros are dropped.
+template <typename T>
+void build_mirror_fix(int order, int64_t R3, ConvStore<T>& st, hipStream_t s) {
+  const int64_t nside = int64_t(1) << order, npf = nside * nside;
+  const int S = st.S;
+  DevArray<uint8_t> dcnt(nside);
+  DevArray<int32_t> dcell(size_t(nside) * S);
+  DevArray<T> dcoef(size_t(nside) * S * 2 * kL);
+  k_gather_ring<T><<<blocks_for(nside * S, 256), 256, 0, s>>>(order, R3, S, st.cnt.data(), st.cell.data(),
+                                                              st.coef.data(), dcnt.data(), dcell.data(), dcoef.data());
+  STL_HIP(hipGetLastError());
+  std::vector<uint8_t> hcnt(nside);
+  std::vector<int32_t> hcell(dcell.size());
+  std::vector<T> hcoef(dcoef.size());
+  STL_HIP(hipMemcpyAsync(hcnt.data(), dcnt.data(), dcnt.bytes(), hipMemcpyDeviceToHost, s));
+  STL_HIP(hipMemcpyAsync(hcell.data(), dcell.data(), dcell.bytes(), hipMemcpyDeviceToHost, s));
+  STL_HIP(hipMemcpyAsync(hcoef.data(), dcoef.data(), dcoef.bytes(), hipMemcpyDeviceToHost, s));
+  STL_HIP(hipStreamSynchronize(s));
+  std::map<std::pair<int32_t, int32_t>, std::array<double, 2 * kL>> acc;  // (cell, row) -> delta
+  for (int k = 0; k < 4; ++k)
+    for (int64_t i = 0; i < nside; ++i) {
+      int64_t loc = 0;
+      for (int b = 0; b < order; ++b) loc |= (((i >> b) & 1) << (2 * b)) | ((((nside - 1 - i) >> b) & 1) << (2 * b + 1));
+      const int32_t row = int32_t((4 + k) * npf + loc);
+      for (int sl = 0; sl < hcnt[i]; ++sl) {
+        const size_t t = size_t(i) * S + sl;
+        const int64_t c = rot_pix(hcell[t], k, order);
+        const int fc = int(c / npf);
+        if (fc >= 4 && fc < 8) continue;
+        const bool south = fc >= 8;
+        auto& d = acc[{int32_t(south ? c : mirror_pix(c, order)), row}];
+        for (int g = 0; g < kL; ++g) {
+          const int gs = gauge_src(k, g);
+          double re = double(hcoef[t * 2 * kL + 2 * gs]);
+          double im = double(hcoef[t * 2 * kL + 2 * gs + 1]);
+          if (gauge_conj(k, g)) im = -im;
+          // South: + A.  North: - conj(A).
+          d[2 * g] += south ? re : -re;
+          d[2 * g + 1] += im;
+        }
+      }
+    }
+  std::vector<int32_t> fcell, fptr, frow;
+  std::vector<T> fcoef;
+  for (const auto& kv : acc) {
+    bool zero = true;
+    for (double v : kv.second) zero = zero && v == 0.0;
+    if (zero) continue;
+    if (fcell.empty() || fcell.back() != kv.first.first) {
+      fcell.push_back(kv.first.first);
+      fptr.push_back(int32_t(frow.size()));
+    }
+    frow.push_back(kv.first.second);
+    for (double v : kv.second) fcoef.push_back(T(v));
+  }
+  fptr.push_back(int32_t(frow.size()));
+  st.nfix = int(fcell.size());
+  if (st.nfix == 0) return;
+  st.fix_cell.resize(fcell.size());
+  st.fix_ptr.resize(fptr.size());
+  st.fix_row.resize(frow.size());
+  st.fix_coef.resize(fcoef.size());
+  STL_HIP(hipMemcpyAsync(st.fix_cell.data(), fcell.data(), st.fix_cell.bytes(), hipMemcpyHostToDevice, s));
+  STL_HIP(hipMemcpyAsync(st.fix_ptr.data(), fptr.data(), st.fix_ptr.bytes(), hipMemcpyHostToDevice, s));
+  STL_HIP(hipMemcpyAsync(st.fix_row.data(), frow.data(), st.fix_row.bytes(), hipMemcpyHostToDevice, s));
+  STL_HIP(hipMemcpyAsync(st.fix_coef.data(), fcoef.data(), st.fix_coef.bytes(), hipMemcpyHostToDevice, s));
+  STL_HIP(hipStreamSynchronize(s));
+}
+
+// Rows and columns of the faces 0 and 4 only (the faces 8-11 are mirrors).
+template <typename T>
+void mirror_store(int order, ConvStore<T>& st, hipStream_t s) {
+  const int64_t npf = int64_t(1) << (2 * order), R3 = 3 * npf, R2 = 2 * npf;
+  build_mirror_fix<T>(order, R3, st, s);
+  keep_rows(st.cnt, 1, R3, R2, 1, s);
+  keep_rows(st.cell, size_t(st.S), R3, R2, 1, s);
+  keep_rows(st.off, size_t(st.S), R3, R2, 1, s);
+  keep_rows(st.coef, size_t(st.S), R3, R2, 2 * kL, s);
+  keep_rows(st.tcnt, 1, R3, R2, 1, s);
+  keep_rows(st.trow, size_t(st.Tn), R3, R2, 1, s);
+  keep_rows(st.toff, size_t(st.Tn), R3, R2, 1, s);
+  keep_rows(st.tcoef, size_t(kL) * st.Tn, R3, R2, 2, s);
+}
+
 }  // namespace
 
 std::vector<double> wavelet_kernel_f32(int ksz) {
@@ -660,6 +781,8 @@ void build_conv(int order, const std::vector<double>& ker, ConvStore<double>* c6
     c32->coef = std::move(coef32);
   }
   STL_HIP(hipStreamSynchronize(s));
+  if (c64) mirror_store(order, *c64, s);
+  if (c32) mirror_store(order, *c32, s);
 }
 
 void build_dec(int order_f, DecStore<double>* d64, DecStore<float>* d32, hipStream_t s) {

@@ -22,21 +22,28 @@ struct ConvStore {
   DevArray<int32_t> cell, trow;
   DevArray<int16_t> off, toff;
   DevArray<T> coef, tcoef;
+  // Correction of the mirrored columns (ConvOp::fix_*).
+  int nfix = 0;
+  DevArray<int32_t> fix_cell, fix_ptr, fix_row;
+  DevArray<T> fix_coef;
   ConvOp<T> view() const {
     ConvOp<T> v;
     v.order = order;
     v.npf = int64_t(1) << (2 * order);
-    v.R = 3 * v.npf;
+    v.R = 2 * v.npf;
     v.S = S;
     v.Tn = Tn;
     v.tiled = tiled;
     v.cnt = cnt.data(); v.cell = cell.data(); v.off = off.data(); v.coef = coef.data();
     v.tcnt = tcnt.data(); v.trow = trow.data(); v.toff = toff.data(); v.tcoef = tcoef.data();
+    v.nfix = nfix;
+    v.fix_cell = fix_cell.data(); v.fix_ptr = fix_ptr.data(); v.fix_row = fix_row.data();
+    v.fix_coef = fix_coef.data();
     return v;
   }
   size_t bytes() const {
     return cnt.bytes() + tcnt.bytes() + cell.bytes() + trow.bytes() + off.bytes() + toff.bytes() + coef.bytes() +
-           tcoef.bytes();
+           tcoef.bytes() + fix_cell.bytes() + fix_ptr.bytes() + fix_row.bytes() + fix_coef.bytes();
   }
 };
 

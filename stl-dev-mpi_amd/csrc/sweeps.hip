@@ -7,7 +7,9 @@
 // the 4 lanes of a rep row share its coefficients (one fetch per wavefront),
 // each applies them to its own face through the face-rotation symmetry
 // (gauge g reads the rep coefficient of gauge g ^ 2(k & 1), conjugated per
-// gauge_conj).
+// gauge_conj).  Convolution rows come in 3 npf "virtual" rows: the stored
+// rows of the faces 0 and 4, then the mirrors of face 0's rows (conjugated
+// coefficients, cells and outputs at their mirror pixels).
 #include "sweeps.hpp"
 
 #include <algorithm>
@@ -95,8 +97,9 @@ template <typename T, int NI>
 __global__ __launch_bounds__(256) void k_conv_fwd(ConvOp<T> op, const T* __restrict__ x,
                                                   cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
   int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (t >= op.R * 4) return;
-  int64_t r = t >> 2;
+  if (t >= 12 * op.npf) return;
+  const bool m = (t >> 2) >= op.R;
+  int64_t r = m ? (t >> 2) - op.R : t >> 2;
   int k = int(t & 3);
   int row = blockIdx.y;
   int64_t N = 12 * op.npf;
@@ -110,6 +113,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvOp<T> op, const T* __restr
   for (int s = 0; s < cnt; ++s) {
     size_t at = size_t(s) * op.R + r;
     int64_t ck = rot_pix(op.cell[at], k, op.order);
+    if (m) ck = mirror_pix(ck, op.order);
     T c[8], a[8];
     Load<T>::coef8(op.coef + at * 8, c);
     permute<T>(k, c, a);
@@ -124,10 +128,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvOp<T> op, const T* __restr
       }
   }
   int64_t p = rot_pix(rep_pixel(r, op.order), k, op.order);
+  if (m) p = mirror_pix(p, op.order);
 #pragma unroll
   for (int g = 0; g < kL; ++g)
 #pragma unroll
-    for (int j = 0; j < NI; ++j) y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(acc[j][g][0], acc[j][g][1]);
+    for (int j = 0; j < NI; ++j)
+      y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(acc[j][g][0], m ? -acc[j][g][1] : acc[j][g][1]);
   if (mod != nullptr && NI == 1) {
 #pragma unroll
     for (int g = 0; g < kL; ++g) mod[(size_t(row) * N + p) * kL + g] = hypot(acc[0][g][0], acc[0][g][1]);
@@ -139,8 +145,9 @@ template <typename T, int NI>
 __global__ __launch_bounds__(256) void k_conv_adj(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
                                                   T* __restrict__ gx) {
   int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (t >= op.R * 4) return;
-  int64_t q = t >> 2;
+  if (t >= 12 * op.npf) return;
+  const bool m = (t >> 2) >= op.R;
+  int64_t q = m ? (t >> 2) - op.R : t >> 2;
   int k = int(t & 3);
   int row = blockIdx.y;
   int64_t N = 12 * op.npf;
@@ -152,12 +159,13 @@ __global__ __launch_bounds__(256) void k_conv_adj(ConvOp<T> op, const cplx_t<T>*
   for (int s = 0; s < cnt; ++s) {
     size_t at = size_t(s) * op.R + q;
     int64_t pk = rot_pix(op.trow[at], k, op.order);
+    if (m) pk = mirror_pix(pk, op.order);
     T c[8], a[8];
 #pragma unroll
     for (int g = 0; g < kL; ++g) {
       cplx_t<T> v = reinterpret_cast<const cplx_t<T>*>(op.tcoef)[(size_t(g) * op.Tn + s) * op.R + q];
       c[2 * g] = v.x;
-      c[2 * g + 1] = v.y;
+      c[2 * g + 1] = m ? -v.y : v.y;
     }
     permute<T>(k, c, a);
 #pragma unroll
@@ -171,6 +179,7 @@ __global__ __launch_bounds__(256) void k_conv_adj(ConvOp<T> op, const cplx_t<T>*
     }
   }
   int64_t qk = rot_pix(rep_pixel(q, op.order), k, op.order);
+  if (m) qk = mirror_pix(qk, op.order);
 #pragma unroll
   for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + qk) * NI + j] = acc[j];
 }
@@ -184,8 +193,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd_split(ConvOp<T> op, const T* _
                                                         cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
   const int64_t o = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) / LP;
   const int lane = int(threadIdx.x % LP);
-  const bool active = o < op.R * 4;
-  const int64_t r = active ? o >> 2 : 0;
+  const bool active = o < 12 * op.npf;
+  const bool m = active && (o >> 2) >= op.R;
+  const int64_t r = active ? (m ? (o >> 2) - op.R : o >> 2) : 0;
   const int k = int(o & 3);
   const int row = blockIdx.y;
   const int64_t N = 12 * op.npf;
@@ -199,6 +209,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_split(ConvOp<T> op, const T* _
   for (int s = lane; s < cnt; s += LP) {
     size_t at = size_t(s) * op.R + r;
     int64_t ck = rot_pix(op.cell[at], k, op.order);
+    if (m) ck = mirror_pix(ck, op.order);
     T c[8], a[8];
     Load<T>::coef8(op.coef + at * 8, c);
     permute<T>(k, c, a);
@@ -223,10 +234,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd_split(ConvOp<T> op, const T* _
       }
   if (!active || lane != 0) return;
   int64_t p = rot_pix(rep_pixel(r, op.order), k, op.order);
+  if (m) p = mirror_pix(p, op.order);
 #pragma unroll
   for (int g = 0; g < kL; ++g)
 #pragma unroll
-    for (int j = 0; j < NI; ++j) y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(acc[j][g][0], acc[j][g][1]);
+    for (int j = 0; j < NI; ++j)
+      y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(acc[j][g][0], m ? -acc[j][g][1] : acc[j][g][1]);
   if (mod != nullptr && NI == 1) {
 #pragma unroll
     for (int g = 0; g < kL; ++g) mod[(size_t(row) * N + p) * kL + g] = hypot(acc[0][g][0], acc[0][g][1]);
@@ -238,8 +251,9 @@ __global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx
                                                         T* __restrict__ gx) {
   const int64_t o = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) / LP;
   const int lane = int(threadIdx.x % LP);
-  const bool active = o < op.R * 4;
-  const int64_t q = active ? o >> 2 : 0;
+  const bool active = o < 12 * op.npf;
+  const bool m = active && (o >> 2) >= op.R;
+  const int64_t q = active ? (m ? (o >> 2) - op.R : o >> 2) : 0;
   const int k = int(o & 3);
   const int row = blockIdx.y;
   const int64_t N = 12 * op.npf;
@@ -251,12 +265,13 @@ __global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx
   for (int s = lane; s < cnt; s += LP) {
     size_t at = size_t(s) * op.R + q;
     int64_t pk = rot_pix(op.trow[at], k, op.order);
+    if (m) pk = mirror_pix(pk, op.order);
     T c[8], a[8];
 #pragma unroll
     for (int g = 0; g < kL; ++g) {
       cplx_t<T> v = reinterpret_cast<const cplx_t<T>*>(op.tcoef)[(size_t(g) * op.Tn + s) * op.R + q];
       c[2 * g] = v.x;
-      c[2 * g + 1] = v.y;
+      c[2 * g + 1] = m ? -v.y : v.y;
     }
     permute<T>(k, c, a);
 #pragma unroll
@@ -275,16 +290,19 @@ __global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx
     for (int j = 0; j < NI; ++j) acc[j] += __shfl_xor(acc[j], m);
   if (!active || lane != 0) return;
   int64_t qk = rot_pix(rep_pixel(q, op.order), k, op.order);
+  if (m) qk = mirror_pix(qk, op.order);
 #pragma unroll
   for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + qk) * NI + j] = acc[j];
 }
 
 // ---------------------------------------------------------------------------
 // LDS-tiled convolution (levels with nside >= 8).  A workgroup is one TS x TS
-// tile of one face (an aligned range of TS^2 NESTED ids, block index
-// blockIdx.x / 4 over the rep faces, face shift blockIdx.x % 4); the inputs of
-// the tile plus a halo of kWin are staged in LDS from the face's extended
-// frame, and every slot reads LDS at its stored (dx, dy) offset.
+// tile of one face (an aligned range of TS^2 NESTED ids of a stored face, its
+// shift k and mirror flag); the inputs of the tile plus a halo of kWin are
+// staged in LDS from the face's extended frame, and every slot reads LDS at
+// its stored (dx, dy) offset.  A mirrored tile stages, at every position, the
+// mirror of the pixel the north tile stages there, so the same offsets read
+// the mirrored cells.
 // ---------------------------------------------------------------------------
 
 __device__ inline int64_t compress_bits64(int64_t v) {
@@ -339,8 +357,9 @@ __device__ inline int64_t ext_pixel(int face, int ex, int ey, int order) {
 }
 
 struct TileInfo {
-  int face, fr, k, x0, y0;
-  int64_t base;  // local NESTED index of the first pixel of the tile
+  int face, fr, k, x0, y0;  // face = 4 fr + k: the north / equatorial face staged
+  bool m;                   // mirrored (outputs in face 8 + k)
+  int64_t base;             // local NESTED index of the first pixel of the tile
 };
 
 // Workgroup -> (rep tile, face shift k, row).  The 4 * rows workgroups of one
@@ -350,7 +369,65 @@ struct TileInfo {
 // tile group * 8 + b % 8 and way (b / 8) % ways.
 constexpr int kXcds = 8;
 
+// Tile unit u of 3 ntile -> (stored face fr, tile t, mirror m): units
+// 2 ntile .. 3 ntile are the equatorial tiles; below, a north tile and its
+// mirror (the same coefficients) are kXcds units apart in groups of 2 kXcds,
+// so that their workgroups run back to back on one XCD.
+__device__ inline void unit_tile(int64_t u, int64_t ntile, int& fr, int64_t& t, bool& m) {
+  if (u >= 2 * ntile) {
+    fr = 1;
+    t = u - 2 * ntile;
+    m = false;
+    return;
+  }
+  fr = 0;
+  if (ntile % kXcds == 0) {
+    t = (u / (2 * kXcds)) * kXcds + u % kXcds;
+    m = ((u / kXcds) & 1) != 0;
+  } else {
+    t = u >> 1;
+    m = (u & 1) != 0;
+  }
+}
+
+// Pixel staged at extended coordinates (ex, ey) of a tile: face's pixel, or
+// its mirror.
+__device__ inline int64_t staged_pixel(int face, bool m, int ex, int ey, int order) {
+  int64_t p = ext_pixel(face, ex, ey, order);
+  return (m && p >= 0) ? mirror_pix(p, order) : p;
+}
+
+// Staging position pi of a tile's window (SW x SW) -> its LDS index.  A
+// mirrored tile walks the window's columns (iy fastest): they are the rows of
+// the mirrored face, so that consecutive lanes read consecutive pixels along x
+// there, like the north tiles.
+template <int SW>
+__device__ inline int stage_index(bool m, int pi) {
+  return m ? (pi % SW) * SW + pi / SW : pi;
+}
+
+// Lane li of a tile writes output pixel tile_out + li.  In a mirrored tile it
+// is the mirror of rep-local pixel base + rep_lane(li) (the mirror of base + l
+// is the mirrored tile's base plus the mirror of l within the tile), so the
+// lane reads that pixel's row and LDS position, and the writes stay
+// consecutive.
 template <int TS>
+__device__ inline int rep_lane(bool m, int li) {
+  constexpr int kTileOrder = TS == 32 ? 5 : (TS == 16 ? 4 : 3);
+  static_assert((1 << kTileOrder) == TS, "tile side");
+  return m ? int(mirror_loc(li, kTileOrder)) : li;
+}
+
+template <int TS>
+__device__ inline int64_t tile_out(int face, bool m, int64_t base, int order) {
+  const int64_t npf = int64_t(1) << (2 * order);
+  return m ? int64_t(face + 8) * npf + (mirror_loc(base, order) & ~int64_t(TS * TS - 1))
+           : int64_t(face) * npf + base;
+}
+
+// MIRROR: tile units of the convolution (unit_tile); otherwise unit u is tile
+// u % ntile of the stored face u / ntile (decimation: faces 0, 4, 8).
+template <int TS, bool MIRROR>
 __device__ inline TileInfo tile_info(int order, int rows, int& row) {
   TileInfo t;
   const int64_t npf = int64_t(1) << (2 * order);
@@ -372,8 +449,15 @@ __device__ inline TileInfo tile_info(int order, int rows, int& row) {
   }
   t.k = j & 3;
   row = j >> 2;
-  t.fr = int(tr / ntile);
-  t.base = (tr % ntile) * (TS * TS);
+  int64_t tile;
+  if (MIRROR) {
+    unit_tile(tr, ntile, t.fr, tile, t.m);
+  } else {
+    t.fr = int(tr / ntile);
+    tile = tr % ntile;
+    t.m = false;
+  }
+  t.base = tile * (TS * TS);
   t.face = 4 * t.fr + t.k;
   t.x0 = int(compress_bits64(t.base));
   t.y0 = int(compress_bits64(t.base >> 1));
@@ -387,18 +471,19 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const 
   // Planar [j][pixel]: consecutive pixels in consecutive banks.
   __shared__ T xs[NI * SA];
   int row;
-  const TileInfo ti = tile_info<TS>(op.order, rows, row);
+  const TileInfo ti = tile_info<TS, true>(op.order, rows, row);
   const int64_t N = 12 * op.npf;
   const T* xr = x + size_t(row) * N * NI;
-  for (int idx = threadIdx.x; idx < SA; idx += TS * TS) {
-    int64_t p = ext_pixel(ti.face, ti.x0 - kWin + idx % SW, ti.y0 - kWin + idx / SW, op.order);
+  for (int i = threadIdx.x; i < SA; i += TS * TS) {
+    const int idx = stage_index<SW>(ti.m, i);
+    int64_t p = staged_pixel(ti.face, ti.m, ti.x0 - kWin + idx % SW, ti.y0 - kWin + idx / SW, op.order);
 #pragma unroll
     for (int j = 0; j < NI; ++j) xs[j * SA + idx] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
   }
   __syncthreads();
-  const int li = threadIdx.x;
-  const int pos = (int(compress_bits64(li >> 1)) + kWin) * SW + int(compress_bits64(li)) + kWin;
-  const int64_t r = int64_t(ti.fr) * op.npf + ti.base + li;
+  const int li = threadIdx.x, rl = rep_lane<TS>(ti.m, li);
+  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * SW + int(compress_bits64(rl)) + kWin;
+  const int64_t r = int64_t(ti.fr) * op.npf + ti.base + rl;
   // FP32: rep-frame sums, permuted to the face's gauges at the end (the same
   // products and additions); FP64 permutes each slot's coefficients (faster
   // there: more loads in flight).
@@ -432,12 +517,12 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const 
       }
     }
   }
-  const int64_t p = int64_t(ti.face) * op.npf + ti.base + li;
+  const int64_t p = tile_out<TS>(ti.face, ti.m, ti.base, op.order) + li;
 #pragma unroll
   for (int j = 0; j < NI; ++j) {
     T sum[8], out[8];
 #pragma unroll
-    for (int g = 0; g < kL; ++g) { sum[g * 2] = acc[j][g][0]; sum[g * 2 + 1] = acc[j][g][1]; }
+    for (int g = 0; g < kL; ++g) { sum[g * 2] = acc[j][g][0]; sum[g * 2 + 1] = ti.m ? -acc[j][g][1] : acc[j][g][1]; }
     if constexpr (kRepSums) {
       permute<T>(ti.k, sum, out);
     } else {
@@ -457,6 +542,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const 
 // on one XCD as in tile_info.
 template <int TS, int FW>
 __device__ inline void tile_group(int order, int rows, int64_t& tr, int& fg, int& row) {
+  // tr: tile unit (unit_tile).
   const int64_t npf = int64_t(1) << (2 * order);
   const int64_t tiles = 3 * (npf / (TS * TS));
   const int ways = (4 / FW) * rows;
@@ -489,22 +575,25 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
   int fg, row;
   tile_group<TS, FW>(op.order, rows, tr, fg, row);
   const int64_t ntile = op.npf / (TS * TS);
-  const int fr = int(tr / ntile);
-  const int64_t base = (tr % ntile) * (TS * TS);
+  int fr;
+  int64_t tile;
+  bool m;
+  unit_tile(tr, ntile, fr, tile, m);
+  const int64_t base = tile * (TS * TS);
   const int x0 = int(compress_bits64(base)), y0 = int(compress_bits64(base >> 1));
   const int64_t N = 12 * op.npf;
   const T* xr = x + size_t(row) * N * NI;
   for (int idx = threadIdx.x; idx < FW * SA; idx += TS * TS) {
-    const int f = idx / SA, pi = idx % SA;
+    const int f = idx / SA, pi = stage_index<SW>(m, idx % SA);
     const int face = 4 * fr + fg * FW + f;
-    int64_t p = ext_pixel(face, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order);
+    int64_t p = staged_pixel(face, m, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order);
 #pragma unroll
     for (int j = 0; j < NI; ++j) xs[(f * NI + j) * SA + pi] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
   }
   __syncthreads();
-  const int li = threadIdx.x;
-  const int pos = (int(compress_bits64(li >> 1)) + kWin) * SW + int(compress_bits64(li)) + kWin;
-  const int64_t r = int64_t(fr) * op.npf + base + li;
+  const int li = threadIdx.x, rl = rep_lane<TS>(m, li);
+  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * SW + int(compress_bits64(rl)) + kWin;
+  const int64_t r = int64_t(fr) * op.npf + base + rl;
   T acc[FW][NI][kL][2];
 #pragma unroll
   for (int f = 0; f < FW; ++f)
@@ -513,6 +602,9 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
 #pragma unroll
       for (int g = 0; g < kL; ++g) acc[f][j][g][0] = acc[f][j][g][1] = T(0);
   const int cnt = op.cnt[r];
+  // The mirror's conjugation, on the coefficients: 4 products per slot for
+  // all the faces (at the outputs it would hold more registers across the loop).
+  const T sg = m ? T(-1) : T(1);
 #pragma unroll 2
   for (int s = 0; s < cnt; ++s) {
     size_t at = size_t(s) * op.R + r;
@@ -520,6 +612,8 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
     int src = pos + off_dy(o) * SW + off_dx(o);
     T c[8];
     Load<T>::coef8(op.coef + at * 8, c);
+#pragma unroll
+    for (int g = 0; g < kL; ++g) c[2 * g + 1] *= sg;
 #pragma unroll
     for (int f = 0; f < FW; ++f) {
       T a[8];
@@ -538,7 +632,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
   }
 #pragma unroll
   for (int f = 0; f < FW; ++f) {
-    const int64_t p = int64_t(4 * fr + fg * FW + f) * op.npf + base + li;
+    const int64_t p = tile_out<TS>(4 * fr + fg * FW + f, m, base, op.order) + li;
 #pragma unroll
     for (int g = 0; g < kL; ++g)
 #pragma unroll
@@ -559,12 +653,12 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
   // Planar [gg][j][pixel]: consecutive pixels in consecutive banks.
   __shared__ cplx_t<T> gs[GP * NI * SA];
   int row;
-  const TileInfo ti = tile_info<TS>(op.order, rows, row);
+  const TileInfo ti = tile_info<TS, true>(op.order, rows, row);
   const int64_t N = 12 * op.npf;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
-  const int li = threadIdx.x;
-  const int pos = (int(compress_bits64(li >> 1)) + kWin) * SW + int(compress_bits64(li)) + kWin;
-  const int64_t q = int64_t(ti.fr) * op.npf + ti.base + li;
+  const int li = threadIdx.x, rl = rep_lane<TS>(ti.m, li);
+  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * SW + int(compress_bits64(rl)) + kWin;
+  const int64_t q = int64_t(ti.fr) * op.npf + ti.base + rl;
   const int cnt = op.tcnt[q];
   const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
   T acc[NI];
@@ -578,14 +672,16 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
     const int g0 = GP == kL ? 0 : ((pass * GP) ^ (2 * (ti.k & 1)));
     __syncthreads();
     for (int idx = threadIdx.x; idx < GP * SA; idx += TS * TS) {
-      int gg = idx / SA, pi = idx % SA;
-      int64_t p = ext_pixel(ti.face, ti.x0 - kWin + pi % SW, ti.y0 - kWin + pi / SW, op.order);
-    // The face shift's conjugation of a gauge is applied to the staged
-    // cotangent: a.y (-v.y) instead of (-a.y) v.y, the same products.
+      int gg = idx / SA, pi = stage_index<SW>(ti.m, idx % SA);
+      int64_t p = staged_pixel(ti.face, ti.m, ti.x0 - kWin + pi % SW, ti.y0 - kWin + pi / SW, op.order);
+      // The face shift's (and the mirror's) conjugation of a gauge is applied
+      // to the staged cotangent: a.y (-v.y) instead of (-a.y) v.y, the same
+      // products.
+      const bool cj = gauge_conj(ti.k, g0 + gg) != ti.m;
 #pragma unroll
       for (int j = 0; j < NI; ++j) {
         cplx_t<T> v = p >= 0 ? gr[(size_t(g0 + gg) * N + p) * NI + j] : make_c<T>(T(0), T(0));
-        if (gauge_conj(ti.k, g0 + gg)) v.y = -v.y;
+        if (cj) v.y = -v.y;
         gs[(gg * NI + j) * SA + pi] = v;
       }
     }
@@ -607,7 +703,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
       }
     }
   }
-  const int64_t p = int64_t(ti.face) * op.npf + ti.base + li;
+  const int64_t p = tile_out<TS>(ti.face, ti.m, ti.base, op.order) + li;
 #pragma unroll
   for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + p) * NI + j] = acc[j];
 }
@@ -626,14 +722,17 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
   int fg, row;
   tile_group<TS, FW>(op.order, rows, tr, fg, row);
   const int64_t ntile = op.npf / (TS * TS);
-  const int fr = int(tr / ntile);
-  const int64_t base = (tr % ntile) * (TS * TS);
+  int fr;
+  int64_t tile;
+  bool m;
+  unit_tile(tr, ntile, fr, tile, m);
+  const int64_t base = tile * (TS * TS);
   const int x0 = int(compress_bits64(base)), y0 = int(compress_bits64(base >> 1));
   const int64_t N = 12 * op.npf;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
-  const int li = threadIdx.x;
-  const int pos = (int(compress_bits64(li >> 1)) + kWin) * SW + int(compress_bits64(li)) + kWin;
-  const int64_t q = int64_t(fr) * op.npf + base + li;
+  const int li = threadIdx.x, rl = rep_lane<TS>(m, li);
+  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * SW + int(compress_bits64(rl)) + kWin;
+  const int64_t q = int64_t(fr) * op.npf + base + rl;
   const int cnt = op.tcnt[q];
   const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
   T acc[FW][NI];
@@ -644,11 +743,11 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
   for (int pass = 0; pass < kL / GP; ++pass) {
     __syncthreads();
     for (int idx = threadIdx.x; idx < FW * GP * SA; idx += TS * TS) {
-      const int f = idx / (GP * SA), rem = idx % (GP * SA), gg = rem / SA, pi = rem % SA;
+      const int f = idx / (GP * SA), rem = idx % (GP * SA), gg = rem / SA, pi = stage_index<SW>(m, rem % SA);
       const int k = fg * FW + f;
       const int g = (pass * GP + gg) ^ (2 * (k & 1));
-      int64_t p = ext_pixel(4 * fr + k, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order);
-      const bool cj = gauge_conj(k, g);
+      int64_t p = staged_pixel(4 * fr + k, m, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order);
+      const bool cj = gauge_conj(k, g) != m;
 #pragma unroll
       for (int j = 0; j < NI; ++j) {
         cplx_t<T> v = p >= 0 ? gr[(size_t(g) * N + p) * NI + j] : make_c<T>(T(0), T(0));
@@ -679,10 +778,35 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
   }
 #pragma unroll
   for (int f = 0; f < FW; ++f) {
-    const int64_t p = int64_t(4 * fr + fg * FW + f) * op.npf + base + li;
+    const int64_t p = tile_out<TS>(4 * fr + fg * FW + f, m, base, op.order) + li;
 #pragma unroll
     for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + p) * NI + j] = acc[f][j];
   }
+}
+
+// Correction of the mirrored columns, after the gather: thread per (south cell
+// of the list, j), gx += sum over its entries of Re(conj(delta_g) gy[g][row]).
+template <typename T, int NI>
+__global__ __launch_bounds__(256) void k_conv_adj_fix(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
+                                                      T* __restrict__ gx) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(op.nfix) * NI) return;
+  const int i = int(t / NI), j = int(t % NI);
+  const int row = blockIdx.y;
+  const int64_t N = 12 * op.npf;
+  const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
+  const cplx_t<T>* d = reinterpret_cast<const cplx_t<T>*>(op.fix_coef);
+  T acc = T(0);
+  for (int e = op.fix_ptr[i]; e < op.fix_ptr[i + 1]; ++e) {
+    const int64_t p = op.fix_row[e];
+#pragma unroll
+    for (int g = 0; g < kL; ++g) {
+      const cplx_t<T> a = d[size_t(e) * kL + g];
+      const cplx_t<T> v = gr[(size_t(g) * N + p) * NI + j];
+      acc += a.x * v.x + a.y * v.y;
+    }
+  }
+  gx[(size_t(row) * N + op.fix_cell[i]) * NI + j] += acc;
 }
 
 // LDS-tiled decimation: a workgroup is a TC x TC tile of coarse pixels of one
@@ -696,7 +820,7 @@ __global__ __launch_bounds__(TC * TC) void k_dec_fwd_tiled(DecOp<T> op, const T*
   static_assert(SW == 2 * TC + 8, "fine window");
   __shared__ T xs[SA * NI];
   int row;
-  const TileInfo ti = tile_info<TC>(op.order_c, rows, row);
+  const TileInfo ti = tile_info<TC, false>(op.order_c, rows, row);
   const int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
   const T* xr = x + size_t(row) * Nf * NI;
   const int fx0 = 2 * ti.x0 + kDecLo, fy0 = 2 * ti.y0 + kDecLo;
@@ -735,7 +859,7 @@ __global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T*
   constexpr int SW = TF / 2 + 2 * kDecT, SA = SW * SW;
   __shared__ T gs[SA * NI];
   int row;
-  const TileInfo ti = tile_info<TF>(op.order_f, rows, row);
+  const TileInfo ti = tile_info<TF, false>(op.order_f, rows, row);
   const int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
   const T* gr = gy + size_t(row) * Nc * NI;
   const int cx0 = ti.x0 / 2 - kDecT, cy0 = ti.y0 / 2 - kDecT;
@@ -1201,14 +1325,14 @@ __global__ void k_cast(int64_t n, const TI* __restrict__ in, TO* __restrict__ ou
 
 template <typename T, int TS>
 void conv_fwd_tiled(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s) {
-  dim3 grid(unsigned(op.R / (TS * TS) * 4 * rows));
+  dim3 grid(unsigned(3 * op.npf / (TS * TS) * 4 * rows));
   if (NI == 1) k_conv_fwd_tiled<T, 1, TS><<<grid, TS * TS, 0, s>>>(op, x, rows, y, mod);
   else k_conv_fwd_tiled<T, kL, TS><<<grid, TS * TS, 0, s>>>(op, x, rows, y, nullptr);
 }
 
 template <typename T, int TS>
 void conv_adj_tiled(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
-  dim3 grid(unsigned(op.R / (TS * TS) * 4 * rows));
+  dim3 grid(unsigned(3 * op.npf / (TS * TS) * 4 * rows));
   if constexpr (TS == 32) {
     // 42 x 42 staged pixels: one gauge per pass for NI = kL (56 KB FP32,
     // 113 KB FP64), 4 (FP32) or 2 (FP64) gauges for NI = 1.
@@ -1227,10 +1351,10 @@ void conv_fwd_faces(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>
   // FP64 with NI = kL: 2 faces per workgroup (registers); otherwise 4.
   constexpr int FW4 = sizeof(T) == 8 ? 2 : 4;
   if (NI == 1) {
-    dim3 grid(unsigned(op.R / (TS * TS) * rows));
+    dim3 grid(unsigned(3 * op.npf / (TS * TS) * rows));
     k_conv_fwd_faces<T, 1, TS, 4><<<grid, TS * TS, 0, s>>>(op, x, rows, y, mod);
   } else {
-    dim3 grid(unsigned(op.R / (TS * TS) * (4 / FW4) * rows));
+    dim3 grid(unsigned(3 * op.npf / (TS * TS) * (4 / FW4) * rows));
     k_conv_fwd_faces<T, kL, TS, FW4><<<grid, TS * TS, 0, s>>>(op, x, rows, y, nullptr);
   }
 }
@@ -1241,7 +1365,7 @@ void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T
   STL_CHECK(NI == 1 || NI == kL, "conv: unsupported inner width");
   if (variant == kConvSplit) {
     constexpr int LP = 8;
-    dim3 grid(nblocks(op.R * 4 * LP, 256), rows);
+    dim3 grid(nblocks(12 * op.npf * LP, 256), rows);
     if (NI == 1) k_conv_fwd_split<T, 1, LP><<<grid, 256, 0, s>>>(op, x, y, mod);
     else k_conv_fwd_split<T, kL, LP><<<grid, 256, 0, s>>>(op, x, y, nullptr);
     STL_HIP(hipGetLastError());
@@ -1257,7 +1381,7 @@ void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T
     STL_HIP(hipGetLastError());
     return;
   }
-  dim3 grid(nblocks(op.R * 4, 256), rows);
+  dim3 grid(nblocks(12 * op.npf, 256), rows);
   if (NI == 1) k_conv_fwd<T, 1><<<grid, 256, 0, s>>>(op, x, y, mod);
   else if (NI == kL) k_conv_fwd<T, kL><<<grid, 256, 0, s>>>(op, x, y, nullptr);
   else throw std::runtime_error("conv: unsupported inner width");
@@ -1271,12 +1395,12 @@ template <typename T, int TS>
 bool conv_adj_faces(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
   if (NI == 1) {
     constexpr int GP1 = sizeof(T) == 8 ? 1 : 2;
-    dim3 grid(unsigned(op.R / (TS * TS) * rows));
+    dim3 grid(unsigned(3 * op.npf / (TS * TS) * rows));
     k_conv_adj_faces<T, 1, TS, 4, GP1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
     return true;
   }
   if constexpr (sizeof(T) == 4) {
-    dim3 grid(unsigned(op.R / (TS * TS) * 2 * rows));
+    dim3 grid(unsigned(3 * op.npf / (TS * TS) * 2 * rows));
     k_conv_adj_faces<T, kL, TS, 2, 1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
     return true;
   }
@@ -1289,13 +1413,10 @@ bool conv_adj_has_variant(int NI, int variant) {
 }
 
 template <typename T>
-void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s,
-                     int variant) {
-  STL_CHECK(NI == 1 || NI == kL, "conv adjoint: unsupported inner width");
-  STL_CHECK(conv_adj_has_variant<T>(NI, variant), "conv adjoint: no such variant");
+void conv_adj_gather(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s, int variant) {
   if (variant == kConvSplit) {
     constexpr int LP = 8;
-    dim3 grid(nblocks(op.R * 4 * LP, 256), rows);
+    dim3 grid(nblocks(12 * op.npf * LP, 256), rows);
     if (NI == 1) k_conv_adj_split<T, 1, LP><<<grid, 256, 0, s>>>(op, gy, gx);
     else k_conv_adj_split<T, kL, LP><<<grid, 256, 0, s>>>(op, gy, gx);
     STL_HIP(hipGetLastError());
@@ -1311,11 +1432,25 @@ void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows,
     STL_HIP(hipGetLastError());
     return;
   }
-  dim3 grid(nblocks(op.R * 4, 256), rows);
+  dim3 grid(nblocks(12 * op.npf, 256), rows);
   if (NI == 1) k_conv_adj<T, 1><<<grid, 256, 0, s>>>(op, gy, gx);
   else if (NI == kL) k_conv_adj<T, kL><<<grid, 256, 0, s>>>(op, gy, gx);
   else throw std::runtime_error("conv adjoint: unsupported inner width");
   STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s,
+                     int variant) {
+  STL_CHECK(NI == 1 || NI == kL, "conv adjoint: unsupported inner width");
+  STL_CHECK(conv_adj_has_variant<T>(NI, variant), "conv adjoint: no such variant");
+  conv_adj_gather(op, NI, gy, rows, gx, s, variant);
+  if (op.nfix > 0) {
+    dim3 grid(nblocks(int64_t(op.nfix) * NI, 256), rows);
+    if (NI == 1) k_conv_adj_fix<T, 1><<<grid, 256, 0, s>>>(op, gy, gx);
+    else k_conv_adj_fix<T, kL><<<grid, 256, 0, s>>>(op, gy, gx);
+    STL_HIP(hipGetLastError());
+  }
 }
 
 template <typename T>
