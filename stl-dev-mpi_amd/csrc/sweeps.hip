@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_conv_adj(ConvOp<T> op, const cplx_t<T>*
 #pragma unroll
       for (int j = 0; j < NI; ++j) {
         cplx_t<T> v = src[j];
-        acc[j] += a[g * 2] * v.x + a[g * 2 + 1] * v.y;
+        acc[j] = fma(a[g * 2 + 1], v.y, fma(a[g * 2], v.x, acc[j]));
       }
     }
   }
@@ -194,8 +194,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd_split(ConvOp<T> op, const T* _
   const int64_t o = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) / LP;
   const int lane = int(threadIdx.x % LP);
   const bool active = o < 12 * op.npf;
-  const bool m = active && (o >> 2) >= op.R;
-  const int64_t r = active ? (m ? (o >> 2) - op.R : o >> 2) : 0;
+  const bool mir = active && (o >> 2) >= op.R;
+  const int64_t r = active ? (mir ? (o >> 2) - op.R : o >> 2) : 0;
   const int k = int(o & 3);
   const int row = blockIdx.y;
   const int64_t N = 12 * op.npf;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_split(ConvOp<T> op, const T* _
   for (int s = lane; s < cnt; s += LP) {
     size_t at = size_t(s) * op.R + r;
     int64_t ck = rot_pix(op.cell[at], k, op.order);
-    if (m) ck = mirror_pix(ck, op.order);
+    if (mir) ck = mirror_pix(ck, op.order);
     T c[8], a[8];
     Load<T>::coef8(op.coef + at * 8, c);
     permute<T>(k, c, a);
@@ -234,12 +234,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd_split(ConvOp<T> op, const T* _
       }
   if (!active || lane != 0) return;
   int64_t p = rot_pix(rep_pixel(r, op.order), k, op.order);
-  if (m) p = mirror_pix(p, op.order);
+  if (mir) p = mirror_pix(p, op.order);
 #pragma unroll
   for (int g = 0; g < kL; ++g)
 #pragma unroll
     for (int j = 0; j < NI; ++j)
-      y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(acc[j][g][0], m ? -acc[j][g][1] : acc[j][g][1]);
+      y[((size_t(row) * kL + g) * N + p) * NI + j] = make_c<T>(acc[j][g][0], mir ? -acc[j][g][1] : acc[j][g][1]);
   if (mod != nullptr && NI == 1) {
 #pragma unroll
     for (int g = 0; g < kL; ++g) mod[(size_t(row) * N + p) * kL + g] = hypot(acc[0][g][0], acc[0][g][1]);
@@ -252,8 +252,8 @@ __global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx
   const int64_t o = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) / LP;
   const int lane = int(threadIdx.x % LP);
   const bool active = o < 12 * op.npf;
-  const bool m = active && (o >> 2) >= op.R;
-  const int64_t q = active ? (m ? (o >> 2) - op.R : o >> 2) : 0;
+  const bool mir = active && (o >> 2) >= op.R;
+  const int64_t q = active ? (mir ? (o >> 2) - op.R : o >> 2) : 0;
   const int k = int(o & 3);
   const int row = blockIdx.y;
   const int64_t N = 12 * op.npf;
@@ -265,13 +265,13 @@ __global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx
   for (int s = lane; s < cnt; s += LP) {
     size_t at = size_t(s) * op.R + q;
     int64_t pk = rot_pix(op.trow[at], k, op.order);
-    if (m) pk = mirror_pix(pk, op.order);
+    if (mir) pk = mirror_pix(pk, op.order);
     T c[8], a[8];
 #pragma unroll
     for (int g = 0; g < kL; ++g) {
       cplx_t<T> v = reinterpret_cast<const cplx_t<T>*>(op.tcoef)[(size_t(g) * op.Tn + s) * op.R + q];
       c[2 * g] = v.x;
-      c[2 * g + 1] = m ? -v.y : v.y;
+      c[2 * g + 1] = mir ? -v.y : v.y;
     }
     permute<T>(k, c, a);
 #pragma unroll
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx
 #pragma unroll
       for (int j = 0; j < NI; ++j) {
         cplx_t<T> v = src[j];
-        acc[j] += a[g * 2] * v.x + a[g * 2 + 1] * v.y;
+        acc[j] = fma(a[g * 2 + 1], v.y, fma(a[g * 2], v.x, acc[j]));
       }
     }
   }
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx
     for (int j = 0; j < NI; ++j) acc[j] += __shfl_xor(acc[j], m);
   if (!active || lane != 0) return;
   int64_t qk = rot_pix(rep_pixel(q, op.order), k, op.order);
-  if (m) qk = mirror_pix(qk, op.order);
+  if (mir) qk = mirror_pix(qk, op.order);
 #pragma unroll
   for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + qk) * NI + j] = acc[j];
 }
@@ -661,6 +661,19 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
   const int64_t q = int64_t(ti.fr) * op.npf + ti.base + rl;
   const int cnt = op.tcnt[q];
   const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
+  // Window positions this thread stages (LDS index, pixel or -1), the same in
+  // every pass.
+  constexpr int kPer = (SA + TS * TS - 1) / (TS * TS);
+  int spos[kPer];
+  int32_t spix[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int idx = threadIdx.x + i * TS * TS;
+    spos[i] = stage_index<SW>(ti.m, idx < SA ? idx : 0);
+    spix[i] = idx < SA ? int32_t(staged_pixel(ti.face, ti.m, ti.x0 - kWin + spos[i] % SW,
+                                              ti.y0 - kWin + spos[i] / SW, op.order))
+                       : -2;
+  }
   T acc[NI];
 #pragma unroll
   for (int j = 0; j < NI; ++j) acc[j] = T(0);
@@ -671,18 +684,22 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
     // pass and share them in L2.
     const int g0 = GP == kL ? 0 : ((pass * GP) ^ (2 * (ti.k & 1)));
     __syncthreads();
-    for (int idx = threadIdx.x; idx < GP * SA; idx += TS * TS) {
-      int gg = idx / SA, pi = stage_index<SW>(ti.m, idx % SA);
-      int64_t p = staged_pixel(ti.face, ti.m, ti.x0 - kWin + pi % SW, ti.y0 - kWin + pi / SW, op.order);
-      // The face shift's (and the mirror's) conjugation of a gauge is applied
-      // to the staged cotangent: a.y (-v.y) instead of (-a.y) v.y, the same
-      // products.
-      const bool cj = gauge_conj(ti.k, g0 + gg) != ti.m;
 #pragma unroll
-      for (int j = 0; j < NI; ++j) {
-        cplx_t<T> v = p >= 0 ? gr[(size_t(g0 + gg) * N + p) * NI + j] : make_c<T>(T(0), T(0));
-        if (cj) v.y = -v.y;
-        gs[(gg * NI + j) * SA + pi] = v;
+    for (int i = 0; i < kPer; ++i) {
+      if (spix[i] == -2) continue;
+      const int64_t p = spix[i];
+#pragma unroll
+      for (int gg = 0; gg < GP; ++gg) {
+        // The face shift's (and the mirror's) conjugation of a gauge is
+        // applied to the staged cotangent: a.y (-v.y) instead of (-a.y) v.y,
+        // the same products.
+        const bool cj = gauge_conj(ti.k, g0 + gg) != ti.m;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+          cplx_t<T> v = p >= 0 ? gr[(size_t(g0 + gg) * N + p) * NI + j] : make_c<T>(T(0), T(0));
+          if (cj) v.y = -v.y;
+          gs[(gg * NI + j) * SA + spos[i]] = v;
+        }
       }
     }
     __syncthreads();
@@ -698,7 +715,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
           cplx_t<T> v = gs[(gg * NI + j) * SA + src];
-          acc[j] += a.x * v.x + a.y * v.y;
+          acc[j] = fma(a.y, v.y, fma(a.x, v.x, acc[j]));
         }
       }
     }
@@ -735,6 +752,22 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
   const int64_t q = int64_t(fr) * op.npf + base + rl;
   const int cnt = op.tcnt[q];
   const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
+  // Window positions (face f, pixel) this thread stages: LDS index of
+  // (f, gg = 0, j = 0) and the pixel (or -1; -2 past the window), the same in
+  // every pass.
+  constexpr int kFace = GP * NI * SA;
+  constexpr int kPer = (FW * SA + TS * TS - 1) / (TS * TS);
+  int spos[kPer];
+  int32_t spix[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int idx = threadIdx.x + i * TS * TS;
+    const bool in = idx < FW * SA;
+    const int f = in ? idx / SA : 0, pi = stage_index<SW>(m, in ? idx % SA : 0);
+    spos[i] = f * kFace + pi;
+    spix[i] = in ? int32_t(staged_pixel(4 * fr + fg * FW + f, m, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order))
+                 : -2;
+  }
   T acc[FW][NI];
 #pragma unroll
   for (int f = 0; f < FW; ++f)
@@ -742,17 +775,21 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
     for (int j = 0; j < NI; ++j) acc[f][j] = T(0);
   for (int pass = 0; pass < kL / GP; ++pass) {
     __syncthreads();
-    for (int idx = threadIdx.x; idx < FW * GP * SA; idx += TS * TS) {
-      const int f = idx / (GP * SA), rem = idx % (GP * SA), gg = rem / SA, pi = stage_index<SW>(m, rem % SA);
-      const int k = fg * FW + f;
-      const int g = (pass * GP + gg) ^ (2 * (k & 1));
-      int64_t p = staged_pixel(4 * fr + k, m, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order);
-      const bool cj = gauge_conj(k, g) != m;
 #pragma unroll
-      for (int j = 0; j < NI; ++j) {
-        cplx_t<T> v = p >= 0 ? gr[(size_t(g) * N + p) * NI + j] : make_c<T>(T(0), T(0));
-        if (cj) v.y = -v.y;
-        gs[((f * GP + gg) * NI + j) * SA + pi] = v;
+    for (int i = 0; i < kPer; ++i) {
+      if (spix[i] == -2) continue;
+      const int64_t p = spix[i];
+      const int k = fg * FW + spos[i] / kFace;
+#pragma unroll
+      for (int gg = 0; gg < GP; ++gg) {
+        const int g = (pass * GP + gg) ^ (2 * (k & 1));
+        const bool cj = gauge_conj(k, g) != m;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+          cplx_t<T> v = p >= 0 ? gr[(size_t(g) * N + p) * NI + j] : make_c<T>(T(0), T(0));
+          if (cj) v.y = -v.y;
+          gs[spos[i] + (gg * NI + j) * SA] = v;
+        }
       }
     }
     __syncthreads();
@@ -770,7 +807,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
 #pragma unroll
           for (int j = 0; j < NI; ++j) {
             cplx_t<T> v = gs[((f * GP + gg) * NI + j) * SA + src];
-            acc[f][j] += a.x * v.x + a.y * v.y;
+            acc[f][j] = fma(a.y, v.y, fma(a.x, v.x, acc[f][j]));
           }
         }
       }
@@ -803,7 +840,7 @@ __global__ __launch_bounds__(256) void k_conv_adj_fix(ConvOp<T> op, const cplx_t
     for (int g = 0; g < kL; ++g) {
       const cplx_t<T> a = d[size_t(e) * kL + g];
       const cplx_t<T> v = gr[(size_t(g) * N + p) * NI + j];
-      acc += a.x * v.x + a.y * v.y;
+      acc = fma(a.y, v.y, fma(a.x, v.x, acc));
     }
   }
   gx[(size_t(row) * N + op.fix_cell[i]) * NI + j] += acc;
