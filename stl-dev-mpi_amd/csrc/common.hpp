@@ -132,6 +132,10 @@ struct ConvOp {
   const int32_t* trow = nullptr;  // [Tn][R] NESTED row id (any face), ascending
   const int16_t* toff = nullptr;  // [Tn][R] offset of that row in the cell's extended face frame
   const T* tcoef = nullptr;       // [kL][Tn][R][2] that row's coefficient (its own gauges), gauge planes
+  // toff in linear form dy * SW + dx for the LDS windows of pitch SW = 42 and
+  // 26 (32 x 32 and 16 x 16 tiles); null where the level has no such tiles.
+  const int16_t* tlin42 = nullptr;  // [Tn][R]
+  const int16_t* tlin26 = nullptr;  // [Tn][R]
   // Adjoint correction of the mirrored columns: for each south cell read by an
   // equator-ring row, the true coefficients minus the mirrored ones, summed
   // after the gather (CSR by cell, rows ascending).

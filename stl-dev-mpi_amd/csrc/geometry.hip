@@ -647,7 +647,20 @@ void build_mirror_fix(int order, int64_t R3, ConvStore<T>& st, hipStream_t s) {
   STL_HIP(hipStreamSynchronize(s));
 }
 
-// Rows and columns of the faces 0 and 4 only (the faces 8-11 are mirrors).
+__global__ void k_lin_off(int64_t n, const int16_t* __restrict__ off, int pitch, int16_t* __restrict__ lin) {
+  int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < n) lin[i] = int16_t(off_dy(off[i]) * pitch + off_dx(off[i]));
+}
+
+void linear_offsets(const DevArray<int16_t>& off, int pitch, DevArray<int16_t>& lin, hipStream_t s) {
+  lin.resize(off.size());
+  k_lin_off<<<blocks_for(int64_t(off.size()), 256), 256, 0, s>>>(int64_t(off.size()), off.data(), pitch, lin.data());
+  STL_HIP(hipGetLastError());
+  STL_HIP(hipStreamSynchronize(s));
+}
+
+// Rows and columns of the faces 0 and 4 only (the faces 8-11 are mirrors),
+// and the linear transposed offsets of the tiled adjoints.
 template <typename T>
 void mirror_store(int order, ConvStore<T>& st, hipStream_t s) {
   const int64_t npf = int64_t(1) << (2 * order), R3 = 3 * npf, R2 = 2 * npf;
@@ -660,6 +673,8 @@ void mirror_store(int order, ConvStore<T>& st, hipStream_t s) {
   keep_rows(st.trow, size_t(st.Tn), R3, R2, 1, s);
   keep_rows(st.toff, size_t(st.Tn), R3, R2, 1, s);
   keep_rows(st.tcoef, size_t(kL) * st.Tn, R3, R2, 2, s);
+  if (st.tiled && order >= 5) linear_offsets(st.toff, 32 + 2 * kWin, st.tlin42, s);
+  if (st.tiled && order >= 4) linear_offsets(st.toff, 16 + 2 * kWin, st.tlin26, s);
 }
 
 }  // namespace

@@ -464,8 +464,14 @@ __device__ inline TileInfo tile_info(int order, int rows, int& row) {
   return t;
 }
 
+// A 32 x 32 tile is a 16-wave workgroup: two of them per CU need 8 waves per
+// SIMD, i.e. at most 64 VGPRs; at 7 waves only one fits.
+template <typename T, int TS>
+constexpr int tile32_waves() { return TS == 32 && sizeof(T) == 4 ? 8 : 1; }
+
 template <typename T, int NI, int TS>
-__global__ __launch_bounds__(TS * TS) void k_conv_fwd_tiled(ConvOp<T> op, const T* __restrict__ x, int rows,
+__global__ __launch_bounds__(TS * TS) __attribute__((amdgpu_waves_per_eu(tile32_waves<T, TS>())))
+void k_conv_fwd_tiled(ConvOp<T> op, const T* __restrict__ x, int rows,
                                                            cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
   constexpr int SW = TS + 2 * kWin, SA = SW * SW;
   // Planar [j][pixel]: consecutive pixels in consecutive banks.
@@ -645,9 +651,34 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
   }
 }
 
+// Element at byte offset lane_bytes of a [planes][R] array's plane: a uniform
+// plane base plus a loop-invariant 32-bit lane offset.
+template <typename E>
+__device__ inline E plane_load(const E* plane, uint32_t lane_bytes) {
+  return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(plane) + lane_bytes);
+}
+
+// Window offset (pitch TS + 2 kWin) of transposed slot s of the cell whose
+// lane offset in an int16 plane is qo2: the stored linear form for 32 x 32 and
+// 16 x 16 tiles, else the packed (dx, dy) decoded.
+template <int TS, typename T>
+__device__ inline int tslot_off(const ConvOp<T>& op, int s, uint32_t qo2) {
+  constexpr int SW = TS + 2 * kWin;
+  const size_t plane = size_t(s) * op.R;
+  if constexpr (TS == 32) {
+    return plane_load(op.tlin42 + plane, qo2);
+  } else if constexpr (TS == 16) {
+    return plane_load(op.tlin26 + plane, qo2);
+  } else {
+    const int16_t o = plane_load(op.toff + plane, qo2);
+    return off_dy(o) * SW + off_dx(o);
+  }
+}
+
 // GP gauges of the cotangent staged per pass.
 template <typename T, int NI, int TS, int GP>
-__global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
+__global__ __launch_bounds__(TS * TS) __attribute__((amdgpu_waves_per_eu(tile32_waves<T, TS>())))
+void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
                                                            T* __restrict__ gx) {
   constexpr int SW = TS + 2 * kWin, SA = SW * SW;
   // Planar [gg][j][pixel]: consecutive pixels in consecutive banks.
@@ -661,17 +692,17 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
   const int64_t q = int64_t(ti.fr) * op.npf + ti.base + rl;
   const int cnt = op.tcnt[q];
   const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
-  // Window positions this thread stages (LDS index, pixel or -1), the same in
-  // every pass.
+  const uint32_t qo2 = uint32_t(q) * 2u, qoc = uint32_t(q) * uint32_t(sizeof(cplx_t<T>));
+  // Pixels (or -1; -2 past the window) of the window positions this thread
+  // stages, the same in every pass; their LDS indexes are recomputed (cheaper
+  // than the registers).
   constexpr int kPer = (SA + TS * TS - 1) / (TS * TS);
-  int spos[kPer];
   int32_t spix[kPer];
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int idx = threadIdx.x + i * TS * TS;
-    spos[i] = stage_index<SW>(ti.m, idx < SA ? idx : 0);
-    spix[i] = idx < SA ? int32_t(staged_pixel(ti.face, ti.m, ti.x0 - kWin + spos[i] % SW,
-                                              ti.y0 - kWin + spos[i] / SW, op.order))
+    const int pi = stage_index<SW>(ti.m, idx < SA ? idx : 0);
+    spix[i] = idx < SA ? int32_t(staged_pixel(ti.face, ti.m, ti.x0 - kWin + pi % SW, ti.y0 - kWin + pi / SW, op.order))
                        : -2;
   }
   T acc[NI];
@@ -688,6 +719,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
     for (int i = 0; i < kPer; ++i) {
       if (spix[i] == -2) continue;
       const int64_t p = spix[i];
+      const int pi = stage_index<SW>(ti.m, threadIdx.x + i * TS * TS);
 #pragma unroll
       for (int gg = 0; gg < GP; ++gg) {
         // The face shift's (and the mirror's) conjugation of a gauge is
@@ -698,20 +730,18 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_tiled(ConvOp<T> op, const 
         for (int j = 0; j < NI; ++j) {
           cplx_t<T> v = p >= 0 ? gr[(size_t(g0 + gg) * N + p) * NI + j] : make_c<T>(T(0), T(0));
           if (cj) v.y = -v.y;
-          gs[(gg * NI + j) * SA + spos[i]] = v;
+          gs[(gg * NI + j) * SA + pi] = v;
         }
       }
     }
     __syncthreads();
 #pragma unroll 4
     for (int s = 0; s < cnt; ++s) {
-      size_t at = size_t(s) * op.R + q;
-      int16_t o = op.toff[at];
-      int src = pos + off_dy(o) * SW + off_dx(o);
+      const int src = pos + tslot_off<TS>(op, s, qo2);
 #pragma unroll
       for (int gg = 0; gg < GP; ++gg) {
         const int g = g0 + gg;
-        cplx_t<T> a = tc[(size_t(gauge_src(ti.k, g)) * op.Tn + s) * op.R + q];
+        const cplx_t<T> a = plane_load(tc + (size_t(gauge_src(ti.k, g)) * op.Tn + s) * op.R, qoc);
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
           cplx_t<T> v = gs[(gg * NI + j) * SA + src];
@@ -752,19 +782,16 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
   const int64_t q = int64_t(fr) * op.npf + base + rl;
   const int cnt = op.tcnt[q];
   const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
-  // Window positions (face f, pixel) this thread stages: LDS index of
-  // (f, gg = 0, j = 0) and the pixel (or -1; -2 past the window), the same in
-  // every pass.
-  constexpr int kFace = GP * NI * SA;
+  const uint32_t qo2 = uint32_t(q) * 2u, qoc = uint32_t(q) * uint32_t(sizeof(cplx_t<T>));
+  // Pixels (or -1; -2 past the window) of the window positions (face f,
+  // pixel) this thread stages, the same in every pass.
   constexpr int kPer = (FW * SA + TS * TS - 1) / (TS * TS);
-  int spos[kPer];
   int32_t spix[kPer];
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int idx = threadIdx.x + i * TS * TS;
     const bool in = idx < FW * SA;
     const int f = in ? idx / SA : 0, pi = stage_index<SW>(m, in ? idx % SA : 0);
-    spos[i] = f * kFace + pi;
     spix[i] = in ? int32_t(staged_pixel(4 * fr + fg * FW + f, m, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order))
                  : -2;
   }
@@ -779,7 +806,9 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
     for (int i = 0; i < kPer; ++i) {
       if (spix[i] == -2) continue;
       const int64_t p = spix[i];
-      const int k = fg * FW + spos[i] / kFace;
+      const int idx = threadIdx.x + i * TS * TS;
+      const int f = idx / SA, pi = stage_index<SW>(m, idx % SA);
+      const int k = fg * FW + f;
 #pragma unroll
       for (int gg = 0; gg < GP; ++gg) {
         const int g = (pass * GP + gg) ^ (2 * (k & 1));
@@ -788,20 +817,18 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
         for (int j = 0; j < NI; ++j) {
           cplx_t<T> v = p >= 0 ? gr[(size_t(g) * N + p) * NI + j] : make_c<T>(T(0), T(0));
           if (cj) v.y = -v.y;
-          gs[spos[i] + (gg * NI + j) * SA] = v;
+          gs[((f * GP + gg) * NI + j) * SA + pi] = v;
         }
       }
     }
     __syncthreads();
 #pragma unroll 2
     for (int s = 0; s < cnt; ++s) {
-      size_t at = size_t(s) * op.R + q;
-      int16_t o = op.toff[at];
-      int src = pos + off_dy(o) * SW + off_dx(o);
+      const int src = pos + tslot_off<TS>(op, s, qo2);
 #pragma unroll
       for (int gg = 0; gg < GP; ++gg) {
         const int plane = pass * GP + gg;
-        const cplx_t<T> a = tc[(size_t(plane) * op.Tn + s) * op.R + q];
+        const cplx_t<T> a = plane_load(tc + (size_t(plane) * op.Tn + s) * op.R, qoc);
 #pragma unroll
         for (int f = 0; f < FW; ++f) {
 #pragma unroll
