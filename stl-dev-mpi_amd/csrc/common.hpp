@@ -105,6 +105,11 @@ __host__ __device__ inline int64_t mirror_pix(int64_t pix, int order) {
 // nside >= 8 (kernel 5, L = 4).
 constexpr int kWin = 5;
 
+// Row pitch (in elements) of the LDS windows of the tiled adjoints, per tile
+// side: 4 or 12 mod 16, so that the 4 rows of a 4 x 4 block of lanes (16 lanes,
+// one LDS cycle of 64-bit reads) start 24 banks apart and fill all 32 banks.
+constexpr int lds_pitch(int ts) { return ts == 32 ? 44 : (ts == 16 ? 28 : 20); }
+
 // Offset (dx, dy) packed as two int8 in an int16.
 __host__ __device__ inline int16_t pack_off(int dx, int dy) {
   return int16_t((dx & 0xFF) | ((dy & 0xFF) << 8));
@@ -129,13 +134,13 @@ struct ConvOp {
   const int16_t* off = nullptr;   // [S][R] offset of that cell in the row's extended face frame
   const T* coef = nullptr;        // [S][R][kL][2]
   const uint8_t* tcnt = nullptr;  // [R] rows reading each rep cell
-  const int32_t* trow = nullptr;  // [Tn][R] NESTED row id (any face), ascending
+  const int32_t* trow = nullptr;  // [Tn][R] NESTED row id (any face), by window offset (dy, dx)
   const int16_t* toff = nullptr;  // [Tn][R] offset of that row in the cell's extended face frame
   const T* tcoef = nullptr;       // [kL][Tn][R][2] that row's coefficient (its own gauges), gauge planes
-  // toff in linear form dy * SW + dx for the LDS windows of pitch SW = 42 and
-  // 26 (32 x 32 and 16 x 16 tiles); null where the level has no such tiles.
-  const int16_t* tlin42 = nullptr;  // [Tn][R]
-  const int16_t* tlin26 = nullptr;  // [Tn][R]
+  // toff in linear form dy * lds_pitch(TS) + dx for the LDS windows of the
+  // 32 x 32 and 16 x 16 tiles; null where the level has no such tiles.
+  const int16_t* tlin32 = nullptr;  // [Tn][R]
+  const int16_t* tlin16 = nullptr;  // [Tn][R]
   // Adjoint correction of the mirrored columns: for each south cell read by an
   // equator-ring row, the true coefficients minus the mirrored ones, summed
   // after the gather (CSR by cell, rows ascending).

@@ -18,8 +18,9 @@
 // zeros dropped, columns ascending.
 //
 // Transposes: every forward entry (row p, cell c) of a rep row is moved by the
-// one face shift that brings c into a rep face; entries of a rep cell are
-// sorted by row id, which fixes the summation order of the adjoints.
+// one face shift that brings c into a rep face; the entries of a rep cell are
+// sorted by their window offset (then row id) for the convolution, by row id
+// for the decimation, which fixes the summation order of the adjoints.
 #include "geometry.hpp"
 
 #include <algorithm>
@@ -407,6 +408,13 @@ __global__ void k_conv_tfill(int64_t R, int S, const uint8_t* __restrict__ cnt, 
   }
 }
 
+// Sort key of a transposed entry: its window offset, dy then dx (the lanes of
+// a tile then read neighbouring LDS words at each slot), then the row id
+// (unique; it orders the entries of untiled levels, whose offsets may repeat).
+__device__ inline int64_t tkey(int16_t off, int32_t row) {
+  return (int64_t((off_dy(off) + 64) * 128 + off_dx(off) + 64) << 32) | uint32_t(row);
+}
+
 template <typename T>
 __global__ void k_conv_tsort(int64_t R, int Tn, const int* __restrict__ tc, uint8_t* __restrict__ tcnt,
                              int32_t* __restrict__ trow, int16_t* __restrict__ toff, T* __restrict__ tcoef) {
@@ -418,11 +426,12 @@ __global__ void k_conv_tsort(int64_t R, int Tn, const int* __restrict__ tc, uint
   for (int a = 1; a < n; ++a) {
     int32_t key = trow[size_t(a) * R + q];
     int16_t ko = toff[size_t(a) * R + q];
+    const int64_t k64 = tkey(ko, key);
     T v[2 * kL];
     for (int g = 0; g < kL; ++g)
       for (int o = 0; o < 2; ++o) v[g * 2 + o] = tcoef[cidx(g, a, o)];
     int b = a - 1;
-    while (b >= 0 && trow[size_t(b) * R + q] > key) {
+    while (b >= 0 && tkey(toff[size_t(b) * R + q], trow[size_t(b) * R + q]) > k64) {
       trow[size_t(b + 1) * R + q] = trow[size_t(b) * R + q];
       toff[size_t(b + 1) * R + q] = toff[size_t(b) * R + q];
       for (int g = 0; g < kL; ++g)
@@ -673,8 +682,8 @@ void mirror_store(int order, ConvStore<T>& st, hipStream_t s) {
   keep_rows(st.trow, size_t(st.Tn), R3, R2, 1, s);
   keep_rows(st.toff, size_t(st.Tn), R3, R2, 1, s);
   keep_rows(st.tcoef, size_t(kL) * st.Tn, R3, R2, 2, s);
-  if (st.tiled && order >= 5) linear_offsets(st.toff, 32 + 2 * kWin, st.tlin42, s);
-  if (st.tiled && order >= 4) linear_offsets(st.toff, 16 + 2 * kWin, st.tlin26, s);
+  if (st.tiled && order >= 5) linear_offsets(st.toff, lds_pitch(32), st.tlin32, s);
+  if (st.tiled && order >= 4) linear_offsets(st.toff, lds_pitch(16), st.tlin16, s);
 }
 
 }  // namespace

@@ -22,7 +22,7 @@ struct ConvStore {
   DevArray<int32_t> cell, trow;
   DevArray<int16_t> off, toff;
   DevArray<T> coef, tcoef;
-  DevArray<int16_t> tlin42, tlin26;  // ConvOp::tlin42 / tlin26
+  DevArray<int16_t> tlin32, tlin16;  // ConvOp::tlin32 / tlin16
   // Correction of the mirrored columns (ConvOp::fix_*).
   int nfix = 0;
   DevArray<int32_t> fix_cell, fix_ptr, fix_row;
@@ -37,7 +37,7 @@ struct ConvStore {
     v.tiled = tiled;
     v.cnt = cnt.data(); v.cell = cell.data(); v.off = off.data(); v.coef = coef.data();
     v.tcnt = tcnt.data(); v.trow = trow.data(); v.toff = toff.data(); v.tcoef = tcoef.data();
-    v.tlin42 = tlin42.data(); v.tlin26 = tlin26.data();
+    v.tlin32 = tlin32.data(); v.tlin16 = tlin16.data();
     v.nfix = nfix;
     v.fix_cell = fix_cell.data(); v.fix_ptr = fix_ptr.data(); v.fix_row = fix_row.data();
     v.fix_coef = fix_coef.data();
@@ -45,7 +45,7 @@ struct ConvStore {
   }
   size_t bytes() const {
     return cnt.bytes() + tcnt.bytes() + cell.bytes() + trow.bytes() + off.bytes() + toff.bytes() + coef.bytes() +
-           tcoef.bytes() + tlin42.bytes() + tlin26.bytes() + fix_cell.bytes() + fix_ptr.bytes() + fix_row.bytes() +
+           tcoef.bytes() + tlin32.bytes() + tlin16.bytes() + fix_cell.bytes() + fix_ptr.bytes() + fix_row.bytes() +
            fix_coef.bytes();
   }
 };

@@ -658,21 +658,26 @@ __device__ inline E plane_load(const E* plane, uint32_t lane_bytes) {
   return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(plane) + lane_bytes);
 }
 
-// Window offset (pitch TS + 2 kWin) of transposed slot s of the cell whose
-// lane offset in an int16 plane is qo2: the stored linear form for 32 x 32 and
-// 16 x 16 tiles, else the packed (dx, dy) decoded.
+// Window offset (LDS pitch lds_pitch(TS)) of transposed slot s of the cell
+// whose lane offset in an int16 plane is qo2: the stored linear form for
+// 32 x 32 and 16 x 16 tiles, else the packed (dx, dy) decoded.
 template <int TS, typename T>
 __device__ inline int tslot_off(const ConvOp<T>& op, int s, uint32_t qo2) {
-  constexpr int SW = TS + 2 * kWin;
   const size_t plane = size_t(s) * op.R;
   if constexpr (TS == 32) {
-    return plane_load(op.tlin42 + plane, qo2);
+    return plane_load(op.tlin32 + plane, qo2);
   } else if constexpr (TS == 16) {
-    return plane_load(op.tlin26 + plane, qo2);
+    return plane_load(op.tlin16 + plane, qo2);
   } else {
     const int16_t o = plane_load(op.toff + plane, qo2);
-    return off_dy(o) * SW + off_dx(o);
+    return off_dy(o) * lds_pitch(TS) + off_dx(o);
   }
+}
+
+// LDS index of window position wi (SW x SW, row-major) in planes of pitch PW.
+template <int SW, int PW>
+__device__ inline int lds_index(int wi) {
+  return (wi / SW) * PW + wi % SW;
 }
 
 // GP gauges of the cotangent staged per pass.
@@ -680,7 +685,8 @@ template <typename T, int NI, int TS, int GP>
 __global__ __launch_bounds__(TS * TS) __attribute__((amdgpu_waves_per_eu(tile32_waves<T, TS>())))
 void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
                                                            T* __restrict__ gx) {
-  constexpr int SW = TS + 2 * kWin, SA = SW * SW;
+  // Window of SW x SW positions, planes of SW rows of pitch PW.
+  constexpr int SW = TS + 2 * kWin, PW = lds_pitch(TS), NW = SW * SW, SA = SW * PW;
   // Planar [gg][j][pixel]: consecutive pixels in consecutive banks.
   __shared__ cplx_t<T> gs[GP * NI * SA];
   int row;
@@ -688,7 +694,7 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
   const int64_t N = 12 * op.npf;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
   const int li = threadIdx.x, rl = rep_lane<TS>(ti.m, li);
-  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * SW + int(compress_bits64(rl)) + kWin;
+  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * PW + int(compress_bits64(rl)) + kWin;
   const int64_t q = int64_t(ti.fr) * op.npf + ti.base + rl;
   const int cnt = op.tcnt[q];
   const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
@@ -696,13 +702,13 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
   // Pixels (or -1; -2 past the window) of the window positions this thread
   // stages, the same in every pass; their LDS indexes are recomputed (cheaper
   // than the registers).
-  constexpr int kPer = (SA + TS * TS - 1) / (TS * TS);
+  constexpr int kPer = (NW + TS * TS - 1) / (TS * TS);
   int32_t spix[kPer];
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int idx = threadIdx.x + i * TS * TS;
-    const int pi = stage_index<SW>(ti.m, idx < SA ? idx : 0);
-    spix[i] = idx < SA ? int32_t(staged_pixel(ti.face, ti.m, ti.x0 - kWin + pi % SW, ti.y0 - kWin + pi / SW, op.order))
+    const int wi = stage_index<SW>(ti.m, idx < NW ? idx : 0);
+    spix[i] = idx < NW ? int32_t(staged_pixel(ti.face, ti.m, ti.x0 - kWin + wi % SW, ti.y0 - kWin + wi / SW, op.order))
                        : -2;
   }
   T acc[NI];
@@ -719,7 +725,7 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
     for (int i = 0; i < kPer; ++i) {
       if (spix[i] == -2) continue;
       const int64_t p = spix[i];
-      const int pi = stage_index<SW>(ti.m, threadIdx.x + i * TS * TS);
+      const int pi = lds_index<SW, PW>(stage_index<SW>(ti.m, threadIdx.x + i * TS * TS));
 #pragma unroll
       for (int gg = 0; gg < GP; ++gg) {
         // The face shift's (and the mirror's) conjugation of a gauge is
@@ -762,7 +768,8 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
 template <typename T, int NI, int TS, int FW, int GP>
 __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
                                                            int rows, T* __restrict__ gx) {
-  constexpr int SW = TS + 2 * kWin, SA = SW * SW;
+  // Window of SW x SW positions, planes of SW rows of pitch PW.
+  constexpr int SW = TS + 2 * kWin, PW = lds_pitch(TS), NW = SW * SW, SA = SW * PW;
   // Planar [face][gg][j][pixel].
   __shared__ cplx_t<T> gs[FW * GP * NI * SA];
   int64_t tr;
@@ -778,21 +785,21 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
   const int64_t N = 12 * op.npf;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
   const int li = threadIdx.x, rl = rep_lane<TS>(m, li);
-  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * SW + int(compress_bits64(rl)) + kWin;
+  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * PW + int(compress_bits64(rl)) + kWin;
   const int64_t q = int64_t(fr) * op.npf + base + rl;
   const int cnt = op.tcnt[q];
   const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
   const uint32_t qo2 = uint32_t(q) * 2u, qoc = uint32_t(q) * uint32_t(sizeof(cplx_t<T>));
   // Pixels (or -1; -2 past the window) of the window positions (face f,
-  // pixel) this thread stages, the same in every pass.
-  constexpr int kPer = (FW * SA + TS * TS - 1) / (TS * TS);
+  // position) this thread stages, the same in every pass.
+  constexpr int kPer = (FW * NW + TS * TS - 1) / (TS * TS);
   int32_t spix[kPer];
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int idx = threadIdx.x + i * TS * TS;
-    const bool in = idx < FW * SA;
-    const int f = in ? idx / SA : 0, pi = stage_index<SW>(m, in ? idx % SA : 0);
-    spix[i] = in ? int32_t(staged_pixel(4 * fr + fg * FW + f, m, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order))
+    const bool in = idx < FW * NW;
+    const int f = in ? idx / NW : 0, wi = stage_index<SW>(m, in ? idx % NW : 0);
+    spix[i] = in ? int32_t(staged_pixel(4 * fr + fg * FW + f, m, x0 - kWin + wi % SW, y0 - kWin + wi / SW, op.order))
                  : -2;
   }
   T acc[FW][NI];
@@ -807,7 +814,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
       if (spix[i] == -2) continue;
       const int64_t p = spix[i];
       const int idx = threadIdx.x + i * TS * TS;
-      const int f = idx / SA, pi = stage_index<SW>(m, idx % SA);
+      const int f = idx / NW, pi = lds_index<SW, PW>(stage_index<SW>(m, idx % NW));
       const int k = fg * FW + f;
 #pragma unroll
       for (int gg = 0; gg < GP; ++gg) {
@@ -1398,13 +1405,13 @@ template <typename T, int TS>
 void conv_adj_tiled(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
   dim3 grid(unsigned(3 * op.npf / (TS * TS) * 4 * rows));
   if constexpr (TS == 32) {
-    // 42 x 42 staged pixels: one gauge per pass for NI = kL (56 KB FP32,
-    // 113 KB FP64), 4 (FP32) or 2 (FP64) gauges for NI = 1.
+    // 42 rows of pitch 44: one gauge per pass for NI = kL (59 KB FP32,
+    // 118 KB FP64), 4 (FP32) or 2 (FP64) gauges for NI = 1.
     constexpr int GP1 = sizeof(T) == 8 ? 2 : 4;
     if (NI == 1) k_conv_adj_tiled<T, 1, TS, GP1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
     else k_conv_adj_tiled<T, kL, TS, 1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
   } else {
-    constexpr int GP4 = sizeof(T) == 8 ? 1 : 2;  // NI = kL: 43 KB of LDS per pass
+    constexpr int GP4 = sizeof(T) == 8 ? 1 : 2;  // NI = kL: 47 KB of LDS per pass
     if (NI == 1) k_conv_adj_tiled<T, 1, TS, 4><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
     else k_conv_adj_tiled<T, kL, TS, GP4><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
   }
@@ -1452,9 +1459,9 @@ void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T
   STL_HIP(hipGetLastError());
 }
 
-// Multi-face adjoint within 43 KB of LDS: FP32 NI = kL: 2 faces, 1 gauge per
+// Multi-face adjoint within 47 KB of LDS: FP32 NI = kL: 2 faces, 1 gauge per
 // pass; NI = 1: 4 faces, 2 (FP32) or 1 (FP64) gauges per pass.  FP64 with
-// NI = kL has no such variant (it would need 86 KB).
+// NI = kL has no such variant (it would need 93 KB).
 template <typename T, int TS>
 bool conv_adj_faces(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
   if (NI == 1) {
