@@ -145,8 +145,11 @@ int stlp_train_front(void* h, const double* x, size_t n, bool b, double* out);
  * L-BFGS of loss = sum_k w_k |S_k(x) - target_k|^2 (cotangent 2 w (S - target))
  * from the device maps x_dev (float64, n = batch * 12 nside^2, updated in
  * place), target interleaved (re, im) [batch, ncoef], weights [ncoef] or NULL
- * (all 1), options as stlp_lbfgs.  The loss of every evaluation goes to
- * loss_history (up to cap).  Returns the number of evaluations or -1. */
+ * (all 1), options as stlp_lbfgs.  Option "ensemble": true makes the batch
+ * one ensemble (the reference's stl-synth --ensemble, synthesis.rs:362-470
+ * without its scatter term): loss = sum_k w_k |mean over the members of S_k -
+ * target_k|^2 with target one row [ncoef].  The loss of every evaluation goes
+ * to loss_history (up to cap).  Returns the number of evaluations or -1. */
 int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* target, const double* weights,
                         const char* options, double* loss_history, size_t cap);
 
@@ -176,6 +179,38 @@ int stlx_ensemble_vjp_device(StlEnsemble* e, const double* cot, size_t n, const 
  * 1 conv adjoint (gy -> gx), 2 decimation forward level -> level + 1,
  * 3 decimation adjoint.  Arrays are float64 on the device. */
 int stlx_apply_operator(StlHandle* h, int kind, int level, int rows, uint64_t in_dev, uint64_t out_dev);
+
+/* Map files and initial maps (no counterpart in the reference's C ABI: they
+ * restate the Rust I/O of its command-line tools, crates/stl-mpi/src/io.rs:18-520,
+ * and the initial-map draws of crates/stl-cli/src/lib.rs:523-540).  Maps are
+ * full-sky NESTED float64; the format follows the extension: ".fits" / ".fit"
+ * HEALPix binary tables (float32 "E" or float64 "D" column; RING tables are
+ * reordered to NESTED on reading), anything else ".npy" (read: f8, f4, i8, i4
+ * of either byte order; written: little-endian f8).  0 or -1. */
+/* Pixels of a map file, read from its header only (io.rs:370-379). */
+int stlx_map_len(const char* path, size_t* out);
+/* The whole map; n must be its length (io.rs:227-233). */
+int stlx_read_map(const char* path, double* out, size_t n);
+/* NESTED pixels [start, end), read at their offset (io.rs:387-416). */
+int stlx_read_map_range(const char* path, size_t start, size_t end, double* out);
+/* A .npy array flattened (C order) with its shape (io.rs:27-49).  out NULL:
+ * size query, the rank to *ndim and, when shape is not NULL, up to *ndim
+ * dimensions to shape.  Otherwise n must be the element count. */
+int stlx_read_npy(const char* path, double* out, size_t n, size_t* shape, size_t* ndim);
+/* Little-endian float64 .npy of the given shape (io.rs:52-78). */
+int stlx_write_npy(const char* path, const double* values, const size_t* shape, size_t ndim);
+/* A map file: HEALPix FITS (column name, dbl: "D" else "E") or a flat .npy
+ * (io.rs:99-170, 52-55). */
+int stlx_write_map(const char* path, const double* values, size_t n, const char* column, bool dbl);
+/* A map written by slices (io.rs:431-520): headers and a file sized for n
+ * pixels, then pixel ranges [start, start + n) at their offsets.  dbl must
+ * be the same in both calls (FITS "D" or "E"; .npy is always f8). */
+int stlx_sliced_create(const char* path, size_t n, const char* column, bool dbl);
+int stlx_sliced_write(const char* path, bool dbl, size_t start, const double* values, size_t n);
+/* Standard normal draws of pixels first .. first + n - 1 for seed:
+ * splitmix64 of the seed and the pixel id, Box-Muller of two 53-bit uniforms
+ * (lib.rs:523-540); the same map whatever the decomposition. */
+int stlx_pixel_normal(uint64_t seed, int64_t first, size_t n, double* out);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,7 @@
 #include "engine.hpp"
 #include "ensemble.hpp"
 #include "lbfgs.hpp"
+#include "mapio.hpp"
 #include "vectors.hpp"
 
 namespace {
@@ -448,6 +449,7 @@ int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* ta
     if (n != h->input_len()) fail("input shape mismatch");
     Json v = Json::parse(options);
     stl::LbfgsOptions opt = lbfgs_options(v);
+    const bool ensemble = v.boolean("ensemble", false);
     h->current.reset();
     const size_t nc = h->ncoef(), nv = size_t(h->batch) * nc;
     std::vector<stl::cplx> values(nv), cot(nv);
@@ -466,12 +468,26 @@ int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* ta
         STL_HIP(hipMemcpyAsync(h->xbuf.data(), p.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         h->engine->forward_device(h->xbuf.data(), false, values.data(), nullptr);
         double loss = 0.0;
-        for (size_t k = 0; k < nv; ++k) {
-          double w = weights ? weights[k % nc] : 1.0;
-          stl::cplx r = values[k] - stl::cplx(target[2 * k], target[2 * k + 1]);
-          double a = std::abs(r);
-          loss += w * (a * a);
-          cot[k] = (2.0 * w) * r;
+        if (ensemble) {
+          // Loss on the mean over the members; each member gets 2 w r / B
+          // (synthesis.rs:376-470 without the scatter term).
+          const double b = double(h->batch);
+          for (size_t i = 0; i < nc; ++i) {
+            stl::cplx mean = 0.0;
+            for (int m = 0; m < h->batch; ++m) mean += values[size_t(m) * nc + i];
+            const double w = weights ? weights[i] : 1.0;
+            const stl::cplx r = mean / b - stl::cplx(target[2 * i], target[2 * i + 1]);
+            loss += w * std::norm(r);
+            for (int m = 0; m < h->batch; ++m) cot[size_t(m) * nc + i] = (2.0 * w / b) * r;
+          }
+        } else {
+          for (size_t k = 0; k < nv; ++k) {
+            double w = weights ? weights[k % nc] : 1.0;
+            stl::cplx r = values[k] - stl::cplx(target[2 * k], target[2 * k + 1]);
+            double a = std::abs(r);
+            loss += w * (a * a);
+            cot[k] = (2.0 * w) * r;
+          }
         }
         if (!std::isfinite(loss)) fail("non-finite loss");
         h->engine->vjp_device(cot.data(), h->gbuf.data());
@@ -560,6 +576,108 @@ int stlx_apply_operator(StlHandle* h, int kind, int level, int rows, uint64_t in
     if (!h || !in_dev || !out_dev) fail("null argument");
     h->engine->apply_operator(kind, level, rows, reinterpret_cast<const double*>(in_dev),
                               reinterpret_cast<double*>(out_dev));
+    return 0;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Map files and initial maps (crates/stl-mpi/src/io.rs, crates/stl-cli/src/lib.rs:523-540).
+// ---------------------------------------------------------------------------
+
+int stlx_map_len(const char* path, size_t* out) {
+  return guard(-1, [&] {
+    if (!path || !out) fail("null argument");
+    *out = stl::io::map_len(path);
+    return 0;
+  });
+}
+
+int stlx_read_map(const char* path, double* out, size_t n) {
+  return guard(-1, [&] {
+    if (!path || (!out && n)) fail("null argument");
+    std::vector<double> v = stl::io::read_map(path);
+    if (v.size() != n)
+      fail(std::string(path) + ": " + std::to_string(v.size()) + " values, buffer of " + std::to_string(n));
+    std::memcpy(out, v.data(), n * sizeof(double));
+    return 0;
+  });
+}
+
+int stlx_read_map_range(const char* path, size_t start, size_t end, double* out) {
+  return guard(-1, [&] {
+    if (!path || (!out && end > start)) fail("null argument");
+    std::vector<double> v = stl::io::read_map_range(path, start, end);
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return 0;
+  });
+}
+
+int stlx_read_npy(const char* path, double* out, size_t n, size_t* shape, size_t* ndim) {
+  return guard(-1, [&] {
+    if (!path || !ndim) fail("null argument");
+    std::vector<size_t> dims;
+    std::vector<double> v = stl::io::read_npy(path, &dims);
+    if (!out) {
+      // Size query: shape (when room) and rank only.
+      if (shape)
+        for (size_t i = 0; i < dims.size() && i < *ndim; ++i) shape[i] = dims[i];
+      *ndim = dims.size();
+      return 0;
+    }
+    if (v.size() != n)
+      fail(std::string(path) + ": " + std::to_string(v.size()) + " values, buffer of " + std::to_string(n));
+    std::memcpy(out, v.data(), n * sizeof(double));
+    if (shape)
+      for (size_t i = 0; i < dims.size() && i < *ndim; ++i) shape[i] = dims[i];
+    *ndim = dims.size();
+    return 0;
+  });
+}
+
+int stlx_write_npy(const char* path, const double* values, const size_t* shape, size_t ndim) {
+  return guard(-1, [&] {
+    if (!path || (!shape && ndim)) fail("null argument");
+    std::vector<size_t> dims(shape, shape + ndim);
+    size_t n = 1;
+    for (size_t d : dims) n *= d;
+    if (!values && n) fail("null argument");
+    stl::io::write_npy(path, values, dims);
+    return 0;
+  });
+}
+
+int stlx_write_map(const char* path, const double* values, size_t n, const char* column, bool dbl) {
+  return guard(-1, [&] {
+    if (!path || (!values && n)) fail("null argument");
+    std::string p(path);
+    auto dot = p.rfind('.');
+    std::string ext = dot == std::string::npos ? "" : p.substr(dot + 1);
+    if (ext == "fits" || ext == "fit") stl::io::write_healpix_fits(p, values, n, column ? column : "VALUE", dbl);
+    else stl::io::write_npy(p, values, {n});
+    return 0;
+  });
+}
+
+int stlx_sliced_create(const char* path, size_t n, const char* column, bool dbl) {
+  return guard(-1, [&] {
+    if (!path) fail("null argument");
+    stl::io::sliced_create(path, n, column ? column : "VALUE", dbl);
+    return 0;
+  });
+}
+
+int stlx_sliced_write(const char* path, bool dbl, size_t start, const double* values, size_t n) {
+  return guard(-1, [&] {
+    if (!path || (!values && n)) fail("null argument");
+    stl::io::sliced_write(path, dbl, start, values, n);
+    return 0;
+  });
+}
+
+int stlx_pixel_normal(uint64_t seed, int64_t first, size_t n, double* out) {
+  return guard(-1, [&] {
+    if (!out && n) fail("null argument");
+    for (size_t i = 0; i < n; ++i) out[i] = stl::io::pixel_normal(seed, first + int64_t(i));
     return 0;
   });
 }

@@ -27,17 +27,17 @@ __device__ inline double remainder_py(double a, double b) {
   return m;
 }
 
-__device__ inline int64_t floordiv(int64_t a, int64_t b) {
+__host__ __device__ inline int64_t floordiv(int64_t a, int64_t b) {
   int64_t q = a / b;
   if ((a % b != 0) && ((a < 0) != (b < 0))) q -= 1;
   return q;
 }
 
 // Base-face row / column offsets (healpix_base jrll, jpll).
-__device__ inline int64_t jrll(int f) { return 2 + f / 4; }
-__device__ inline int64_t jpll(int f) { return 2 * (f & 3) + ((f >> 2) == 1 ? 0 : 1); }
+__host__ __device__ inline int64_t jrll(int f) { return 2 + f / 4; }
+__host__ __device__ inline int64_t jpll(int f) { return 2 * (f & 3) + ((f >> 2) == 1 ? 0 : 1); }
 
-__device__ inline int64_t compress_bits(int64_t v) {
+__host__ __device__ inline int64_t compress_bits(int64_t v) {
   v &= 0x5555555555555555LL;
   v = (v | (v >> 1)) & 0x3333333333333333LL;
   v = (v | (v >> 2)) & 0x0F0F0F0F0F0F0F0FLL;
@@ -46,7 +46,7 @@ __device__ inline int64_t compress_bits(int64_t v) {
   return (v | (v >> 16)) & 0x00000000FFFFFFFFLL;
 }
 
-__device__ inline int64_t spread_bits(int64_t v) {
+__host__ __device__ inline int64_t spread_bits(int64_t v) {
   v &= 0x00000000FFFFFFFFLL;
   v = (v | (v << 16)) & 0x0000FFFF0000FFFFLL;
   v = (v | (v << 8)) & 0x00FF00FF00FF00FFLL;
@@ -62,7 +62,7 @@ __device__ inline void nest2xyf(int64_t pix, int order, int64_t& ix, int64_t& iy
   iy = compress_bits(local >> 1);
 }
 
-__device__ inline int64_t xyf2nest(int64_t ix, int64_t iy, int face, int order) {
+__host__ __device__ inline int64_t xyf2nest(int64_t ix, int64_t iy, int face, int order) {
   return (int64_t(face) << (2 * order)) + spread_bits(ix) + (spread_bits(iy) << 1);
 }
 
@@ -107,7 +107,7 @@ __device__ inline void centre_lonlat(int64_t pix, int order, double& lon, double
   lat = deg2rad(90.0 - rad2deg(theta));
 }
 
-__device__ inline int64_t isqrt(int64_t v) {
+__host__ __device__ inline int64_t isqrt(int64_t v) {
   int64_t r = int64_t(sqrt(double(v)));
   if (r * r > v) r -= 1;
   if ((r + 1) * (r + 1) <= v) r += 1;
@@ -115,7 +115,7 @@ __device__ inline int64_t isqrt(int64_t v) {
 }
 
 // RING -> NESTED (healpix.rs:157-210).
-__device__ inline int64_t ring2nest(int64_t pix, int order) {
+__host__ __device__ inline int64_t ring2nest(int64_t pix, int order) {
   int64_t nside = int64_t(1) << order;
   int64_t np = 12 * nside * nside;
   int64_t ncap = 2 * nside * (nside - 1);

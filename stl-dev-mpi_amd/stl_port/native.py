@@ -99,6 +99,15 @@ SIGNATURES = {
     "stlx_ensemble_ncoef": ([_VOID, _SIZE], C.c_int),
     "stlx_ensemble_forward_device": ([_VOID, _UINT64, C.c_size_t, C.c_bool, _DOUBLE, _DOUBLE], C.c_int),
     "stlx_ensemble_vjp_device": ([_VOID, _DOUBLE, C.c_size_t, _UINT64, C.c_size_t], C.c_int),
+    "stlx_map_len": ([C.c_char_p, _SIZE], C.c_int),
+    "stlx_read_map": ([C.c_char_p, _DOUBLE, C.c_size_t], C.c_int),
+    "stlx_read_map_range": ([C.c_char_p, C.c_size_t, C.c_size_t, _DOUBLE], C.c_int),
+    "stlx_read_npy": ([C.c_char_p, _DOUBLE, C.c_size_t, _SIZE, _SIZE], C.c_int),
+    "stlx_write_npy": ([C.c_char_p, _DOUBLE, _SIZE, C.c_size_t], C.c_int),
+    "stlx_write_map": ([C.c_char_p, _DOUBLE, C.c_size_t, C.c_char_p, C.c_bool], C.c_int),
+    "stlx_sliced_create": ([C.c_char_p, C.c_size_t, C.c_char_p, C.c_bool], C.c_int),
+    "stlx_sliced_write": ([C.c_char_p, C.c_bool, C.c_size_t, _DOUBLE, C.c_size_t], C.c_int),
+    "stlx_pixel_normal": ([C.c_uint64, C.c_int64, C.c_size_t, _DOUBLE], C.c_int),
 }
 
 
@@ -373,18 +382,21 @@ class NativeEngine:
             self._current_key = key
 
     def synthesize(self, x, target, weights=None, lr=1.0, max_iter=20, max_eval=None, tolerance_grad=1e-12,
-                   tolerance_change=1e-15, history_size=10):
+                   tolerance_change=1e-15, history_size=10, ensemble=False):
         """L-BFGS on the device of sum_k w_k |S_k(x) - target_k|^2 from the float64 device tensor x
-        [batch, npix] (updated in place); returns the loss of every evaluation."""
+        [batch, npix] (updated in place); returns the loss of every evaluation.  With `ensemble` the
+        batch is one ensemble: the loss is on the mean statistics of the members, target [ncoef]
+        (the reference's stl-synth --ensemble, crates/stl-mpi/src/synthesis.rs:362-470, no scatter term)."""
         with self._lock:
             self._open()
             addr, size = _device_address(x, self.batch * self.npix)
-            t = np.ascontiguousarray(np.asarray(target, dtype=np.complex128).reshape(self._shape()))
+            shape = (self.ncoeff,) if ensemble else self._shape()
+            t = np.ascontiguousarray(np.asarray(target, dtype=np.complex128).reshape(shape))
             w = None if weights is None else np.ascontiguousarray(np.asarray(weights, dtype=np.float64).reshape(-1))
             if w is not None and w.size != self.ncoeff:
                 raise ValueError(f"expected {self.ncoeff} weights")
             opts = dict(lr=lr, max_iter=max_iter, tolerance_grad=tolerance_grad, tolerance_change=tolerance_change,
-                        history_size=history_size)
+                        history_size=history_size, ensemble=bool(ensemble))
             if max_eval is not None:
                 opts["max_eval"] = max_eval
             cap = (max_eval or max_iter * 5 // 4) + 64

@@ -34,6 +34,10 @@ KEEP = {
     "synthesis": lambda name: True,
 }
 
+# Single data files copied unchanged: the nside-64 NESTED convergence map of the reference's
+# statistics and synthesis examples (Takahashi et al. 2017, r000, zs1).
+FILES = ["takahashi/takahashi_r000_zs1_nside64_nested.npy"]
+
 
 def main(src):
     src = Path(src)
@@ -51,6 +55,12 @@ def main(src):
             (dst / case["name"]).chmod(0o755)
         (dst / "index.json").write_text(json.dumps(kept, indent=1) + "\n")
         print(f"{suite}: {len(kept)} of {len(index)} cases")
+    for name in FILES:
+        dst = HERE / name
+        dst.parent.mkdir(exist_ok=True)
+        shutil.copyfile(src / name, dst)
+        dst.chmod(0o644)
+        print(f"{name}: copied")
 
 
 if __name__ == "__main__":
