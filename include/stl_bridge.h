@@ -12,8 +12,8 @@
  *
  * Scope of this library: the GPU backend of the complete one-channel plan
  * (mean, var, S1..S4; kernel 5, 4 orientations, gauge cosmo, fewer = false,
- * dg(j) = j).  The other products the reference's bridge also exports
- * (patches, spectra, U-Net, training, masks, spin 2, cross statistics) are
+ * dg(j) = j), whole-sky or masked averages.  The other products the reference's bridge also exports
+ * (patches, spectra, U-Net, training, spin 2, cross statistics) are
  * present as symbols that fail with "not implemented".
  */
 #ifndef STL_BRIDGE_H
@@ -49,12 +49,13 @@ void stlp_free(StlHandle* h);
 char* stlp_info(StlHandle* h);
 
 /* Forward sweep of x (n = batch * 12 nside^2 float64, host memory): values
- * to out (2 * batch * ncoef), with moments the spatial variances of the
- * terms to var (batch * ncoef) (lib.rs:474-530).  0 or -1. */
+ * to out (2 * batch * K * ncoef, K = 1 without masks), with moments the
+ * spatial variances of the terms to var (batch * K * ncoef) (lib.rs:474-530).
+ * 0 or -1. */
 int stlp_forward(StlHandle* h, const double* x, size_t n, uint64_t generation, uint64_t member, uint64_t objective,
                  bool moments, double* out, double* var);
 /* Map cotangent out (batch * 12 nside^2, host) of the cotangents cot
- * (2 n, n = batch * ncoef) after the forward of the same identity
+ * (2 n, n = batch * K * ncoef) after the forward of the same identity
  * (lib.rs:532-581).  0 or -1. */
 int stlp_vjp(StlHandle* h, const double* cot, size_t n, uint64_t generation, uint64_t member, uint64_t objective,
              double* out);
@@ -69,7 +70,8 @@ int stlp_forward_device(StlHandle* h, uint64_t x_dev, size_t n, uint64_t generat
 int stlp_vjp_device(StlHandle* h, const double* cot, size_t n, uint64_t generation, uint64_t member,
                     uint64_t objective, uint64_t out_dev);
 
-/* Pixels each coefficient averages (ncoef size_t) (lib.rs:456-472). */
+/* Pixels each coefficient averages ([K, ncoef] size_t, K = max(masks, 1))
+ * (lib.rs:456-472). */
 int stlp_native_pixels(StlHandle* h, size_t* out);
 /* Forward on maps given at every level (concatenated [batch, 12 (nside/2^d)^2]
  * over levels d), no decimation; values as stlp_forward; no vjp may follow
@@ -90,8 +92,13 @@ typedef double (*stlp_callback)(const double* x, size_t n, double* grad, void* c
  * Returns the number of evaluations or -1 (lib.rs:714-741). */
 int64_t stlp_lbfgs(double* x, size_t n, const char* options, stlp_callback callback, void* context);
 
-/* Masks: k = 0 restores whole-sky averages; k > 0 is not implemented here
- * (lib.rs:393-422). */
+/* Averages over k masks from the next forward on (lib.rs:393-422):
+ * weights [k, 12 nside^2] (nonzero: in the mask), level n + 1 holding the
+ * parents of the level-n mask (mask.rs:43-70); every coefficient is then the
+ * mean of its terms over the mask at its native level, and values, variances
+ * and cotangents are [batch, k, ncoef].  k = 0 restores the whole sky.  At
+ * most 64 masks here; an empty mask or non-finite weight fails.  Drops the
+ * forward state. */
 int stlp_set_masks(StlHandle* h, const double* weights, size_t k);
 /* Spin-2 front end: not implemented here (lib.rs:625-643). */
 int stlp_spin_fields(StlHandle* h, const double* x, size_t n, size_t level, double* qu, double* chi);

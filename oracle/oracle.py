@@ -66,6 +66,8 @@ def lib():
             "or_engine_ncoef": ([C.c_void_p], C.c_int),
             "or_engine_load": ([C.c_void_p, _D], C.c_int),
             "or_engine_forward": ([C.c_void_p, _D, _D], C.c_int),
+            "or_engine_set_masks": ([C.c_void_p, _D, C.c_int], C.c_int),
+            "or_engine_mask_count": ([C.c_void_p, C.c_int, C.c_int], C.c_int64),
             "or_engine_reverse": ([C.c_void_p, _D, _D], C.c_int),
             "or_engine_analysis_vjp": ([C.c_void_p, _D, _D], C.c_int),
             "or_engine_get_map": ([C.c_void_p, C.c_int, _D], C.c_int),
@@ -249,14 +251,32 @@ class Engine:
         x = np.ascontiguousarray(x, np.float64).reshape(self.B * self.C * self.npix[0])
         _check(lib().or_engine_load(self.h, _p(x)))
 
+    nmasks = 0
+
+    def set_masks(self, w):
+        """Masked averages (mask.rs): w [K, 12 nside^2] (nonzero: in), or None for the whole sky."""
+        if w is None:
+            _check(lib().or_engine_set_masks(self.h, None, 0))
+            self.nmasks = 0
+            return
+        w = np.ascontiguousarray(np.atleast_2d(w), np.float64)
+        _check(lib().or_engine_set_masks(self.h, _p(w), w.shape[0]))
+        self.nmasks = w.shape[0]
+
+    def mask_count(self, k, n):
+        return int(lib().or_engine_mask_count(self.h, int(k), int(n)))
+
+    def _vshape(self):
+        return (self.B, self.nmasks, self.ncoef) if self.nmasks else (self.B, self.ncoef)
+
     def forward(self, moments=False):
-        v = np.zeros((self.B, self.ncoef), np.complex128)
-        s = np.zeros((self.B, self.ncoef)) if moments else None
+        v = np.zeros(self._vshape(), np.complex128)
+        s = np.zeros(self._vshape()) if moments else None
         _check(lib().or_engine_forward(self.h, _p(v.view(np.float64)), _p(s) if moments else None))
         return (v, s) if moments else v
 
     def reverse(self, cot):
-        cot = np.ascontiguousarray(cot, np.complex128).reshape(self.B, self.ncoef)
+        cot = np.ascontiguousarray(cot, np.complex128).reshape(self._vshape())
         total = sum(self.B * self.C * n for n in self.npix)
         mg = np.zeros(total)
         _check(lib().or_engine_reverse(self.h, _p(cot.view(np.float64)), _p(mg)))

@@ -920,6 +920,43 @@ struct Engine {
   std::vector<std::vector<double>> maps;           // [B*C][N_d]
   std::map<std::pair<int, int>, std::vector<double>> m;  // (d, n): [B*C*L][N_n]
   std::vector<std::vector<cplx>> zc;                 // cached z_n [B*C][L][N_n]
+  // Masks (mask.rs:14-70): mw[k][n][p] = 1 on the pixels of mask k at level
+  // n (level n + 1: the parents of level n), counts mcnt[k][n]; none: the
+  // whole sky.
+  std::vector<std::vector<std::vector<uint8_t>>> mw;
+  std::vector<std::vector<int64_t>> mcnt;
+
+  int nmask() const { return mw.empty() ? 1 : int(mw.size()); }
+
+  // Masks::from_weights (mask.rs:43-70): nonzero level-0 weights are in.
+  void set_masks(const double* w, int k) {
+    mw.clear();
+    mcnt.clear();
+    for (int q = 0; q < k; ++q) {
+      std::vector<uint8_t> lev(size_t(pixels(0)));
+      for (int64_t p = 0; p < pixels(0); ++p) {
+        double v = w[size_t(q) * pixels(0) + p];
+        if (!std::isfinite(v)) throw std::runtime_error("mask weights must be finite");
+        lev[p] = v != 0.0;
+      }
+      std::vector<std::vector<uint8_t>> per;
+      std::vector<int64_t> cnt;
+      for (int n = 0; n < levels; ++n) {
+        if (n > 0) {
+          std::vector<uint8_t> up(lev.size() / 4);
+          for (size_t p = 0; p < up.size(); ++p) up[p] = lev[4 * p] | lev[4 * p + 1] | lev[4 * p + 2] | lev[4 * p + 3];
+          lev.swap(up);
+        }
+        int64_t c = 0;
+        for (uint8_t v : lev) c += v;
+        if (c == 0) throw std::runtime_error("empty mask");
+        per.push_back(lev);
+        cnt.push_back(c);
+      }
+      mw.push_back(per);
+      mcnt.push_back(cnt);
+    }
+  }
 
   int64_t nside(int d) const { return nside0 >> d; }
   int64_t pixels(int d) const { return npix(nside(d)); }
@@ -1041,10 +1078,12 @@ struct Engine {
     }
   }
 
-  // term_sums / term_second_sums (engine.rs:1194-1240), blocks of block_pixels.
+  // term_sums / term_second_sums (engine.rs:1194-1240), blocks of block_pixels;
+  // with a mask (sel not null) the pixels of the mask only, over n_total
+  // = its pixel count (engine.rs:521-540).
   void partial(const Coef& c, const std::vector<cplx>& a, const std::vector<cplx>& b, int64_t N, cplx* values,
-               double* second) {
-    double n_total = double(N);
+               double* second, const uint8_t* sel = nullptr, double n_total = 0.0) {
+    if (!sel) n_total = double(N);
     for (int bi = 0; bi < B; ++bi) {
       const cplx* pa = &a[size_t(bi) * N];
       const cplx* pb = b.empty() ? nullptr : &b[size_t(bi) * N];
@@ -1053,6 +1092,7 @@ struct Engine {
         cplx sum(0.0, 0.0);
         double sum2 = 0.0;
         for (int64_t p = s0; p < s1; ++p) {
+          if (sel && !sel[p]) continue;
           cplx t;
           double t2;
           switch (c.family) {
@@ -1074,11 +1114,13 @@ struct Engine {
     }
   }
 
-  // Forward (engine.rs:500-602); values [B][np], second [B][np] when moments.
+  // Forward (engine.rs:500-602); values [B][K][np], second [B][K][np] when
+  // moments (K masks, 1 without).
   void forward(cplx* values, double* second) {
     size_t np = plan.size();
-    for (size_t i = 0; i < size_t(B) * np; ++i) values[i] = cplx(0.0, 0.0);
-    if (second) std::fill(second, second + size_t(B) * np, 0.0);
+    const int K = nmask();
+    for (size_t i = 0; i < size_t(B) * K * np; ++i) values[i] = cplx(0.0, 0.0);
+    if (second) std::fill(second, second + size_t(B) * K * np, 0.0);
     m.clear();
     for (int d = 0; d < levels; ++d)
       for (int n = d + 1; n < levels; ++n) m[{d, n}].assign(size_t(rows()) * L * pixels(n), 0.0);
@@ -1090,12 +1132,18 @@ struct Engine {
         if (plan[i].native_dg != n) continue;
         std::vector<cplx> a, b;
         operands(plan[i], n, branches, a, b);
-        std::fill(vb.begin(), vb.end(), cplx());
-        std::fill(sb.begin(), sb.end(), 0.0);
-        partial(plan[i], a, b, pixels(plan[i].family <= F_VAR ? 0 : n), vb.data(), second ? sb.data() : nullptr);
-        for (int bi = 0; bi < B; ++bi) {
-          values[bi * np + i] = cplx(values[bi * np + i].real() + vb[bi].real(), values[bi * np + i].imag() + vb[bi].imag());
-          if (second) second[bi * np + i] += sb[bi];
+        const int lev = plan[i].family <= F_VAR ? 0 : n;
+        for (int k = 0; k < K; ++k) {
+          std::fill(vb.begin(), vb.end(), cplx());
+          std::fill(sb.begin(), sb.end(), 0.0);
+          const uint8_t* sel = mw.empty() ? nullptr : mw[k][lev].data();
+          double total = mw.empty() ? 0.0 : double(mcnt[k][lev]);
+          partial(plan[i], a, b, pixels(lev), vb.data(), second ? sb.data() : nullptr, sel, total);
+          for (int bi = 0; bi < B; ++bi) {
+            size_t o = (size_t(bi) * K + k) * np + i;
+            values[o] = cplx(values[o].real() + vb[bi].real(), values[o].imag() + vb[bi].imag());
+            if (second) second[o] += sb[bi];
+          }
         }
       }
       // m[n][n2] = D^(n2-n) |z_n|
@@ -1151,12 +1199,14 @@ struct Engine {
     }
   }
 
-  // Reverse (engine.rs:610-724, 1062-1174): map cotangents of every level.
+  // Reverse (engine.rs:610-724, 1062-1174): map cotangents of every level
+  // for the cotangents gradient [B][K][np].
   std::vector<std::vector<double>> reverse(const cplx* gradient) {
     size_t np = plan.size();
+    const int K = nmask();
     std::vector<bool> active(np, false);
     for (size_t i = 0; i < np; ++i)
-      for (int bi = 0; bi < B; ++bi) active[i] = active[i] || gradient[bi * np + i] != cplx(0.0, 0.0);
+      for (int r = 0; r < B * K; ++r) active[i] = active[i] || gradient[r * np + i] != cplx(0.0, 0.0);
     std::map<std::pair<int, int>, std::vector<double>> gm;
     for (auto& kv : m) gm[kv.first].assign(kv.second.size(), 0.0);
     std::vector<std::vector<double>> mg(levels);
@@ -1211,9 +1261,28 @@ struct Engine {
         if (c.native_dg != n || !active[i]) continue;
         std::vector<cplx> a, b, ga, gb;
         operands(c, n, branches, a, b);
-        for (int bi = 0; bi < B; ++bi) gcoef[bi] = gradient[bi * np + i];
         int64_t nf = (c.family <= F_VAR) ? pixels(0) : N;
-        operand_cotangents(c, a, b, gcoef.data(), nf, double(nf), ga, gb);
+        if (mw.empty()) {
+          for (int bi = 0; bi < B; ++bi) gcoef[bi] = gradient[bi * np + i];
+          operand_cotangents(c, a, b, gcoef.data(), nf, double(nf), ga, gb);
+        } else {
+          // Operand cotangents of every mask on its pixels, added
+          // (engine.rs:640-662).
+          const int lev = c.family <= F_VAR ? 0 : n;
+          std::vector<cplx> gak, gbk;
+          for (int k = 0; k < K; ++k) {
+            for (int bi = 0; bi < B; ++bi) gcoef[bi] = gradient[(size_t(bi) * K + k) * np + i];
+            operand_cotangents(c, a, b, gcoef.data(), nf, double(mcnt[k][lev]), gak, gbk);
+            const uint8_t* sel = mw[k][lev].data();
+            if (ga.empty()) ga.assign(gak.size(), cplx());
+            if (!gbk.empty() && gb.empty()) gb.assign(gbk.size(), cplx());
+            for (size_t q = 0; q < gak.size(); ++q) {
+              if (!sel[q % size_t(nf)]) continue;
+              ga[q] = cplx(ga[q].real() + gak[q].real(), ga[q].imag() + gak[q].imag());
+              if (!gbk.empty()) gb[q] = cplx(gb[q].real() + gbk[q].real(), gb[q].imag() + gbk[q].imag());
+            }
+          }
+        }
         switch (c.family) {
           case F_MEAN:
           case F_VAR: add_map(0, c.c1, ga); break;
@@ -1470,21 +1539,33 @@ int or_engine_load(void* h, const double* x) {
   return guarded([&] { static_cast<Engine*>(h)->load(x); });
 }
 
-// values: [B][np] interleaved (re, im); second: [B][np] or null.
+// k masks [k][12 nside^2] (nonzero: in), or k = 0: the whole sky.
+int or_engine_set_masks(void* h, const double* w, int k) {
+  return guarded([&] { static_cast<Engine*>(h)->set_masks(w, k); });
+}
+
+// Pixels of mask k at level n.
+int64_t or_engine_mask_count(void* h, int k, int n) {
+  Engine* e = static_cast<Engine*>(h);
+  if (e->mw.empty()) return e->pixels(n);
+  return e->mcnt[k][n];
+}
+
+// values: [B][K][np] interleaved (re, im); second: [B][K][np] or null.
 int or_engine_forward(void* h, double* values, double* second) {
   return guarded([&] {
     Engine* e = static_cast<Engine*>(h);
-    std::vector<cplx> v(size_t(e->B) * e->plan.size());
+    std::vector<cplx> v(size_t(e->B) * e->nmask() * e->plan.size());
     e->forward(v.data(), second);
     for (size_t i = 0; i < v.size(); ++i) { values[2 * i] = v[i].real(); values[2 * i + 1] = v[i].imag(); }
   });
 }
 
-// cot: [B][np] interleaved; maps_grad: concatenation over levels of [B*C][N_d].
+// cot: [B][K][np] interleaved; maps_grad: concatenation over levels of [B*C][N_d].
 int or_engine_reverse(void* h, const double* cot, double* maps_grad) {
   return guarded([&] {
     Engine* e = static_cast<Engine*>(h);
-    std::vector<cplx> g(size_t(e->B) * e->plan.size());
+    std::vector<cplx> g(size_t(e->B) * e->nmask() * e->plan.size());
     for (size_t i = 0; i < g.size(); ++i) g[i] = cplx(cot[2 * i], cot[2 * i + 1]);
     auto mg = e->reverse(g.data());
     size_t at = 0;

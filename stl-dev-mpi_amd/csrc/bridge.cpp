@@ -2,6 +2,7 @@
 // (crates/stl-bridge/src/lib.rs:257-741) over the MI355X engine.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cctype>
 #include <cmath>
@@ -147,6 +148,10 @@ struct StlHandle {
   stl::DevArray<double> xbuf, gbuf;
   std::string precision;
   size_t ncoef() const { return engine->plan().size(); }
+  // Averages per map: one per mask, 1 without masks.
+  size_t nmask_rows() const { return size_t(std::max(engine->masks(), 1)); }
+  // Values / cotangents: [batch][K][ncoef].
+  size_t nvals() const { return size_t(batch) * nmask_rows() * ncoef(); }
   size_t input_len() const { return size_t(batch) * npix; }
   Identity identity(uint64_t g, uint64_t m, uint64_t o) const { return Identity{id, g, m, o}; }
 };
@@ -220,7 +225,7 @@ char* stlp_info(StlHandle* h) {
       << ", \"batch\": " << h->batch << ", \"channels\": 1, \"cross\": false, \"fewer\": false, \"backend\": \"gpu\""
       << ", \"precision\": \"" << h->precision << "\", \"dtype\": \""
       << (h->precision == "fp64" ? "float64/complex128" : "float32 fields and operators, float64 sums") << "\""
-      << ", \"masks\": 0, \"spin\": 0, \"conv\": \"folded gather, 4-fold face symmetry\", \"fields\": \"keep\"";
+      << ", \"masks\": " << h->engine->masks() << ", \"spin\": 0, \"conv\": \"folded gather, 4-fold face symmetry\", \"fields\": \"keep\"";
     o << ", \"device_memory\": {\"total_bytes\": " << h->engine->device_bytes()
       << ", \"peak_bytes\": " << stl::device_bytes_peak() << "}";
     o << ", \"setup_seconds\": " << h->engine->setup_seconds();
@@ -277,7 +282,7 @@ static int forward_impl(StlHandle* h, const double* x_dev, size_t n, uint64_t g,
   if (!out) fail("null output");
   if (moments && !var) fail("null variance output");
   h->current.reset();
-  std::vector<stl::cplx> values(size_t(h->batch) * h->ncoef());
+  std::vector<stl::cplx> values(h->nvals());
   std::vector<double> variance(moments ? values.size() : 0);
   h->engine->forward_device(x_dev, moments, values.data(), moments ? variance.data() : nullptr);
   write_values(values, out);
@@ -311,7 +316,7 @@ static std::vector<stl::cplx> read_cot(StlHandle* h, const double* cot, size_t n
   if (!h) fail("null handle");
   if (!h->current || !(*h->current == h->identity(g, m, o)))
     fail("stale, wrong-member or wrong-objective forward state");
-  if (n != size_t(h->batch) * h->ncoef()) fail("cotangent shape mismatch");
+  if (n != h->nvals()) fail("cotangent shape mismatch");
   if (!cot) fail("null cotangent");
   std::vector<stl::cplx> c(n);
   for (size_t i = 0; i < n; ++i) c[i] = stl::cplx(cot[2 * i], cot[2 * i + 1]);
@@ -342,9 +347,17 @@ int stlp_vjp_device(StlHandle* h, const double* cot, size_t n, uint64_t g, uint6
 int stlp_native_pixels(StlHandle* h, size_t* out) {
   return guard(-1, [&] {
     if (!h || !out) fail("null argument");
+    // [K][ncoef]: the pixels of mask k at the coefficient's level (lib.rs:456-472).
     const auto& plan = h->engine->plan();
-    for (size_t i = 0; i < plan.size(); ++i) out[i] = size_t(12) * (size_t(h->nside) >> plan[i].native_dg) *
-                                                     (size_t(h->nside) >> plan[i].native_dg);
+    const int K = h->engine->masks(), levels = h->engine->levels();
+    const auto counts = h->engine->mask_counts();
+    for (int k = 0; k < std::max(K, 1); ++k)
+      for (size_t i = 0; i < plan.size(); ++i) {
+        const int d = plan[i].native_dg;
+        out[size_t(k) * plan.size() + i] =
+            K > 0 ? size_t(counts[size_t(k) * levels + d])
+                  : size_t(12) * (size_t(h->nside) >> d) * (size_t(h->nside) >> d);
+      }
     return 0;
   });
 }
@@ -359,7 +372,7 @@ int stlp_forward_levels(StlHandle* h, const double* x, size_t n, double* out) {
     h->current.reset();
     stl::DevArray<double> buf(n);
     STL_HIP(hipMemcpy(buf.data(), x, n * sizeof(double), hipMemcpyHostToDevice));
-    std::vector<stl::cplx> values(size_t(h->batch) * h->ncoef());
+    std::vector<stl::cplx> values(h->nvals());
     h->engine->forward_levels_device(buf.data(), values.data());
     write_values(values, out);
     return 0;
@@ -394,10 +407,11 @@ int stlp_restore(StlHandle* h, const StlSnapshot* s, uint64_t g, uint64_t m, uin
 
 int stlp_set_masks(StlHandle* h, const double* weights, size_t k) {
   return guard(-1, [&] {
-    (void)weights;
     if (!h) fail("null handle");
     h->current.reset();
-    if (k != 0) fail("masks are not implemented by this library");
+    if (k > 0 && !weights) fail("null mask weights");
+    if (k > 64) fail("at most 64 masks");
+    h->engine->set_masks(weights, int(k));
     return 0;
   });
 }
@@ -451,7 +465,7 @@ int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* ta
     stl::LbfgsOptions opt = lbfgs_options(v);
     const bool ensemble = v.boolean("ensemble", false);
     h->current.reset();
-    const size_t nc = h->ncoef(), nv = size_t(h->batch) * nc;
+    const size_t nc = h->nmask_rows() * h->ncoef(), nv = size_t(h->batch) * nc;
     std::vector<stl::cplx> values(nv), cot(nv);
     size_t evals = 0;
     stl::VectorOps ops(int64_t(n), h->engine->stream());

@@ -19,10 +19,12 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <unordered_map>
 
 #include "geometry.hpp"
 #include "sweeps.hpp"
@@ -242,6 +244,70 @@ class EngineT final : public Engine {
   const EngineOptions& options() const override { return opt_; }
   const std::vector<Coefficient>& plan() const override { return plan_; }
   int levels() const override { return levels_; }
+  int masks() const override { return nmask_; }
+  std::vector<int64_t> mask_counts() const override { return mcount_; }
+
+  void set_masks(const double* w, int k) override {
+    STL_CHECK(k >= 0 && k <= 64, "at most 64 masks");
+    // Membership bits of every pixel at every level: level n + 1 holds the
+    // parents of level n (a parent is in a mask when one of its 4 NESTED
+    // children is).
+    std::vector<std::vector<uint64_t>> bits(levels_);
+    std::vector<int64_t> counts(size_t(k) * levels_, 0);
+    if (k > 0) {
+      STL_CHECK(w != nullptr, "null mask weights");
+      const int64_t N0 = N_[0];
+      bits[0].assign(size_t(N0), 0);
+      for (int q = 0; q < k; ++q)
+        for (int64_t p = 0; p < N0; ++p) {
+          const double v = w[size_t(q) * N0 + p];
+          STL_CHECK(std::isfinite(v), "mask weights must be finite");
+          if (v != 0.0) bits[0][p] |= uint64_t(1) << q;
+        }
+      for (int n = 1; n < levels_; ++n) {
+        bits[n].assign(size_t(N_[n]), 0);
+        const auto& f = bits[n - 1];
+        for (int64_t p = 0; p < N_[n]; ++p) bits[n][p] = f[4 * p] | f[4 * p + 1] | f[4 * p + 2] | f[4 * p + 3];
+      }
+      for (int n = 0; n < levels_; ++n)
+        for (uint64_t b : bits[n])
+          for (int q = 0; q < k; ++q) counts[size_t(q) * levels_ + n] += int64_t((b >> q) & 1);
+      for (int64_t c : counts) STL_CHECK(c > 0, "empty mask");
+    }
+    // Classes: the distinct membership patterns of each level, numbered in
+    // order of first appearance.
+    std::vector<std::vector<uint16_t>> cls(levels_);
+    std::vector<std::vector<uint64_t>> pat(levels_);
+    if (k > 0)
+      for (int n = 0; n < levels_; ++n) {
+        std::unordered_map<uint64_t, int> id;
+        cls[n].resize(bits[n].size());
+        for (size_t p = 0; p < bits[n].size(); ++p) {
+          auto it = id.find(bits[n][p]);
+          if (it == id.end()) {
+            STL_CHECK(pat[n].size() < 65535, "too many distinct mask overlaps");
+            it = id.emplace(bits[n][p], int(pat[n].size())).first;
+            pat[n].push_back(bits[n][p]);
+          }
+          cls[n][p] = uint16_t(it->second);
+        }
+      }
+    drop_graphs();
+    nmask_ = k;
+    Km_ = std::max(k, 1);
+    mcount_ = counts;
+    mcls_.clear();
+    mpat_.clear();
+    mC_.assign(levels_, 1);
+    mcls_.resize(levels_);
+    mpat_.resize(levels_);
+    for (int n = 0; n < levels_ && k > 0; ++n) {
+      upload(mcls_[n], cls[n]);
+      upload(mpat_[n], pat[n]);
+      mC_[n] = int(pat[n].size());
+    }
+    alloc_sums();
+  }
   hipStream_t stream() const override { return s_; }
   size_t device_bytes() const override { return device_bytes_live(); }
   double setup_seconds() const override { return setup_s_; }
@@ -261,7 +327,7 @@ class EngineT final : public Engine {
   }
 
   void stage_cotangent(const cplx* cot) override {
-    std::memcpy(hcot_.data(), cot, size_t(B_) * plan_.size() * sizeof(double2));
+    std::memcpy(hcot_.data(), cot, nvals() * sizeof(double2));
   }
 
   hipGraph_t capture_vjp(double* grad_dev) override {
@@ -288,13 +354,29 @@ class EngineT final : public Engine {
   }
 
   void sweep_reverse(double* grad_dev) {
-    const size_t nv = size_t(B_) * plan_.size();
+    const size_t nv = nvals();
     STL_HIP(hipMemcpyAsync(dcot_.data(), hcot_.data(), nv * sizeof(double2), hipMemcpyHostToDevice, s_));
-    launch_scatter_cot(nrt_, B_, int(plan_.size()), drt_.data(), drk_.data(), dcot_.data(), ddst_.data(), s_);
+    launch_scatter_cot(nrt_, B_, Km_, levels_, int(plan_.size()), drt_.data(), drk_.data(), dnd_.data(),
+                       dcot_.data(), ddst_.data(), s_);
+    if (nmask_ > 0) {
+      // Per-mask operands -> per-class ones (the sum over the masks holding the class).
+      for (int n = 0; n < levels_; ++n) {
+        const int K = gram_k(n), C = mC_[n];
+        const uint64_t* pat = mpat_[n].data();
+        launch_mask_combine(B_, Km_, kL, C, int64_t(K) * K * 2, pat, reinterpret_cast<const double*>(dH_[n].data()),
+                            reinterpret_cast<double*>(Hc_[n].data()), s_);
+        launch_mask_combine(B_, Km_, kL, C, 1, pat, ds1_[n].data(), s1c_[n].data(), s_);
+        launch_mask_combine(B_, Km_, kL, C, 1, pat, ds2_[n].data(), s2c_[n].data(), s_);
+      }
+      launch_mask_combine(B_, Km_, 1, mC_[0], 1, mpat_[0].data(), dgm_.data(), gmc_.data(), s_);
+      launch_mask_combine(B_, Km_, 1, mC_[0], 1, mpat_[0].data(), dgv_.data(), gvc_.data(), s_);
+    }
     for (int n = levels_ - 1; n >= 0; --n) {
       const int64_t N = N_[n];
+      const MaskView mv = mask_view(n);
+      const double2* H = nmask_ > 0 ? Hc_[n].data() : dH_[n].data();
       int id = prof_.start();
-      launch_gv<T>(n, N, B_, br_[n].data(), z_[n].data(), dH_[n].data(), gbr_.data(), gzp_.data(), s_);
+      launch_gv<T>(n, N, B_, br_[n].data(), z_[n].data(), H, gbr_.data(), gzp_.data(), mv, s_);
       prof_.stop(id, tag("gv", n), double((n + 2) * B_ * kL) * N * kL * 2 * sizeof(T) * 2);
 
       T* chain = acc_[n % 2].data();
@@ -303,7 +385,9 @@ class EngineT final : public Engine {
       if (n + 1 < levels_) dec_adj(n, kL, chain_prev, chain, (n + 1) * B_, chain);
 
       id = prof_.start();
-      launch_gz<T>(n, N, B_, z_[n].data(), gzp_.data(), chain, ds1_[n].data(), ds2_[n].data(), gz_.data(), s_);
+      const double* s1 = nmask_ > 0 ? s1c_[n].data() : ds1_[n].data();
+      const double* s2 = nmask_ > 0 ? s2c_[n].data() : ds2_[n].data();
+      launch_gz<T>(n, N, B_, z_[n].data(), gzp_.data(), chain, s1, s2, gz_.data(), mv, s_);
       prof_.stop(id, tag("gz", n), double(B_) * kL * N * (6 * sizeof(T)));
 
       T* mg = mg_[n % 2].data();
@@ -311,7 +395,8 @@ class EngineT final : public Engine {
       conv_adj(n, 1, gz_.data(), B_, mg);
       if (n + 1 < levels_) dec_adj(n, 1, mg_prev, mg, B_, mg);
     }
-    launch_meanvar_grad<T>(N_[0], B_, maps_[0].data(), dgm_.data(), dgv_.data(), mg_[0].data(), s_);
+    launch_meanvar_grad<T>(N_[0], B_, maps_[0].data(), nmask_ > 0 ? gmc_.data() : dgm_.data(),
+                           nmask_ > 0 ? gvc_.data() : dgv_.data(), mg_[0].data(), mask_view(0), s_);
     cast_out(mg_[0].data(), grad_dev, int64_t(B_) * N_[0]);
   }
 
@@ -383,6 +468,29 @@ class EngineT final : public Engine {
     z_.resize(levels_);
     M_.resize(levels_);
     br_.resize(levels_);
+    for (int n = 0; n < levels_; ++n) {
+      int64_t N = N_[n];
+      maps_[n].resize(size_t(B_) * N);
+      z_[n].resize(size_t(B_) * kL * N);
+      M_[n].resize(size_t(n + 1) * B_ * N * kL);
+      br_[n].resize(size_t(n + 1) * B_ * kL * N * kL);
+    }
+    gbr_.resize(br_[0].size());
+    for (int n = 1; n < levels_; ++n) gbr_.resize(std::max(gbr_.size(), br_[n].size()));
+    gzp_.resize(z_[0].size());
+    gz_.resize(z_[0].size());
+    acc_[0].resize(M_[0].size());
+    acc_[1].resize(std::max(M_[0].size(), levels_ > 1 ? M_[1].size() : size_t(0)));
+    mg_[0].resize(maps_[0].size());
+    mg_[1].resize(maps_[0].size());
+    mC_.assign(levels_, 1);
+    mcls_.resize(levels_);
+    mpat_.resize(levels_);
+    alloc_sums();
+  }
+
+  // Sums, reverse operands and statistics staging: [B][Km] rows of each.
+  void alloc_sums() {
     gpart_.resize(levels_);
     gpart2_.resize(levels_);
     gsum_.resize(levels_);
@@ -392,42 +500,67 @@ class EngineT final : public Engine {
     dH_.resize(levels_);
     ds1_.resize(levels_);
     ds2_.resize(levels_);
+    Hc_.resize(levels_);
+    s1c_.resize(levels_);
+    s2c_.resize(levels_);
+    nblk_gram_.clear();
+    nblk_z_.clear();
+    const size_t rows = size_t(B_) * Km_;
     for (int n = 0; n < levels_; ++n) {
       int64_t N = N_[n];
-      maps_[n].resize(size_t(B_) * N);
-      z_[n].resize(size_t(B_) * kL * N);
-      M_[n].resize(size_t(n + 1) * B_ * N * kL);
-      br_[n].resize(size_t(n + 1) * B_ * kL * N * kL);
       int K = gram_k(n), E = gram_e(n);
       int64_t tiles = (N + gram_tile(n) - 1) / gram_tile(n);
-      int nb = int(std::min<int64_t>(tiles, std::max<int64_t>(1, 2048 / (B_ * kL))));
+      int nb = int(std::min<int64_t>(tiles, std::max<int64_t>(1, 2048 / (int64_t(rows) * kL))));
       nblk_gram_.push_back(nb);
-      gpart_[n].resize(size_t(nb) * B_ * kL * E * 2);
-      gpart2_[n].resize(size_t(nb) * B_ * kL * E);
-      gsum_[n].resize(size_t(B_) * kL * E * 2);
-      gsum2_[n].resize(size_t(B_) * kL * E);
+      gpart_[n].resize(size_t(nb) * rows * kL * E * 2);
+      gpart2_[n].resize(size_t(nb) * rows * kL * E);
+      gsum_[n].resize(rows * kL * E * 2);
+      gsum2_[n].resize(rows * kL * E);
       int nz = int(std::min<int64_t>((N + 1023) / 1024, 1024));  // ~4 pixels per thread
       nblk_z_.push_back(nz);
-      zpart_[n].resize(size_t(nz) * B_ * kL * 4);
-      zsum_[n].resize(size_t(B_) * kL * 4);
-      dH_[n].resize(size_t(B_) * kL * K * K);
-      ds1_[n].resize(size_t(B_) * kL);
-      ds2_[n].resize(size_t(B_) * kL);
+      zpart_[n].resize(size_t(nz) * rows * kL * 4);
+      zsum_[n].resize(rows * kL * 4);
+      dH_[n].resize(rows * kL * K * K);
+      ds1_[n].resize(rows * kL);
+      ds2_[n].resize(rows * kL);
+      const size_t cls = nmask_ > 0 ? size_t(mC_[n]) : 0;
+      Hc_[n].resize(size_t(B_) * kL * cls * K * K);
+      s1c_[n].resize(size_t(B_) * kL * cls);
+      s2c_[n].resize(size_t(B_) * kL * cls);
     }
     nblk_mv_ = int(std::min<int64_t>((N_[0] + 1023) / 1024, 1024));
-    mvpart_.resize(size_t(nblk_mv_) * B_ * 4);
-    mvsum_.resize(size_t(B_) * 4);
-    gbr_.resize(br_[0].size());
-    for (int n = 1; n < levels_; ++n) gbr_.resize(std::max(gbr_.size(), br_[n].size()));
-    gzp_.resize(z_[0].size());
-    gz_.resize(z_[0].size());
-    acc_[0].resize(M_[0].size());
-    acc_[1].resize(std::max(M_[0].size(), levels_ > 1 ? M_[1].size() : size_t(0)));
-    mg_[0].resize(maps_[0].size());
-    mg_[1].resize(maps_[0].size());
-    dgm_.resize(B_);
-    dgv_.resize(B_);
+    mvpart_.resize(size_t(nblk_mv_) * rows * 4);
+    mvsum_.resize(rows * 4);
+    dgm_.resize(rows);
+    dgv_.resize(rows);
+    gmc_.resize(nmask_ > 0 ? size_t(B_) * mC_[0] : 0);
+    gvc_.resize(nmask_ > 0 ? size_t(B_) * mC_[0] : 0);
+    std::vector<double> nd(size_t(Km_) * levels_);
+    for (int k = 0; k < Km_; ++k)
+      for (int n = 0; n < levels_; ++n)
+        nd[size_t(k) * levels_ + n] = nmask_ > 0 ? double(mcount_[size_t(k) * levels_ + n]) : double(N_[n]);
+    upload(dnd_, nd);
     build_tables();
+  }
+
+  MaskView mask_view(int n) const {
+    MaskView v;
+    if (nmask_ > 0) {
+      v.cls = mcls_[n].data();
+      v.pat = mpat_[n].data();
+      v.K = Km_;
+      v.C = mC_[n];
+    }
+    return v;
+  }
+
+  size_t nvals() const { return size_t(B_) * Km_ * plan_.size(); }
+
+  void drop_graphs() {
+    for (auto* cache : {&fwd_graphs_, &vjp_graphs_}) {
+      for (auto& e : *cache) (void)hipGraphExecDestroy(e.exec);
+      cache->clear();
+    }
   }
 
   void cast_in(const double* x, T* out, int64_t n) {
@@ -495,22 +628,26 @@ class EngineT final : public Engine {
       prof_.stop(id, tag("conv_fwd_branch", n), conv_bytes(n, kL, (n + 1) * B_, false));
       id = prof_.start();
       int nb = nblk_gram_[n];
-      int64_t m = int64_t(B_) * kL * gram_e(n);
-      launch_gram<T>(n, N, B_, br_[n].data(), z_[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(), s_);
+      const MaskView mv = mask_view(n);
+      const int64_t rows = int64_t(B_) * Km_;
+      int64_t m = rows * kL * gram_e(n);
+      launch_gram<T>(n, N, B_, br_[n].data(), z_[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(), mv,
+                     s_);
       launch_reduce_parts(gpart_[n].data(), nb, m * 2, gsum_[n].data(), s_);
       if (moments) launch_reduce_parts(gpart2_[n].data(), nb, m, gsum2_[n].data(), s_);
-      prof_.stop(id, tag("gram", n), double(B_) * kL * N * (gram_k(n)) * 2 * sizeof(T));
+      prof_.stop(id, tag("gram", n), double(rows) * kL * N * (gram_k(n)) * 2 * sizeof(T));
       id = prof_.start();
-      launch_zstats<T>(N, B_, z_[n].data(), nblk_z_[n], zpart_[n].data(), s_);
-      launch_reduce_parts(zpart_[n].data(), nblk_z_[n], int64_t(B_) * kL * 4, zsum_[n].data(), s_);
+      launch_zstats<T>(N, B_, z_[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
+      launch_reduce_parts(zpart_[n].data(), nblk_z_[n], rows * kL * 4, zsum_[n].data(), s_);
       if (n == 0) {
-        launch_meanvar<T>(N, B_, maps_[0].data(), nblk_mv_, mvpart_.data(), s_);
-        launch_reduce_parts(mvpart_.data(), nblk_mv_, int64_t(B_) * 4, mvsum_.data(), s_);
+        launch_meanvar<T>(N, B_, maps_[0].data(), nblk_mv_, mvpart_.data(), mv, s_);
+        launch_reduce_parts(mvpart_.data(), nblk_mv_, rows * 4, mvsum_.data(), s_);
       }
-      prof_.stop(id, tag("zstats", n), double(B_) * kL * N * 2 * sizeof(T));
+      prof_.stop(id, tag("zstats", n), double(rows) * kL * N * 2 * sizeof(T));
     }
-    const size_t nv = size_t(B_) * plan_.size();
-    launch_assemble(int(plan_.size()), B_, dstat_.data(), dsrc_.data(), moments, dvalues_.data(), dvar_.data(), s_);
+    const size_t nv = nvals();
+    launch_assemble(int(plan_.size()), B_, Km_, levels_, dstat_.data(), dsrc_.data(), dnd_.data(), moments,
+                    dvalues_.data(), dvar_.data(), s_);
     STL_HIP(hipMemcpyAsync(hvalues_.data(), dvalues_.data(), nv * sizeof(double2), hipMemcpyDeviceToHost, s_));
     if (moments)
       STL_HIP(hipMemcpyAsync(hvar_.data(), dvar_.data(), nv * sizeof(double), hipMemcpyDeviceToHost, s_));
@@ -524,7 +661,7 @@ class EngineT final : public Engine {
 
   // After the forward's stream work has completed: statistics out of the pinned staging.
   void finish_forward(bool moments, cplx* values, double* variance) override {
-    const size_t nv = size_t(B_) * plan_.size();
+    const size_t nv = nvals();
     std::memcpy(values, hvalues_.data(), nv * sizeof(double2));
     if (moments && variance) std::memcpy(variance, hvar_.data(), nv * sizeof(double));
   }
@@ -694,27 +831,25 @@ class EngineT final : public Engine {
       e = StatEntry{};
       switch (sl.kind) {
         case SumSlot::MEANVAR: {
-          const double Nd = double(N_[0]);
           e.arr = 0; e.re = sl.idx; e.im = -1; e.bs = 4;
-          e.sq_arr = 0; e.sq = 2 + sl.idx; e.sbs = 4; e.Nd = Nd;
-          lists[sl.idx].push_back(RevTerm{i, 0, sl.idx == 0 ? 1.0 : 2.0, Nd});
+          e.sq_arr = 0; e.sq = 2 + sl.idx; e.sbs = 4; e.level = 0;
+          lists[sl.idx].push_back(RevTerm{i, 0, 0, 0, sl.idx == 0 ? 1.0 : 2.0});
           break;
         }
         case SumSlot::ZSTAT: {
-          const double Nd = double(N_[n]);
           e.arr = 3 + 3 * n; e.re = sl.l * 4 + sl.idx; e.im = -1; e.bs = kL * 4;
-          e.sq_arr = 3 + 3 * n; e.sq = sl.l * 4 + 2 + sl.idx; e.sbs = kL * 4; e.Nd = Nd;
-          lists[(sl.idx == 0 ? s1base[n] : s2base[n]) + sl.l].push_back(RevTerm{i, 0, sl.idx == 0 ? 1.0 : 2.0, Nd});
+          e.sq_arr = 3 + 3 * n; e.sq = sl.l * 4 + 2 + sl.idx; e.sbs = kL * 4; e.level = n;
+          lists[(sl.idx == 0 ? s1base[n] : s2base[n]) + sl.l].push_back(
+              RevTerm{i, 0, n, 0, sl.idx == 0 ? 1.0 : 2.0});
           break;
         }
         case SumSlot::GRAM: {
-          const double Nd = double(N_[n]);
           const int K = gram_k(n), E = gram_e(n);
           const int64_t el = int64_t(sl.l) * E + sl.idx;
           e.arr = 1 + 3 * n; e.re = 2 * el; e.im = 2 * el + 1; e.bs = int64_t(kL) * E * 2; e.conj = sl.conj;
-          e.sq_arr = 2 + 3 * n; e.sq = el; e.sbs = int64_t(kL) * E; e.Nd = Nd;
-          lists[hbase[n] + (sl.l * K + sl.gi) * K + sl.gj].push_back(RevTerm{i, 0, 1.0, Nd});
-          lists[hbase[n] + (sl.l * K + sl.gj) * K + sl.gi].push_back(RevTerm{i, 1, 1.0, Nd});
+          e.sq_arr = 2 + 3 * n; e.sq = el; e.sbs = int64_t(kL) * E; e.level = n;
+          lists[hbase[n] + (sl.l * K + sl.gi) * K + sl.gj].push_back(RevTerm{i, 0, n, 0, 1.0});
+          lists[hbase[n] + (sl.l * K + sl.gj) * K + sl.gi].push_back(RevTerm{i, 1, n, 0, 1.0});
           break;
         }
       }
@@ -731,7 +866,7 @@ class EngineT final : public Engine {
     upload(drt_, targets);
     upload(drk_, terms);
     upload(ddst_, dst);
-    const size_t nv = size_t(B_) * np;
+    const size_t nv = nvals();
     dvalues_.resize(nv);
     dvar_.resize(nv);
     dcot_.resize(nv);
@@ -774,6 +909,18 @@ class EngineT final : public Engine {
   std::vector<DevArray<double2>> dH_;
   std::vector<DevArray<double>> ds1_, ds2_;
   DevArray<double> dgm_, dgv_;
+  // masks: count, rows per map (max(masks, 1)), per-level pixel classes and
+  // their membership patterns, pixel counts [Km][levels] (host and device),
+  // and the per-class reverse operands
+  int nmask_ = 0, Km_ = 1;
+  std::vector<int> mC_;
+  std::vector<DevArray<uint16_t>> mcls_;
+  std::vector<DevArray<uint64_t>> mpat_;
+  std::vector<int64_t> mcount_;
+  DevArray<double> dnd_;
+  std::vector<DevArray<double2>> Hc_;
+  std::vector<DevArray<double>> s1c_, s2c_;
+  DevArray<double> gmc_, gvc_;
   // device statistics assembly and cotangent scatter
   int nrt_ = 0;
   DevArray<StatEntry> dstat_;

@@ -64,8 +64,17 @@ class Engine {
   virtual const EngineOptions& options() const = 0;
   virtual const std::vector<Coefficient>& plan() const = 0;
   virtual int levels() const = 0;
+  // Averages over k masks from the next forward on (the reference's
+  // set_masks, crates/stl-core/src/engine.rs:392-404 with Masks::from_weights,
+  // mask.rs:43-70): w [k][12 nside^2] host indicators (nonzero: in), level
+  // n + 1 holding the parents of level n; k = 0: the whole sky.  Values,
+  // variances and cotangents become [B][k][ncoef].  At most 64 masks.
+  virtual void set_masks(const double* w, int k) = 0;
+  // Masks in use (0: the whole sky) and their pixel counts [k][levels].
+  virtual int masks() const = 0;
+  virtual std::vector<int64_t> mask_counts() const = 0;
   // Forward on maps [B][12 nside^2] in device memory (float64); values and
-  // variances ([B][ncoef]) to host memory.
+  // variances ([B][K][ncoef], K = max(masks, 1)) to host memory.
   virtual void forward_device(const double* x_dev, bool moments, cplx* values, double* variance) = 0;
   // Forward on maps given at every level (concatenated, device memory), no decimation of level 0.
   virtual void forward_levels_device(const double* x_dev, cplx* values) = 0;

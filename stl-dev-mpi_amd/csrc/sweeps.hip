@@ -1023,21 +1023,26 @@ __device__ inline double2 to_d2(cplx_t<T> v) {
   return make_double2(double(v.x), double(v.y));
 }
 
+__device__ inline bool in_mask(const MaskView& m, int64_t p, int k) {
+  return m.cls == nullptr || ((m.pat[m.cls[p]] >> k) & 1ull) != 0;
+}
+__device__ inline int mask_class(const MaskView& m, int64_t p) { return m.cls == nullptr ? 0 : int(m.cls[p]); }
+
 // Gram partial sums: block (tile stride, b * kL + l3).
 // Gram products at the levels of small K (all E = K (K + 1) / 2 entries of a
 // pixel fit in registers): each thread reads its pixels' V straight from HBM
 // (40 bytes per pixel at K = 5, coalesced) and accumulates every entry in
 // FP64; the workgroup then sums its threads by a fixed butterfly within each
 // wave and the 4 wave sums in order (deterministic).
-template <typename T, int KT, bool MOM>
+template <typename T, int KT, bool MOM, bool MASKED>
 __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
                                                   const cplx_t<T>* __restrict__ z, double* __restrict__ part,
-                                                  double* __restrict__ part2) {
+                                                  double* __restrict__ part2, MaskView mask) {
   constexpr int E = KT * (KT + 1) / 2;
   constexpr int ND = (KT - 1) / kL;  // branch rows d = 0..n
   __shared__ double red[4][MOM ? 3 : 2][E];
-  const int bl = blockIdx.y;
-  const int b = bl / kL, l3 = bl % kL;
+  const int bl = blockIdx.y;  // (b K + k) kL + l3
+  const int l3 = bl % kL, mk = (bl / kL) % mask.K, b = bl / (kL * mask.K);
   const int tid = threadIdx.x;
   double are[E], aim[E], asq[MOM ? E : 1];
 #pragma unroll
@@ -1047,6 +1052,7 @@ __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const
   }
   const cplx_t<T>* zr = z + (size_t(b) * kL + l3) * N;
   for (int64_t p = blockIdx.x * int64_t(256) + tid; p < N; p += int64_t(gridDim.x) * 256) {
+    if (MASKED && !in_mask(mask, p, mk)) continue;
     double2 v[KT];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
@@ -1106,17 +1112,17 @@ __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const
 // every LDS read feeds 4 products.  The values widen exactly to FP64; the
 // products and their sums are FP64.  The per-group sums are added in group
 // order: deterministic.
-template <typename T, bool MOM, int BS>
+template <typename T, bool MOM, int BS, bool MASKED>
 __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
                                               const cplx_t<T>* __restrict__ z, int TPX, double* __restrict__ part,
-                                              double* __restrict__ part2) {
+                                              double* __restrict__ part2, MaskView mask) {
   extern __shared__ double2 smem[];
   const int K = (n + 1) * kL + 1;
   const int KP = (K + BS - 1) / BS * BS, NR = KP / BS, NBLK = NR * (NR + 1) / 2;
   const int E = K * (K + 1) / 2;
   double2* Vs = smem;
-  const int bl = blockIdx.y;
-  const int b = bl / kL, l3 = bl % kL;
+  const int bl = blockIdx.y;  // (b K + k) kL + l3
+  const int l3 = bl % kL, mk = (bl / kL) % mask.K, b = bl / (kL * mask.K);
   const int tid = threadIdx.x;
   const int G = 256 / NBLK;
   const int blk = tid % NBLK, grp = tid / NBLK;
@@ -1137,11 +1143,14 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
       const cplx_t<T>* src = br + ((size_t(d) * B + b) * kL + l3) * N * kL + size_t(p0) * kL;
       for (int idx = tid; idx < tn * kL; idx += blockDim.x) {
         int p = idx / kL, l1 = idx % kL;
-        Vs[p * KP + d * kL + l1] = to_d2<T>(src[idx]);
+        // Pixels outside the mask stage zeros: their products add nothing.
+        Vs[p * KP + d * kL + l1] =
+            !MASKED || in_mask(mask, p0 + p, mk) ? to_d2<T>(src[idx]) : make_double2(0.0, 0.0);
       }
     }
     const cplx_t<T>* zs = z + (size_t(b) * kL + l3) * N + p0;
-    for (int p = tid; p < tn; p += blockDim.x) Vs[p * KP + K - 1] = to_d2<T>(zs[p]);
+    for (int p = tid; p < tn; p += blockDim.x)
+      Vs[p * KP + K - 1] = !MASKED || in_mask(mask, p0 + p, mk) ? to_d2<T>(zs[p]) : make_double2(0.0, 0.0);
     const int pad = KP - K;
     for (int idx = tid; idx < tn * pad; idx += blockDim.x) Vs[(idx / pad) * KP + K + idx % pad] = make_double2(0.0, 0.0);
     __syncthreads();
@@ -1220,13 +1229,16 @@ __device__ inline void block_sum4(double v[4], double* sh) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_zstats(int64_t N, const cplx_t<T>* __restrict__ z, double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_zstats(int64_t N, const cplx_t<T>* __restrict__ z, double* __restrict__ part,
+                                                MaskView mask) {
   __shared__ double sh[4 * 256];
-  int bl = blockIdx.y;
-  const cplx_t<T>* zs = z + size_t(bl) * N;
+  const int bl = blockIdx.y;  // (b K + k) kL + l
+  const int l = bl % kL, mk = (bl / kL) % mask.K, b = bl / (kL * mask.K);
+  const cplx_t<T>* zs = z + (size_t(b) * kL + l) * N;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < N; p += int64_t(gridDim.x) * blockDim.x) {
+    if (!in_mask(mask, p, mk)) continue;
     double re = double(zs[p].x), im = double(zs[p].y);
     double h = hypot(re, im);
     v[0] += h;
@@ -1240,13 +1252,16 @@ __global__ __launch_bounds__(256) void k_zstats(int64_t N, const cplx_t<T>* __re
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_meanvar(int64_t N, const T* __restrict__ x, double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_meanvar(int64_t N, const T* __restrict__ x, double* __restrict__ part,
+                                                 MaskView mask) {
   __shared__ double sh[4 * 256];
-  int b = blockIdx.y;
+  const int bl = blockIdx.y;  // b K + k
+  const int mk = bl % mask.K, b = bl / mask.K;
   const T* xs = x + size_t(b) * N;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < N; p += int64_t(gridDim.x) * blockDim.x) {
+    if (!in_mask(mask, p, mk)) continue;
     double a = double(xs[p]);
     double sq = fabs(a) * fabs(a);
     v[0] += a;
@@ -1256,7 +1271,7 @@ __global__ __launch_bounds__(256) void k_meanvar(int64_t N, const T* __restrict_
   }
   block_sum4(v, sh);
   if (threadIdx.x == 0)
-    for (int c = 0; c < 4; ++c) part[(size_t(blockIdx.x) * gridDim.y + b) * 4 + c] = v[c];
+    for (int c = 0; c < 4; ++c) part[(size_t(blockIdx.x) * gridDim.y + bl) * 4 + c] = v[c];
 }
 
 // part [nblk][m] -> out [m]: 64 outputs per workgroup, wave w adding the
@@ -1282,62 +1297,63 @@ __global__ __launch_bounds__(64 * kRedWaves) void k_reduce_parts(const double* _
   }
 }
 
-__global__ void k_assemble(int np, int B, const StatEntry* __restrict__ entries, const double* const* src,
-                           bool moments, double2* __restrict__ values, double* __restrict__ variance) {
+__global__ void k_assemble(int np, int rows, int K, int levels, const StatEntry* __restrict__ entries,
+                           const double* const* src, const double* __restrict__ nd, bool moments,
+                           double2* __restrict__ values, double* __restrict__ variance) {
   int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (t >= int64_t(np) * B) return;
-  const int b = int(t / np), i = int(t % np);
+  if (t >= int64_t(np) * rows) return;
+  const int r = int(t / np), i = int(t % np);  // r = b K + k
   const StatEntry e = entries[i];
+  const double Nd = nd[(r % K) * levels + e.level];
   const double* a = src[e.arr];
-  double re = a[e.re + b * e.bs] / e.Nd;
-  double im = e.im < 0 ? 0.0 : a[e.im + b * e.bs] / e.Nd;
+  double re = a[e.re + r * e.bs] / Nd;
+  double im = e.im < 0 ? 0.0 : a[e.im + r * e.bs] / Nd;
   if (e.conj) im = -im;
   values[t] = make_double2(re, im);
   if (moments && variance != nullptr) {
-    double second = e.sq_arr < 0 ? 0.0 : src[e.sq_arr][e.sq + b * e.sbs] / e.Nd;
+    double second = e.sq_arr < 0 ? 0.0 : src[e.sq_arr][e.sq + r * e.sbs] / Nd;
     variance[t] = fmax(second - (re * re + im * im), 0.0);
   }
 }
 
-__global__ void k_scatter_cot(int ntargets, int B, int np, const RevTarget* __restrict__ targets,
-                              const RevTerm* __restrict__ terms, const double2* __restrict__ cot,
-                              double* const* dst) {
+__global__ void k_scatter_cot(int ntargets, int rows, int K, int levels, int np, const RevTarget* __restrict__ targets,
+                              const RevTerm* __restrict__ terms, const double* __restrict__ nd,
+                              const double2* __restrict__ cot, double* const* dst) {
   int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (t >= int64_t(ntargets) * B) return;
-  const int b = int(t / ntargets);
+  if (t >= int64_t(ntargets) * rows) return;
+  const int row = int(t / ntargets);  // b K + k
   const RevTarget r = targets[t % ntargets];
+  const double* ndk = nd + (row % K) * levels;
   double re = 0.0, im = 0.0;
-  for (int k = r.first; k < r.first + r.count; ++k) {
-    const RevTerm m = terms[k];
-    double2 g = cot[size_t(b) * np + m.coef];
-    re += (m.mult * g.x) / m.Nd;
-    im += (m.mult * (m.conj ? -g.y : g.y)) / m.Nd;
+  for (int q = r.first; q < r.first + r.count; ++q) {
+    const RevTerm m = terms[q];
+    const double Nd = ndk[m.level];
+    double2 g = cot[size_t(row) * np + m.coef];
+    re += (m.mult * g.x) / Nd;
+    im += (m.mult * (m.conj ? -g.y : g.y)) / Nd;
   }
-  double* d = dst[r.arr] + r.off + b * r.bs;
+  double* d = dst[r.arr] + r.off + row * r.bs;
   d[0] = re;
   if (r.cplx) d[1] = im;
 }
 
-template <typename T, int NL>
-__global__ __launch_bounds__(256) void k_gv(int64_t N, int B, const cplx_t<T>* __restrict__ br,
-                                            const cplx_t<T>* __restrict__ z, const double2* __restrict__ H,
-                                            cplx_t<T>* __restrict__ gbr, cplx_t<T>* __restrict__ gzp) {
-  constexpr int K = NL * kL + 1;
-  __shared__ double2 Hs[K * K];
-  int bl = blockIdx.y;
-  int b = bl / kL, l3 = bl % kL;
-  for (int e = threadIdx.x; e < K * K; e += blockDim.x) Hs[e] = H[size_t(bl) * K * K + e];
-  __syncthreads();
-  int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (p >= N) return;
-  double2 V[K];
-#pragma unroll
-  for (int d = 0; d < NL; ++d) {
-    const cplx_t<T>* src = br + (((size_t(d) * B + b) * kL + l3) * N + p) * kL;
-#pragma unroll
-    for (int l1 = 0; l1 < kL; ++l1) V[d * kL + l1] = to_d2<T>(src[l1]);
-  }
-  V[K - 1] = to_d2<T>(z[(size_t(b) * kL + l3) * N + p]);
+__global__ void k_mask_combine(int B, int K, int R, int C, int64_t E, const uint64_t* __restrict__ pat,
+                               const double* __restrict__ in, double* __restrict__ out) {
+  int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(B) * R * C * E) return;
+  const int64_t e = t % E;
+  const int c = int((t / E) % C), r = int((t / (E * C)) % R), b = int(t / (E * C * R));
+  const uint64_t bits = pat[c];
+  double acc = 0.0;
+  for (int k = 0; k < K; ++k)
+    if ((bits >> k) & 1ull) acc += in[((size_t(b) * K + k) * R + r) * E + e];
+  out[t] = acc;
+}
+
+// gV = H V of one pixel: writes the branch cotangents and the z part.
+template <typename T, int K>
+__device__ __forceinline__ void gv_apply(const double2* Hs, const double2 (&V)[K], int64_t N, int B, int b, int l3,
+                                         int64_t p, cplx_t<T>* __restrict__ gbr, cplx_t<T>* __restrict__ gzp) {
 #pragma unroll 1
   for (int i = 0; i < K; ++i) {
     double re = 0.0, im = 0.0;
@@ -1356,11 +1372,42 @@ __global__ __launch_bounds__(256) void k_gv(int64_t N, int B, const cplx_t<T>* _
   }
 }
 
+// H of the bl row's C mask classes staged in LDS when LDS (C K^2 double2 of
+// dynamic shared memory), else read through the caches; the pixel uses the
+// matrix of its class.
+template <typename T, int NL, bool LDS>
+__global__ __launch_bounds__(256) void k_gv(int64_t N, int B, const cplx_t<T>* __restrict__ br,
+                                            const cplx_t<T>* __restrict__ z, const double2* __restrict__ H,
+                                            cplx_t<T>* __restrict__ gbr, cplx_t<T>* __restrict__ gzp, MaskView mask) {
+  constexpr int K = NL * kL + 1;
+  extern __shared__ double2 Hsm[];
+  int bl = blockIdx.y;
+  int b = bl / kL, l3 = bl % kL;
+  const double2* Hrow = H + size_t(bl) * mask.C * K * K;
+  if (LDS) {
+    for (int e = threadIdx.x; e < mask.C * K * K; e += blockDim.x) Hsm[e] = Hrow[e];
+    __syncthreads();
+  }
+  int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (p >= N) return;
+  const size_t hc = size_t(mask_class(mask, p)) * K * K;
+  double2 V[K];
+#pragma unroll
+  for (int d = 0; d < NL; ++d) {
+    const cplx_t<T>* src = br + (((size_t(d) * B + b) * kL + l3) * N + p) * kL;
+#pragma unroll
+    for (int l1 = 0; l1 < kL; ++l1) V[d * kL + l1] = to_d2<T>(src[l1]);
+  }
+  V[K - 1] = to_d2<T>(z[(size_t(b) * kL + l3) * N + p]);
+  if (LDS) gv_apply<T, K>(Hsm + hc, V, N, B, b, l3, p, gbr, gzp);
+  else gv_apply<T, K>(Hrow + hc, V, N, B, b, l3, p, gbr, gzp);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_gz(int n, int64_t N, int B, const cplx_t<T>* __restrict__ z,
                                             const cplx_t<T>* __restrict__ gzp, const T* __restrict__ chain,
                                             const double* __restrict__ s1, const double* __restrict__ s2,
-                                            cplx_t<T>* __restrict__ gz) {
+                                            cplx_t<T>* __restrict__ gz, MaskView mask) {
   int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (p >= N) return;
   int bl = blockIdx.y;
@@ -1370,7 +1417,8 @@ __global__ __launch_bounds__(256) void k_gz(int n, int64_t N, int B, const cplx_
   double mag = hypot(zr, zi);
   double den = mag == 0.0 ? 1.0 : mag;
   double h = double(chain[((size_t(n) * B + b) * N + p) * kL + l]);
-  double a = s1[bl], c = s2[bl];
+  const size_t sc = size_t(bl) * mask.C + mask_class(mask, p);
+  double a = s1[sc], c = s2[sc];
   double re = double(gzp[at].x) + (a * zr) / den + c * zr + (h * zr) / den;
   double im = double(gzp[at].y) + (a * zi) / den + c * zi + (h * zi) / den;
   gz[at] = make_c<T>(T(re), T(im));
@@ -1378,12 +1426,13 @@ __global__ __launch_bounds__(256) void k_gz(int n, int64_t N, int B, const cplx_
 
 template <typename T>
 __global__ void k_meanvar_grad(int64_t N, const T* __restrict__ x, const double* __restrict__ gmean,
-                               const double* __restrict__ gvar, T* __restrict__ g0) {
+                               const double* __restrict__ gvar, T* __restrict__ g0, MaskView mask) {
   int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (p >= N) return;
   int b = blockIdx.y;
   size_t at = size_t(b) * N + p;
-  g0[at] = T(double(g0[at]) + gmean[b] + gvar[b] * double(x[at]));
+  const size_t sc = size_t(b) * mask.C + mask_class(mask, p);
+  g0[at] = T(double(g0[at]) + gmean[sc] + gvar[sc] * double(x[at]));
 }
 
 template <typename TO, typename TI>
@@ -1573,20 +1622,21 @@ int gram_tile(int n) {
 
 template <typename T>
 void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, bool moments, int nblk,
-                 double* part, double* part2, hipStream_t s) {
+                 double* part, double* part2, const MaskView& mask, hipStream_t s) {
   const int K = (n + 1) * kL + 1;
-  dim3 grid(nblk, B * kL);
-  if (K == 5) {
-    if (moments) k_gram_reg<T, 5, true><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2);
-    else k_gram_reg<T, 5, false><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2);
-    STL_HIP(hipGetLastError());
-    return;
+  dim3 grid(nblk, B * mask.K * kL);
+  const bool masked = mask.cls != nullptr;
+#define STL_GRAM_REG(KT, MOM)                                                                      \
+  {                                                                                                \
+    if (masked) k_gram_reg<T, KT, MOM, true><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2, mask);  \
+    else k_gram_reg<T, KT, MOM, false><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2, mask);        \
+    STL_HIP(hipGetLastError());                                                                    \
+    return;                                                                                        \
   }
-  if (K == 9 && !moments) {
-    k_gram_reg<T, 9, false><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2);
-    STL_HIP(hipGetLastError());
-    return;
-  }
+  if (K == 5 && moments) STL_GRAM_REG(5, true)
+  if (K == 5) STL_GRAM_REG(5, false)
+  if (K == 9 && !moments) STL_GRAM_REG(9, false)
+#undef STL_GRAM_REG
   // Small K: 2 x 2 blocks waste fewer padded products; larger K: 4 x 4 blocks
   // halve the LDS reads per product.
   const int BS = K <= 13 ? 2 : 4;
@@ -1595,37 +1645,51 @@ void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<
   const int tpx = gram_tile(n);
   size_t lds = std::max(size_t(tpx) * KP * sizeof(double2),
                         (256 + 128) * sizeof(double2) + size_t(NBLK) * BS * BS * (sizeof(double2) + sizeof(double)));
-  if (BS == 2) {
-    if (moments) k_gram<T, true, 2><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
-    else k_gram<T, false, 2><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
-  } else {
-    if (moments) k_gram<T, true, 4><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
-    else k_gram<T, false, 4><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2);
+#define STL_GRAM(MOM, BSZ)                                                                          \
+  {                                                                                                 \
+    if (masked) k_gram<T, MOM, BSZ, true><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2, mask);  \
+    else k_gram<T, MOM, BSZ, false><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2, mask);       \
   }
+  if (BS == 2) {
+    if (moments) STL_GRAM(true, 2) else STL_GRAM(false, 2)
+  } else {
+    if (moments) STL_GRAM(true, 4) else STL_GRAM(false, 4)
+  }
+#undef STL_GRAM
   STL_HIP(hipGetLastError());
 }
 
 template <typename T>
-void launch_zstats(int64_t N, int B, const cplx_t<T>* z, int nblk, double* part, hipStream_t s) {
-  k_zstats<T><<<dim3(nblk, B * kL), 256, 0, s>>>(N, z, part);
+void launch_zstats(int64_t N, int B, const cplx_t<T>* z, int nblk, double* part, const MaskView& mask,
+                   hipStream_t s) {
+  k_zstats<T><<<dim3(nblk, B * mask.K * kL), 256, 0, s>>>(N, z, part, mask);
   STL_HIP(hipGetLastError());
 }
 
 template <typename T>
-void launch_meanvar(int64_t N, int B, const T* x, int nblk, double* part, hipStream_t s) {
-  k_meanvar<T><<<dim3(nblk, B), 256, 0, s>>>(N, x, part);
+void launch_meanvar(int64_t N, int B, const T* x, int nblk, double* part, const MaskView& mask, hipStream_t s) {
+  k_meanvar<T><<<dim3(nblk, B * mask.K), 256, 0, s>>>(N, x, part, mask);
   STL_HIP(hipGetLastError());
 }
 
-void launch_assemble(int np, int B, const StatEntry* entries, const double* const* src, bool moments,
-                     double2* values, double* variance, hipStream_t s) {
-  k_assemble<<<nblocks(int64_t(np) * B, 256), 256, 0, s>>>(np, B, entries, src, moments, values, variance);
+void launch_assemble(int np, int B, int K, int levels, const StatEntry* entries, const double* const* src,
+                     const double* nd, bool moments, double2* values, double* variance, hipStream_t s) {
+  k_assemble<<<nblocks(int64_t(np) * B * K, 256), 256, 0, s>>>(np, B * K, K, levels, entries, src, nd, moments,
+                                                                values, variance);
   STL_HIP(hipGetLastError());
 }
 
-void launch_scatter_cot(int ntargets, int B, int np, const RevTarget* targets, const RevTerm* terms,
-                        const double2* cot, double* const* dst, hipStream_t s) {
-  k_scatter_cot<<<nblocks(int64_t(ntargets) * B, 256), 256, 0, s>>>(ntargets, B, np, targets, terms, cot, dst);
+void launch_scatter_cot(int ntargets, int B, int K, int levels, int np, const RevTarget* targets,
+                        const RevTerm* terms, const double* nd, const double2* cot, double* const* dst,
+                        hipStream_t s) {
+  k_scatter_cot<<<nblocks(int64_t(ntargets) * B * K, 256), 256, 0, s>>>(ntargets, B * K, K, levels, np, targets,
+                                                                         terms, nd, cot, dst);
+  STL_HIP(hipGetLastError());
+}
+
+void launch_mask_combine(int B, int K, int R, int C, int64_t E, const uint64_t* pat, const double* in, double* out,
+                         hipStream_t s) {
+  k_mask_combine<<<nblocks(int64_t(B) * R * C * E, 256), 256, 0, s>>>(B, K, R, C, E, pat, in, out);
   STL_HIP(hipGetLastError());
 }
 
@@ -1636,11 +1700,17 @@ void launch_reduce_parts(const double* part, int nblk, int64_t m, double* out, h
 
 template <typename T>
 void launch_gv(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, const double2* H,
-               cplx_t<T>* gbranch, cplx_t<T>* gz_part, hipStream_t s) {
+               cplx_t<T>* gbranch, cplx_t<T>* gz_part, const MaskView& mask, hipStream_t s) {
   dim3 grid(nblocks(N, 256), B * kL);
+  const int K = (n + 1) * kL + 1;
+  const size_t hbytes = size_t(mask.C) * K * K * sizeof(double2);
+  const bool lds = hbytes <= 64 * 1024;
   switch (n + 1) {
-#define STL_GV(NL) \
-  case NL: k_gv<T, NL><<<grid, 256, 0, s>>>(N, B, branch, z, H, gbranch, gz_part); break;
+#define STL_GV(NL)                                                                                  \
+  case NL:                                                                                          \
+    if (lds) k_gv<T, NL, true><<<grid, 256, hbytes, s>>>(N, B, branch, z, H, gbranch, gz_part, mask); \
+    else k_gv<T, NL, false><<<grid, 256, 0, s>>>(N, B, branch, z, H, gbranch, gz_part, mask);         \
+    break;
     STL_GV(1) STL_GV(2) STL_GV(3) STL_GV(4) STL_GV(5) STL_GV(6) STL_GV(7) STL_GV(8) STL_GV(9) STL_GV(10)
     STL_GV(11) STL_GV(12)
 #undef STL_GV
@@ -1651,14 +1721,15 @@ void launch_gv(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>
 
 template <typename T>
 void launch_gz(int n, int64_t N, int B, const cplx_t<T>* z, const cplx_t<T>* gz_part, const T* chain,
-               const double* s1, const double* s2, cplx_t<T>* gz, hipStream_t s) {
-  k_gz<T><<<dim3(nblocks(N, 256), B * kL), 256, 0, s>>>(n, N, B, z, gz_part, chain, s1, s2, gz);
+               const double* s1, const double* s2, cplx_t<T>* gz, const MaskView& mask, hipStream_t s) {
+  k_gz<T><<<dim3(nblocks(N, 256), B * kL), 256, 0, s>>>(n, N, B, z, gz_part, chain, s1, s2, gz, mask);
   STL_HIP(hipGetLastError());
 }
 
 template <typename T>
-void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, T* g0, hipStream_t s) {
-  k_meanvar_grad<T><<<dim3(nblocks(N, 256), B), 256, 0, s>>>(N, x, gmean, gvar, g0);
+void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, T* g0,
+                         const MaskView& mask, hipStream_t s) {
+  k_meanvar_grad<T><<<dim3(nblocks(N, 256), B), 256, 0, s>>>(N, x, gmean, gvar, g0, mask);
   STL_HIP(hipGetLastError());
 }
 
@@ -1675,14 +1746,15 @@ void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s) {
   template void launch_dec_fwd<T>(const DecOp<T>&, int, const T*, int, T*, hipStream_t);                      \
   template void launch_dec_adj<T>(const DecOp<T>&, int, const T*, const T*, int, T*, hipStream_t);            \
   template void launch_gram<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, bool, int, double*,     \
-                               double*, hipStream_t);                                                         \
-  template void launch_zstats<T>(int64_t, int, const cplx_t<T>*, int, double*, hipStream_t);                  \
-  template void launch_meanvar<T>(int64_t, int, const T*, int, double*, hipStream_t);                         \
+                               double*, const MaskView&, hipStream_t);                                        \
+  template void launch_zstats<T>(int64_t, int, const cplx_t<T>*, int, double*, const MaskView&, hipStream_t); \
+  template void launch_meanvar<T>(int64_t, int, const T*, int, double*, const MaskView&, hipStream_t);        \
   template void launch_gv<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, const double2*,           \
-                             cplx_t<T>*, cplx_t<T>*, hipStream_t);                                            \
+                             cplx_t<T>*, cplx_t<T>*, const MaskView&, hipStream_t);                           \
   template void launch_gz<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, const T*, const double*,  \
-                             const double*, cplx_t<T>*, hipStream_t);                                         \
-  template void launch_meanvar_grad<T>(int64_t, int, const T*, const double*, const double*, T*, hipStream_t);
+                             const double*, cplx_t<T>*, const MaskView&, hipStream_t);                        \
+  template void launch_meanvar_grad<T>(int64_t, int, const T*, const double*, const double*, T*,              \
+                                       const MaskView&, hipStream_t);
 
 STL_INSTANTIATE(double)
 STL_INSTANTIATE(float)

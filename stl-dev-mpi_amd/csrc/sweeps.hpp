@@ -50,66 +50,87 @@ void launch_dec_fwd(const DecOp<T>& op, int NI, const T* x, int rows, T* y, hipS
 template <typename T>
 void launch_dec_adj(const DecOp<T>& op, int NI, const T* gy, const T* add, int rows, T* gx, hipStream_t s);
 
+// Masks of one level (the reference's Masks, crates/stl-core/src/mask.rs):
+// pixel p belongs to mask k when bit k of pat[cls[p]] is set, cls holding
+// the class (distinct membership pattern) of every pixel.  cls == nullptr:
+// no masks, every pixel in the one average (K = C = 1).  The reductions
+// below run over (b, k) rows; the reverse reads per-class operands.
+struct MaskView {
+  const uint16_t* cls = nullptr;
+  const uint64_t* pat = nullptr;
+  int K = 1;  // masks
+  int C = 1;  // classes
+};
+
 // Pixels per LDS tile of the Gram kernel at level n.
 int gram_tile(int n);
 
 // Per-block partial sums of the Gram products V_i conj(V_j), i <= j, of
-// V = [branch(d)[l1][l3] for d <= n, l1 < kL; z[l3]] per (b, l3):
-// part [nblk][B * kL][E][2] (+ [E] second moments |.|^2 when moments).
+// V = [branch(d)[l1][l3] for d <= n, l1 < kL; z[l3]] per (b, mask k, l3)
+// over the pixels of the mask: part [nblk][B * K * kL][E][2] (+ [E] second
+// moments |.|^2 when moments).
 template <typename T>
 void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, bool moments, int nblk,
-                 double* part, double* part2, hipStream_t s);
-// S1/S2 sums of z: part [nblk][B * kL][4] = (sum |z|, sum |z|^2, sum re^2+im^2, sum |z|^4).
+                 double* part, double* part2, const MaskView& mask, hipStream_t s);
+// S1/S2 sums of z: part [nblk][B * K * kL][4] = (sum |z|, sum |z|^2, sum re^2+im^2, sum |z|^4).
 template <typename T>
-void launch_zstats(int64_t N, int B, const cplx_t<T>* z, int nblk, double* part, hipStream_t s);
-// mean/var sums of the level-0 map: part [nblk][B][4] = (sum x, sum x^2, sum x^2, sum x^4).
+void launch_zstats(int64_t N, int B, const cplx_t<T>* z, int nblk, double* part, const MaskView& mask,
+                   hipStream_t s);
+// mean/var sums of the level-0 map: part [nblk][B * K][4] = (sum x, sum x^2, sum x^2, sum x^4).
 template <typename T>
-void launch_meanvar(int64_t N, int B, const T* x, int nblk, double* part, hipStream_t s);
+void launch_meanvar(int64_t N, int B, const T* x, int nblk, double* part, const MaskView& mask, hipStream_t s);
 // Sums the partials over blocks in block order: part [nblk][m] -> out [m].
 void launch_reduce_parts(const double* part, int nblk, int64_t m, double* out, hipStream_t s);
 
-// Statistics from the sky-wide sums, on the device: value i of map b is
-// (src[arr][re + b bs] + i src[arr][im + b bs]) / Nd (imaginary part 0 when
-// im < 0, negated when conj), and with moments the variance of its terms
-// max(src[sq_arr][sq + b sbs] / Nd - |value|^2, 0).
+// Statistics from the sky-wide sums, on the device: value i of row r =
+// b K + k (map b, mask k) is (src[arr][re + r bs] + i src[arr][im + r bs]) / Nd
+// (imaginary part 0 when im < 0, negated when conj), Nd = nd[k levels + level]
+// the pixels averaged, and with moments the variance of its terms
+// max(src[sq_arr][sq + r sbs] / Nd - |value|^2, 0).
 struct StatEntry {
-  int32_t arr, sq_arr, conj, pad;
+  int32_t arr, sq_arr, conj, level;
   int64_t re, im, bs, sq, sbs;
-  double Nd;
 };
-void launch_assemble(int np, int B, const StatEntry* entries, const double* const* src, bool moments,
-                     double2* values, double* variance, hipStream_t s);
+void launch_assemble(int np, int B, int K, int levels, const StatEntry* entries, const double* const* src,
+                     const double* nd, bool moments, double2* values, double* variance, hipStream_t s);
 
 // Coefficient cotangents -> the per-level reverse operands (H, s1, s2 and the
-// mean / var factors), on the device, in a fixed order: target t of map b is
-// the sum over its terms k of (mult_k (conj_k ? conj(g) : g)) / Nd_k with
-// g = cot[b np + coef_k], in the terms' order; written to dst[arr][off + b bs]
-// (and off + 1, the imaginary part, when cplx).
+// mean / var factors), on the device, in a fixed order: target t of row
+// r = b K + k is the sum over its terms q of (mult_q (conj_q ? conj(g) : g)) /
+// Nd_q with g = cot[r np + coef_q], Nd_q = nd[k levels + level_q], in the
+// terms' order; written to dst[arr][off + r bs] (and off + 1, the imaginary
+// part, when cplx).
 struct RevTarget {
   int32_t arr, cplx, first, count;
   int64_t off, bs;
 };
 struct RevTerm {
-  int32_t coef, conj;
-  double mult, Nd;
+  int32_t coef, conj, level, pad;
+  double mult;
 };
-void launch_scatter_cot(int ntargets, int B, int np, const RevTarget* targets, const RevTerm* terms,
-                        const double2* cot, double* const* dst, hipStream_t s);
+void launch_scatter_cot(int ntargets, int B, int K, int levels, int np, const RevTarget* targets,
+                        const RevTerm* terms, const double* nd, const double2* cot, double* const* dst,
+                        hipStream_t s);
+// Per-mask reverse operands -> per-class ones: out [B][R][C][E] = sum over the
+// masks k of class c (bit k of pat[c], k ascending) of in [B][K][R][E].
+void launch_mask_combine(int B, int K, int R, int C, int64_t E, const uint64_t* pat, const double* in, double* out,
+                         hipStream_t s);
 
-// Reverse of the Gram products: gV = H V per pixel and (b, l3), H [B * kL][K][K]
-// complex (double2); writes the branch cotangents [(d B + b)][l3][N][l1] and
-// the z part [B][kL][N].
+// Reverse of the Gram products: gV = H V per pixel and (b, l3), H [B * kL][C][K][K]
+// complex (double2), the matrix of the pixel's mask class; writes the branch
+// cotangents [(d B + b)][l3][N][l1] and the z part [B][kL][N].
 template <typename T>
 void launch_gv(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, const double2* H,
-               cplx_t<T>* gbranch, cplx_t<T>* gz_part, hipStream_t s);
-// gz = gz_part + s1[b,l] z/|z| + s2[b,l] z + (h z/|z|) with h = modulus
-// cotangent row d = n of the chain stack [(d B + b)][N][kL].
+               cplx_t<T>* gbranch, cplx_t<T>* gz_part, const MaskView& mask, hipStream_t s);
+// gz = gz_part + s1[b,l,c] z/|z| + s2[b,l,c] z + (h z/|z|) with c the pixel's
+// mask class and h = modulus cotangent row d = n of the chain stack [(d B + b)][N][kL].
 template <typename T>
 void launch_gz(int n, int64_t N, int B, const cplx_t<T>* z, const cplx_t<T>* gz_part, const T* chain,
-               const double* s1, const double* s2, cplx_t<T>* gz, hipStream_t s);
-// g0 [B][N] += gmean[b] + gvar[b] x.
+               const double* s1, const double* s2, cplx_t<T>* gz, const MaskView& mask, hipStream_t s);
+// g0 [B][N] += gmean[b,c] + gvar[b,c] x, c the pixel's mask class.
 template <typename T>
-void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, T* g0, hipStream_t s);
+void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, T* g0,
+                         const MaskView& mask, hipStream_t s);
 // Element-wise precision casts (maps in, gradients out).
 template <typename TO, typename TI>
 void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s);

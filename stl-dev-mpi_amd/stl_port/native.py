@@ -238,26 +238,43 @@ class NativeEngine:
             raise RuntimeError("native engine is closed")
 
     def set_masks(self, masks):
-        """None restores whole-sky averages; masks are not implemented by this library."""
+        """Average over the masks from now on (None: the whole sky).  masks: a list of NESTED id
+        arrays, or [K, npix] / [npix] indicators (nonzero: in); level n + 1 holds the parents of
+        the level-n mask (reference native.py:243-264, mask.rs:43-70).  Values, variances and
+        cotangents are then [batch, K, ncoeff]."""
         with self._lock:
             self._open()
-            w = np.zeros((0, self.npix)) if masks is None else np.atleast_2d(np.asarray(masks, dtype=np.float64))
+            if masks is None:
+                w = np.zeros((0, self.npix))
+            elif (isinstance(masks, np.ndarray) and masks.dtype != object and masks.shape[-1] == self.npix
+                  and masks.ndim in (1, 2)):
+                w = np.ascontiguousarray(np.atleast_2d(masks), dtype=np.float64)
+            else:
+                w = np.zeros((len(masks), self.npix))
+                for k, ids in enumerate(masks):
+                    ids = np.asarray(ids, dtype=np.int64)
+                    if ids.size and (ids.min() < 0 or ids.max() >= self.npix):
+                        raise ValueError(f"mask {k}: pixel ids out of range")
+                    w[k, ids] = 1.0
+            if not np.isfinite(w).all():
+                raise ValueError("mask weights must be finite")
             self._current_key = None
-            if self._lib.stlp_set_masks(self._handle, _pointer(np.ascontiguousarray(w)), w.shape[0]):
+            if self._lib.stlp_set_masks(self._handle, _pointer(w), w.shape[0]):
                 raise _error(self._lib)
             self.nmasks = w.shape[0]
 
     def native_pixels(self):
-        """Pixels each coefficient averages: [ncoeff]."""
+        """Pixels each coefficient averages: [K, ncoeff] with masks, else [ncoeff]."""
         with self._lock:
             self._open()
-            out = np.empty(self.ncoeff, dtype=np.uintp)
+            out = np.empty((max(self.nmasks, 1), self.ncoeff), dtype=np.uintp)
             if self._lib.stlp_native_pixels(self._handle, out.ctypes.data_as(_SIZE)):
                 raise _error(self._lib)
-            return out.astype(np.int64)
+            out = out.astype(np.int64)
+            return out if self.nmasks else out[0]
 
     def _shape(self):
-        return (self.batch, self.ncoeff)
+        return (self.batch, self.nmasks, self.ncoeff) if self.nmasks else (self.batch, self.ncoeff)
 
     def _map_shape(self):
         return (self.batch, self.npix)

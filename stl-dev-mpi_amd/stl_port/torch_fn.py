@@ -30,12 +30,13 @@ class _Scattering(torch.autograd.Function):
     def backward(ctx, gy):
         cot = gy.detach().to(torch.complex128).cpu().numpy()
         out = torch.empty((ctx.engine.batch, ctx.engine.npix), dtype=torch.float64, device=gy.device)
-        ctx.engine.vjp_device(cot.reshape(ctx.engine.batch, ctx.engine.ncoeff), out, generation=ctx.generation)
+        ctx.engine.vjp_device(cot.reshape(ctx.engine._shape()), out, generation=ctx.generation)
         return out.reshape(ctx.shape), None
 
 
 def scattering(engine, x):
-    """Statistics of NativeEngine as a torch op: x [B, npix] (device) -> complex128 [B, ncoeff]."""
+    """Statistics of NativeEngine as a torch op: x [B, npix] (device) -> complex128 [B, ncoeff]
+    ([B, K, ncoeff] with K masks)."""
     return _Scattering.apply(x, engine)
 
 
