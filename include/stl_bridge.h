@@ -12,8 +12,9 @@
  *
  * Scope of this library: the GPU backend of the complete one-channel plan
  * (mean, var, S1..S4; kernel 5, 4 orientations, gauge cosmo, fewer = false,
- * dg(j) = j), whole-sky or masked averages.  The other products the reference's bridge also exports
- * (patches, spectra, U-Net, training, spin 2, cross statistics) are
+ * dg(j) = j) and the cross plan of map pairs ("cross": true), whole-sky or
+ * masked averages.  The other products the reference's bridge also exports
+ * (patches, spectra, U-Net, training, spin 2) are
  * present as symbols that fail with "not implemented".
  */
 #ifndef STL_BRIDGE_H
@@ -36,7 +37,9 @@ const char* stlp_last_error(void);
 void stlp_string_free(char* s);
 
 /* New handle from a JSON configuration (lib.rs:273-351): nside (128), j (7),
- * batch (1), kernel (5), orientations (4), block_pixels (accepted, unused:
+ * batch (1), cross (false: maps [batch, 12 nside^2]; true: pairs
+ * [batch, 2, 12 nside^2] and their cross plan, lib.rs:238-245), kernel (5),
+ * orientations (4), block_pixels (accepted, unused:
  * the whole sky is resident), backend ("gpu"; "cpu" is refused), device (0),
  * and the keys of this library: precision ("fp64" | "mixed": FP32 bulk
  * fields and operators with FP64 sky-wide sums), timing (false).  NULL on

@@ -142,7 +142,7 @@ std::string json_opt(int v) { return v < 0 ? "null" : std::to_string(v); }
 struct StlHandle {
   std::unique_ptr<stl::Engine> engine;
   uint64_t id = 0;
-  int nside = 0, batch = 1, J = 0;
+  int nside = 0, batch = 1, J = 0, channels = 1;
   size_t npix = 0;
   std::optional<Identity> current;
   stl::DevArray<double> xbuf, gbuf;
@@ -152,7 +152,8 @@ struct StlHandle {
   size_t nmask_rows() const { return size_t(std::max(engine->masks(), 1)); }
   // Values / cotangents: [batch][K][ncoef].
   size_t nvals() const { return size_t(batch) * nmask_rows() * ncoef(); }
-  size_t input_len() const { return size_t(batch) * npix; }
+  // Maps [batch][channels][12 nside^2] (channels 2: the pairs of the cross plan).
+  size_t input_len() const { return size_t(batch) * channels * npix; }
   Identity identity(uint64_t g, uint64_t m, uint64_t o) const { return Identity{id, g, m, o}; }
 };
 
@@ -189,7 +190,6 @@ StlHandle* stlp_new(const char* config) {
     std::string backend = v.string("backend", "gpu");
     if (backend == "cpu") fail("backend cpu is not provided by this library (GPU engine only)");
     if (backend != "gpu") fail("backend must be cpu or gpu, not \"" + backend + "\"");
-    if (v.boolean("cross", false)) fail("cross statistics are not implemented");
     if (v.integer("spin", 0) != 0) fail("spin 2 is not implemented");
     std::string precision = v.string("precision", "fp64");
     if (precision != "fp64" && precision != "mixed") fail("precision must be fp64 or mixed");
@@ -197,6 +197,7 @@ StlHandle* stlp_new(const char* config) {
     opt.nside = int(nside);
     opt.J = int(j);
     opt.batch = int(batch);
+    opt.cross = v.boolean("cross", false);
     opt.fp64 = precision == "fp64";
     opt.device = int(v.integer("device", 0));
     opt.timing = v.boolean("timing", false);
@@ -207,6 +208,7 @@ StlHandle* stlp_new(const char* config) {
     h->id = g_next_id.fetch_add(1);
     h->nside = int(nside);
     h->batch = int(batch);
+    h->channels = opt.cross ? 2 : 1;
     h->J = int(j);
     h->npix = size_t(12) * nside * nside;
     h->precision = precision;
@@ -222,7 +224,8 @@ char* stlp_info(StlHandle* h) {
     std::ostringstream o;
     o.precision(17);
     o << "{\"version\": \"" << kVersion << "\", \"nside\": " << h->nside << ", \"j\": " << h->J
-      << ", \"batch\": " << h->batch << ", \"channels\": 1, \"cross\": false, \"fewer\": false, \"backend\": \"gpu\""
+      << ", \"batch\": " << h->batch << ", \"channels\": " << h->channels
+      << ", \"cross\": " << (h->channels == 2 ? "true" : "false") << ", \"fewer\": false, \"backend\": \"gpu\""
       << ", \"precision\": \"" << h->precision << "\", \"dtype\": \""
       << (h->precision == "fp64" ? "float64/complex128" : "float32 fields and operators, float64 sums") << "\""
       << ", \"masks\": " << h->engine->masks() << ", \"spin\": 0, \"conv\": \"folded gather, 4-fold face symmetry\", \"fields\": \"keep\"";
@@ -367,7 +370,7 @@ int stlp_forward_levels(StlHandle* h, const double* x, size_t n, double* out) {
     if (!h || !x || !out) fail("null argument");
     size_t total = 0;
     for (int d = 0; d < h->engine->levels(); ++d)
-      total += size_t(h->batch) * 12 * (size_t(h->nside) >> d) * (size_t(h->nside) >> d);
+      total += size_t(h->batch) * h->channels * 12 * (size_t(h->nside) >> d) * (size_t(h->nside) >> d);
     if (n != total) fail("input shape mismatch");
     h->current.reset();
     stl::DevArray<double> buf(n);

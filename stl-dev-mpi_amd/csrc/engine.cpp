@@ -75,6 +75,42 @@ std::vector<Coefficient> build_plan(int J, int L) {
   return plan;
 }
 
+std::vector<Coefficient> build_cross_plan(int J, int L) {
+  std::vector<Coefficient> plan;
+  for (int j3 = 0; j3 < J; ++j3) {
+    const int dg = j3;
+    for (int fam : {F_S1, F_S2})
+      for (int l = 0; l < L; ++l) {
+        Coefficient c{fam, dg, 0, 1};
+        c.j1 = j3;
+        c.l1 = l;
+        plan.push_back(c);
+      }
+    for (int ca = 0; ca < 2; ++ca) {
+      const int cb = 1 - ca;
+      for (int j2 = 0; j2 <= j3; ++j2) {
+        for (int l2 = 0; l2 < L; ++l2)
+          for (int l3 = 0; l3 < L; ++l3) {
+            Coefficient c{F_S3, dg, ca, cb};
+            c.j1 = j2; c.j3 = j3; c.l1 = l2; c.l3 = l3;
+            plan.push_back(c);
+          }
+        for (int j1 = 0; j1 <= j2; ++j1)
+          for (int l1 = 0; l1 < L; ++l1)
+            for (int l2 = 0; l2 < L; ++l2)
+              for (int l3 = 0; l3 < L; ++l3) {
+                Coefficient c{F_S4, dg, ca, cb};
+                c.j1 = j1; c.j2 = j2; c.j3 = j3; c.l1 = l1; c.l2 = l2; c.l3 = l3;
+                plan.push_back(c);
+              }
+      }
+    }
+  }
+  std::stable_sort(plan.begin(), plan.end(),
+                   [](const Coefficient& a, const Coefficient& b) { return a.native_dg < b.native_dg; });
+  return plan;
+}
+
 namespace {
 
 inline int gram_k(int n) { return (n + 1) * kL + 1; }
@@ -83,13 +119,41 @@ inline int tri_index(int K, int i, int j) { return i * K - i * (i - 1) / 2 + (j 
 
 // Where a coefficient's sum lives.
 struct SumSlot {
-  enum Kind { MEANVAR, ZSTAT, GRAM } kind;
+  enum Kind { MEANVAR, ZSTAT, GRAM, ZCROSS, GRAMX } kind;
   int n = 0;      // level
   int l = 0;      // gauge (S1/S2: l1, Gram: l3)
   int idx = 0;    // meanvar / zstat column, or Gram entry
   bool conj = false;
   int gi = -1, gj = -1;  // Gram operand indices (reverse)
 };
+
+// A coefficient of the cross plan: S1 from the S1 cross sums; S2, S3 and S4
+// entries (i, j) of the cross Gram block G = V_a V_b^H (channel a of the
+// coefficient first: for c1 = 1 the conjugate of entry (j, i)).
+SumSlot cross_slot_of(const Coefficient& c) {
+  SumSlot s;
+  s.n = c.native_dg;
+  const int K = gram_k(s.n);
+  if (c.family == F_S1) {
+    s.kind = SumSlot::ZCROSS;
+    s.l = c.l1;
+    return s;
+  }
+  s.kind = SumSlot::GRAMX;
+  int i, j;
+  if (c.family == F_S2) {
+    s.l = c.l1;
+    i = j = K - 1;
+  } else {
+    s.l = c.l3;
+    i = c.j1 * kL + c.l1;
+    j = c.family == F_S3 ? K - 1 : c.j2 * kL + c.l2;
+  }
+  if (c.c1 == 0) { s.gi = i; s.gj = j; s.conj = false; }
+  else { s.gi = j; s.gj = i; s.conj = true; }
+  s.idx = s.gi * K + s.gj;
+  return s;
+}
 
 SumSlot slot_of(const Coefficient& c) {
   SumSlot s;
@@ -218,10 +282,12 @@ class EngineT final : public Engine {
     STL_HIP(hipSetDevice(opt.device));
     STL_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     prof_.enable(opt.timing, s_);
-    plan_ = build_plan(opt.J, kL);
+    cross_ = opt.cross;
+    plan_ = cross_ ? build_cross_plan(opt.J, kL) : build_plan(opt.J, kL);
     levels_ = opt.J;
-    for (auto& c : plan_) slots_.push_back(slot_of(c));
-    B_ = opt.batch;
+    for (auto& c : plan_) slots_.push_back(cross_ ? cross_slot_of(c) : slot_of(c));
+    Bs_ = opt.batch;
+    B_ = cross_ ? 2 * opt.batch : opt.batch;
     for (int n = 0; n < levels_; ++n) {
       int order = order0 - n;
       N_.push_back(12 * (int64_t(1) << (2 * order)));
@@ -356,27 +422,31 @@ class EngineT final : public Engine {
   void sweep_reverse(double* grad_dev) {
     const size_t nv = nvals();
     STL_HIP(hipMemcpyAsync(dcot_.data(), hcot_.data(), nv * sizeof(double2), hipMemcpyHostToDevice, s_));
-    launch_scatter_cot(nrt_, B_, Km_, levels_, int(plan_.size()), drt_.data(), drk_.data(), dnd_.data(),
+    launch_scatter_cot(nrt_, Bs_, Km_, levels_, int(plan_.size()), drt_.data(), drk_.data(), dnd_.data(),
                        dcot_.data(), ddst_.data(), s_);
     if (nmask_ > 0) {
       // Per-mask operands -> per-class ones (the sum over the masks holding the class).
       for (int n = 0; n < levels_; ++n) {
         const int K = gram_k(n), C = mC_[n];
         const uint64_t* pat = mpat_[n].data();
-        launch_mask_combine(B_, Km_, kL, C, int64_t(K) * K * 2, pat, reinterpret_cast<const double*>(dH_[n].data()),
-                            reinterpret_cast<double*>(Hc_[n].data()), s_);
-        launch_mask_combine(B_, Km_, kL, C, 1, pat, ds1_[n].data(), s1c_[n].data(), s_);
-        launch_mask_combine(B_, Km_, kL, C, 1, pat, ds2_[n].data(), s2c_[n].data(), s_);
+        launch_mask_combine(Bs_, Km_, kL, C, int64_t(K) * K * 2, pat,
+                            reinterpret_cast<const double*>(dH_[n].data()), reinterpret_cast<double*>(Hc_[n].data()),
+                            s_);
+        launch_mask_combine(Bs_, Km_, kL, C, cross_ ? 2 : 1, pat, ds1_[n].data(), s1c_[n].data(), s_);
+        if (!cross_) launch_mask_combine(Bs_, Km_, kL, C, 1, pat, ds2_[n].data(), s2c_[n].data(), s_);
       }
-      launch_mask_combine(B_, Km_, 1, mC_[0], 1, mpat_[0].data(), dgm_.data(), gmc_.data(), s_);
-      launch_mask_combine(B_, Km_, 1, mC_[0], 1, mpat_[0].data(), dgv_.data(), gvc_.data(), s_);
+      if (!cross_) {
+        launch_mask_combine(Bs_, Km_, 1, mC_[0], 1, mpat_[0].data(), dgm_.data(), gmc_.data(), s_);
+        launch_mask_combine(Bs_, Km_, 1, mC_[0], 1, mpat_[0].data(), dgv_.data(), gvc_.data(), s_);
+      }
     }
     for (int n = levels_ - 1; n >= 0; --n) {
       const int64_t N = N_[n];
       const MaskView mv = mask_view(n);
       const double2* H = nmask_ > 0 ? Hc_[n].data() : dH_[n].data();
       int id = prof_.start();
-      launch_gv<T>(n, N, B_, br_[n].data(), z_[n].data(), H, gbr_.data(), gzp_.data(), mv, s_);
+      if (cross_) launch_gv_x<T>(n, N, Bs_, br_[n].data(), z_[n].data(), H, gbr_.data(), gzp_.data(), mv, s_);
+      else launch_gv<T>(n, N, B_, br_[n].data(), z_[n].data(), H, gbr_.data(), gzp_.data(), mv, s_);
       prof_.stop(id, tag("gv", n), double((n + 2) * B_ * kL) * N * kL * 2 * sizeof(T) * 2);
 
       T* chain = acc_[n % 2].data();
@@ -387,7 +457,11 @@ class EngineT final : public Engine {
       id = prof_.start();
       const double* s1 = nmask_ > 0 ? s1c_[n].data() : ds1_[n].data();
       const double* s2 = nmask_ > 0 ? s2c_[n].data() : ds2_[n].data();
-      launch_gz<T>(n, N, B_, z_[n].data(), gzp_.data(), chain, s1, s2, gz_.data(), mv, s_);
+      if (cross_)
+        launch_gz_x<T>(n, N, Bs_, z_[n].data(), gzp_.data(), chain, reinterpret_cast<const double2*>(s1), gz_.data(),
+                       mv, s_);
+      else
+        launch_gz<T>(n, N, B_, z_[n].data(), gzp_.data(), chain, s1, s2, gz_.data(), mv, s_);
       prof_.stop(id, tag("gz", n), double(B_) * kL * N * (6 * sizeof(T)));
 
       T* mg = mg_[n % 2].data();
@@ -395,8 +469,9 @@ class EngineT final : public Engine {
       conv_adj(n, 1, gz_.data(), B_, mg);
       if (n + 1 < levels_) dec_adj(n, 1, mg_prev, mg, B_, mg);
     }
-    launch_meanvar_grad<T>(N_[0], B_, maps_[0].data(), nmask_ > 0 ? gmc_.data() : dgm_.data(),
-                           nmask_ > 0 ? gvc_.data() : dgv_.data(), mg_[0].data(), mask_view(0), s_);
+    if (!cross_)
+      launch_meanvar_grad<T>(N_[0], B_, maps_[0].data(), nmask_ > 0 ? gmc_.data() : dgm_.data(),
+                             nmask_ > 0 ? gvc_.data() : dgv_.data(), mg_[0].data(), mask_view(0), s_);
     cast_out(mg_[0].data(), grad_dev, int64_t(B_) * N_[0]);
   }
 
@@ -505,10 +580,10 @@ class EngineT final : public Engine {
     s2c_.resize(levels_);
     nblk_gram_.clear();
     nblk_z_.clear();
-    const size_t rows = size_t(B_) * Km_;
+    const size_t rows = size_t(Bs_) * Km_;
     for (int n = 0; n < levels_; ++n) {
       int64_t N = N_[n];
-      int K = gram_k(n), E = gram_e(n);
+      int K = gram_k(n), E = cross_ ? K * K : gram_e(n);
       int64_t tiles = (N + gram_tile(n) - 1) / gram_tile(n);
       int nb = int(std::min<int64_t>(tiles, std::max<int64_t>(1, 2048 / (int64_t(rows) * kL))));
       nblk_gram_.push_back(nb);
@@ -521,20 +596,20 @@ class EngineT final : public Engine {
       zpart_[n].resize(size_t(nz) * rows * kL * 4);
       zsum_[n].resize(rows * kL * 4);
       dH_[n].resize(rows * kL * K * K);
-      ds1_[n].resize(rows * kL);
+      ds1_[n].resize(rows * kL * (cross_ ? 2 : 1));  // cross: complex S1 cotangents
       ds2_[n].resize(rows * kL);
       const size_t cls = nmask_ > 0 ? size_t(mC_[n]) : 0;
-      Hc_[n].resize(size_t(B_) * kL * cls * K * K);
-      s1c_[n].resize(size_t(B_) * kL * cls);
-      s2c_[n].resize(size_t(B_) * kL * cls);
+      Hc_[n].resize(size_t(Bs_) * kL * cls * K * K);
+      s1c_[n].resize(size_t(Bs_) * kL * cls * (cross_ ? 2 : 1));
+      s2c_[n].resize(size_t(Bs_) * kL * cls);
     }
     nblk_mv_ = int(std::min<int64_t>((N_[0] + 1023) / 1024, 1024));
     mvpart_.resize(size_t(nblk_mv_) * rows * 4);
     mvsum_.resize(rows * 4);
     dgm_.resize(rows);
     dgv_.resize(rows);
-    gmc_.resize(nmask_ > 0 ? size_t(B_) * mC_[0] : 0);
-    gvc_.resize(nmask_ > 0 ? size_t(B_) * mC_[0] : 0);
+    gmc_.resize(nmask_ > 0 ? size_t(Bs_) * mC_[0] : 0);
+    gvc_.resize(nmask_ > 0 ? size_t(Bs_) * mC_[0] : 0);
     std::vector<double> nd(size_t(Km_) * levels_);
     for (int k = 0; k < Km_; ++k)
       for (int n = 0; n < levels_; ++n)
@@ -554,7 +629,7 @@ class EngineT final : public Engine {
     return v;
   }
 
-  size_t nvals() const { return size_t(B_) * Km_ * plan_.size(); }
+  size_t nvals() const { return size_t(Bs_) * Km_ * plan_.size(); }
 
   void drop_graphs() {
     for (auto* cache : {&fwd_graphs_, &vjp_graphs_}) {
@@ -629,24 +704,30 @@ class EngineT final : public Engine {
       id = prof_.start();
       int nb = nblk_gram_[n];
       const MaskView mv = mask_view(n);
-      const int64_t rows = int64_t(B_) * Km_;
-      int64_t m = rows * kL * gram_e(n);
-      launch_gram<T>(n, N, B_, br_[n].data(), z_[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(), mv,
-                     s_);
+      const int64_t rows = int64_t(Bs_) * Km_;
+      const int K = gram_k(n);
+      int64_t m = rows * kL * (cross_ ? K * K : gram_e(n));
+      if (cross_)
+        launch_gram_x<T>(n, N, Bs_, br_[n].data(), z_[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(),
+                         mv, s_);
+      else
+        launch_gram<T>(n, N, B_, br_[n].data(), z_[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(), mv,
+                       s_);
       launch_reduce_parts(gpart_[n].data(), nb, m * 2, gsum_[n].data(), s_);
       if (moments) launch_reduce_parts(gpart2_[n].data(), nb, m, gsum2_[n].data(), s_);
       prof_.stop(id, tag("gram", n), double(rows) * kL * N * (gram_k(n)) * 2 * sizeof(T));
       id = prof_.start();
-      launch_zstats<T>(N, B_, z_[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
+      if (cross_) launch_zcross<T>(N, Bs_, z_[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
+      else launch_zstats<T>(N, B_, z_[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
       launch_reduce_parts(zpart_[n].data(), nblk_z_[n], rows * kL * 4, zsum_[n].data(), s_);
-      if (n == 0) {
+      if (n == 0 && !cross_) {
         launch_meanvar<T>(N, B_, maps_[0].data(), nblk_mv_, mvpart_.data(), mv, s_);
         launch_reduce_parts(mvpart_.data(), nblk_mv_, rows * 4, mvsum_.data(), s_);
       }
       prof_.stop(id, tag("zstats", n), double(rows) * kL * N * 2 * sizeof(T));
     }
     const size_t nv = nvals();
-    launch_assemble(int(plan_.size()), B_, Km_, levels_, dstat_.data(), dsrc_.data(), dnd_.data(), moments,
+    launch_assemble(int(plan_.size()), Bs_, Km_, levels_, dstat_.data(), dsrc_.data(), dnd_.data(), moments,
                     dvalues_.data(), dvar_.data(), s_);
     STL_HIP(hipMemcpyAsync(hvalues_.data(), dvalues_.data(), nv * sizeof(double2), hipMemcpyDeviceToHost, s_));
     if (moments)
@@ -819,7 +900,8 @@ class EngineT final : public Engine {
       hbase[n] = int(targets.size());
       for (int e = 0; e < kL * K * K; ++e) targets.push_back(RevTarget{2 + 3 * n, 1, 0, 0, 2 * e, 2 * kL * K * K});
       s1base[n] = int(targets.size());
-      for (int l = 0; l < kL; ++l) targets.push_back(RevTarget{3 + 3 * n, 0, 0, 0, l, kL});
+      for (int l = 0; l < kL; ++l)
+        targets.push_back(cross_ ? RevTarget{3 + 3 * n, 1, 0, 0, 2 * l, 2 * kL} : RevTarget{3 + 3 * n, 0, 0, 0, l, kL});
       s2base[n] = int(targets.size());
       for (int l = 0; l < kL; ++l) targets.push_back(RevTarget{4 + 3 * n, 0, 0, 0, l, kL});
     }
@@ -841,6 +923,20 @@ class EngineT final : public Engine {
           e.sq_arr = 3 + 3 * n; e.sq = sl.l * 4 + 2 + sl.idx; e.sbs = kL * 4; e.level = n;
           lists[(sl.idx == 0 ? s1base[n] : s2base[n]) + sl.l].push_back(
               RevTerm{i, 0, n, 0, sl.idx == 0 ? 1.0 : 2.0});
+          break;
+        }
+        case SumSlot::ZCROSS: {
+          e.arr = 3 + 3 * n; e.re = sl.l * 4; e.im = sl.l * 4 + 1; e.bs = kL * 4;
+          e.sq_arr = 3 + 3 * n; e.sq = sl.l * 4 + 2; e.sbs = kL * 4; e.level = n;
+          lists[s1base[n] + sl.l].push_back(RevTerm{i, 0, n, 0, 1.0});
+          break;
+        }
+        case SumSlot::GRAMX: {
+          const int K = gram_k(n), E = K * K;
+          const int64_t el = int64_t(sl.l) * E + sl.idx;
+          e.arr = 1 + 3 * n; e.re = 2 * el; e.im = 2 * el + 1; e.bs = int64_t(kL) * E * 2; e.conj = sl.conj;
+          e.sq_arr = 2 + 3 * n; e.sq = el; e.sbs = int64_t(kL) * E; e.level = n;
+          lists[hbase[n] + (sl.l * K + sl.gi) * K + sl.gj].push_back(RevTerm{i, sl.conj ? 1 : 0, n, 0, 1.0});
           break;
         }
         case SumSlot::GRAM: {
@@ -889,7 +985,9 @@ class EngineT final : public Engine {
   Profiler prof_;
   std::vector<Coefficient> plan_;
   std::vector<SumSlot> slots_;
-  int levels_ = 0, B_ = 1;
+  // B_: field rows (maps, or 2 per pair with cross); Bs_: statistics rows (maps / pairs).
+  int levels_ = 0, B_ = 1, Bs_ = 1;
+  bool cross_ = false;
   std::vector<int64_t> N_;
   std::vector<std::shared_ptr<const SharedConv<T>>> conv_;
   std::vector<std::shared_ptr<const SharedDec<T>>> dec_;

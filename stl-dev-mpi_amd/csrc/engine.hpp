@@ -27,11 +27,16 @@ struct Coefficient {
 
 // The complete plan of one channel, sorted by native level (plan.rs:378-456).
 std::vector<Coefficient> build_plan(int J, int L);
+// The cross terms of two channels (c1 != c2 of the two-channel plan, the
+// reference's plan_of(cross = true), crates/stl-bridge/src/lib.rs:238-245):
+// per scale S1 and S2 of (0, 1), then S3 and S4 of (0, 1) and of (1, 0).
+std::vector<Coefficient> build_cross_plan(int J, int L);
 
 struct EngineOptions {
   int nside = 128;
   int J = 7;
   int batch = 1;
+  bool cross = false;    // map pairs [batch][2][12 nside^2] and the cross plan
   bool fp64 = true;      // false: FP32 bulk fields and operators, FP64 sums
   int device = 0;
   bool timing = false;   // per-kernel HIP-event timers
@@ -73,8 +78,9 @@ class Engine {
   // Masks in use (0: the whole sky) and their pixel counts [k][levels].
   virtual int masks() const = 0;
   virtual std::vector<int64_t> mask_counts() const = 0;
-  // Forward on maps [B][12 nside^2] in device memory (float64); values and
-  // variances ([B][K][ncoef], K = max(masks, 1)) to host memory.
+  // Forward on maps [B][12 nside^2] ([B][2][12 nside^2] with cross) in
+  // device memory (float64); values and variances ([B][K][ncoef],
+  // K = max(masks, 1)) to host memory.
   virtual void forward_device(const double* x_dev, bool moments, cplx* values, double* variance) = 0;
   // Forward on maps given at every level (concatenated, device memory), no decimation of level 0.
   virtual void forward_levels_device(const double* x_dev, cplx* values) = 0;

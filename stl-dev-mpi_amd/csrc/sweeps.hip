@@ -1228,6 +1228,150 @@ __device__ inline void block_sum4(double v[4], double* sh) {
   for (int c = 0; c < 4; ++c) v[c] = sh[c * 256];
 }
 
+// Cross Gram products of a map pair: G[i][j] = sum_p V_a,i conj(V_b,j), the
+// full K x K block between the V of field row 2b (channel a) and of row
+// 2b + 1 (channel b), per (b, mask k, l3).  Tiles of both V staged in LDS;
+// each thread owns a BS x BS block (bi, bj) of the NR x NR grid and a group
+// of the tile's pixels; the group sums are added in group order per entry
+// and written straight out: part [nblk][B K kL][K K][2] (+ [K K] |.|^2).
+template <typename T, bool MOM, int BS, bool MASKED>
+__global__ __launch_bounds__(256) void k_gram_x(int n, int64_t N, int Bf, const cplx_t<T>* __restrict__ br,
+                                                const cplx_t<T>* __restrict__ z, int TPX, double* __restrict__ part,
+                                                double* __restrict__ part2, MaskView mask) {
+  extern __shared__ double2 smem[];
+  const int K = (n + 1) * kL + 1;
+  const int KP = (K + BS - 1) / BS * BS, NR = KP / BS, NBLK = NR * NR;
+  const int E = K * K;
+  double2* Va = smem;
+  double2* Vb = smem + size_t(TPX) * KP;
+  const int bl = blockIdx.y;  // (b K + k) kL + l3
+  const int l3 = bl % kL, mk = (bl / kL) % mask.K, b = bl / (kL * mask.K);
+  const int tid = threadIdx.x;
+  const int G = 256 / NBLK;
+  const int blk = tid % NBLK, grp = tid / NBLK;
+  const bool act = grp < G;
+  const int bi = blk / NR, bj = blk % NR;
+  double are[BS][BS], aim[BS][BS], asq[BS][BS];
+#pragma unroll
+  for (int r = 0; r < BS; ++r)
+#pragma unroll
+    for (int c = 0; c < BS; ++c) are[r][c] = aim[r][c] = asq[r][c] = 0.0;
+  const int64_t ntiles = (N + TPX - 1) / TPX;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t p0 = tile * TPX;
+    const int tn = int(min<int64_t>(TPX, N - p0));
+    for (int h = 0; h < 2; ++h) {
+      double2* V = h == 0 ? Va : Vb;
+      const int row = 2 * b + h;
+      for (int d = 0; d <= n; ++d) {
+        const cplx_t<T>* src = br + ((size_t(d) * Bf + row) * kL + l3) * N * kL + size_t(p0) * kL;
+        for (int idx = tid; idx < tn * kL; idx += blockDim.x) {
+          int p = idx / kL, l1 = idx % kL;
+          // Channel a outside the mask stages zeros: its products add nothing.
+          V[p * KP + d * kL + l1] = h == 1 || !MASKED || in_mask(mask, p0 + p, mk) ? to_d2<T>(src[idx])
+                                                                                     : make_double2(0.0, 0.0);
+        }
+      }
+      const cplx_t<T>* zs = z + (size_t(row) * kL + l3) * N + p0;
+      for (int p = tid; p < tn; p += blockDim.x)
+        V[p * KP + K - 1] = h == 1 || !MASKED || in_mask(mask, p0 + p, mk) ? to_d2<T>(zs[p]) : make_double2(0.0, 0.0);
+      const int pad = KP - K;
+      for (int idx = tid; idx < tn * pad; idx += blockDim.x) V[(idx / pad) * KP + K + idx % pad] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if (act) {
+      for (int p = grp; p < tn; p += G) {
+        double2 a[BS], c[BS];
+#pragma unroll
+        for (int r = 0; r < BS; ++r) {
+          a[r] = Va[p * KP + BS * bi + r];
+          c[r] = Vb[p * KP + BS * bj + r];
+        }
+#pragma unroll
+        for (int r = 0; r < BS; ++r)
+#pragma unroll
+          for (int q = 0; q < BS; ++q) {
+            double re = a[r].x * c[q].x + a[r].y * c[q].y;
+            double im = a[r].y * c[q].x - a[r].x * c[q].y;
+            are[r][q] += re;
+            aim[r][q] += im;
+            if (MOM) asq[r][q] += re * re + im * im;
+          }
+      }
+    }
+    __syncthreads();
+  }
+  double2* red = smem;                                   // [256]
+  double* red2 = reinterpret_cast<double*>(smem + 256);  // [256]
+  const size_t base = (size_t(blockIdx.x) * gridDim.y + bl) * E;
+#pragma unroll
+  for (int r = 0; r < BS; ++r)
+#pragma unroll
+    for (int q = 0; q < BS; ++q) {
+      red[tid] = make_double2(are[r][q], aim[r][q]);
+      if (MOM) red2[tid] = asq[r][q];
+      __syncthreads();
+      if (tid < NBLK) {
+        const int i = BS * (tid / NR) + r, j = BS * (tid % NR) + q;
+        double2 sum = make_double2(0.0, 0.0);
+        double s2 = 0.0;
+        for (int g = 0; g < G; ++g) {
+          sum.x += red[g * NBLK + tid].x;
+          sum.y += red[g * NBLK + tid].y;
+          if (MOM) s2 += red2[g * NBLK + tid];
+        }
+        if (i < K && j < K) {
+          part[(base + size_t(i) * K + j) * 2] = sum.x;
+          part[(base + size_t(i) * K + j) * 2 + 1] = sum.y;
+          if (MOM) part2[base + size_t(i) * K + j] = s2;
+        }
+      }
+      __syncthreads();
+    }
+}
+
+__device__ inline double2 root_normalise(double2 v) {
+  const double s = sqrt(hypot(v.x, v.y) + 1e-8);
+  return make_double2(v.x / s, v.y / s);
+}
+
+// Cotangent of v for the root-normalised operand v / sqrt(|v| + eps) given
+// the cotangent g of that operand (the reference's root_modulus_vjp,
+// crates/stl-core/src/engine.rs:46-56).
+__device__ inline double2 root_modulus_vjp(double2 v, double2 g) {
+  const double r = hypot(v.x, v.y);
+  const double radial = g.x * v.x + g.y * v.y;
+  const double mag = r == 0.0 ? 1.0 : r;
+  const double q = sqrt(r + 1e-8);
+  const double s = 0.5 * radial / ((r + 1e-8) * q);
+  return make_double2(g.x / q - (s * v.x) / mag, g.y / q - (s * v.y) / mag);
+}
+
+// S1 cross sums of a map pair per (b, mask k, l): t = rn(z_a) conj(rn(z_b)),
+// rn(v) = v / sqrt(|v| + 1e-8): part [nblk][B K kL][4] = (sum re t, sum im t,
+// sum |t|^2, 0).
+template <typename T>
+__global__ __launch_bounds__(256) void k_zcross(int64_t N, const cplx_t<T>* __restrict__ z, double* __restrict__ part,
+                                                MaskView mask) {
+  __shared__ double sh[4 * 256];
+  const int bl = blockIdx.y;
+  const int l = bl % kL, mk = (bl / kL) % mask.K, b = bl / (kL * mask.K);
+  const cplx_t<T>* za = z + (size_t(2 * b) * kL + l) * N;
+  const cplx_t<T>* zb = z + (size_t(2 * b + 1) * kL + l) * N;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < N; p += int64_t(gridDim.x) * blockDim.x) {
+    if (!in_mask(mask, p, mk)) continue;
+    const double2 a = root_normalise(to_d2<T>(za[p])), c = root_normalise(to_d2<T>(zb[p]));
+    const double re = a.x * c.x + a.y * c.y, im = a.y * c.x - a.x * c.y;
+    v[0] += re;
+    v[1] += im;
+    v[2] += re * re + im * im;
+  }
+  block_sum4(v, sh);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 4; ++c) part[(size_t(blockIdx.x) * gridDim.y + bl) * 4 + c] = v[c];
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_zstats(int64_t N, const cplx_t<T>* __restrict__ z, double* __restrict__ part,
                                                 MaskView mask) {
@@ -1401,6 +1545,100 @@ __global__ __launch_bounds__(256) void k_gv(int64_t N, int B, const cplx_t<T>* _
   V[K - 1] = to_d2<T>(z[(size_t(b) * kL + l3) * N + p]);
   if (LDS) gv_apply<T, K>(Hsm + hc, V, N, B, b, l3, p, gbr, gzp);
   else gv_apply<T, K>(Hrow + hc, V, N, B, b, l3, p, gbr, gzp);
+}
+
+// Reverse of the cross Gram products of a map pair: with H [B kL][C][K][K]
+// the cotangent of G = V_a V_b^H (per class), gV_a = H V_b and
+// gV_b = H^H V_a; writes the branch cotangents and z parts of field rows 2b
+// and 2b + 1.  H staged in LDS when LDS.
+template <typename T, int NL, bool LDS>
+__global__ __launch_bounds__(256) void k_gv_x(int64_t N, int Bf, const cplx_t<T>* __restrict__ br,
+                                              const cplx_t<T>* __restrict__ z, const double2* __restrict__ H,
+                                              cplx_t<T>* __restrict__ gbr, cplx_t<T>* __restrict__ gzp, MaskView mask) {
+  constexpr int K = NL * kL + 1;
+  extern __shared__ double2 Hsm[];
+  const int bl = blockIdx.y;  // b kL + l3
+  const int b = bl / kL, l3 = bl % kL;
+  const double2* Hrow = H + size_t(bl) * mask.C * K * K;
+  if (LDS) {
+    for (int e = threadIdx.x; e < mask.C * K * K; e += blockDim.x) Hsm[e] = Hrow[e];
+    __syncthreads();
+  }
+  const int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (p >= N) return;
+  const double2* Hs = (LDS ? static_cast<const double2*>(Hsm) : Hrow) + size_t(mask_class(mask, p)) * K * K;
+  double2 V[K];
+  auto load = [&](int row) {
+#pragma unroll
+    for (int d = 0; d < NL; ++d) {
+      const cplx_t<T>* src = br + (((size_t(d) * Bf + row) * kL + l3) * N + p) * kL;
+#pragma unroll
+      for (int l1 = 0; l1 < kL; ++l1) V[d * kL + l1] = to_d2<T>(src[l1]);
+    }
+    V[K - 1] = to_d2<T>(z[(size_t(row) * kL + l3) * N + p]);
+  };
+  auto store = [&](int row, int i, double re, double im) {
+    if (i < K - 1) {
+      const int d = i / kL, l1 = i % kL;
+      gbr[(((size_t(d) * Bf + row) * kL + l3) * N + p) * kL + l1] = make_c<T>(T(re), T(im));
+    } else {
+      gzp[(size_t(row) * kL + l3) * N + p] = make_c<T>(T(re), T(im));
+    }
+  };
+  // Channel a: gV_a[i] = sum_j H[i][j] V_b[j].
+  load(2 * b + 1);
+#pragma unroll 1
+  for (int i = 0; i < K; ++i) {
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const double2 h = Hs[i * K + j];
+      re += h.x * V[j].x - h.y * V[j].y;
+      im += h.x * V[j].y + h.y * V[j].x;
+    }
+    store(2 * b, i, re, im);
+  }
+  // Channel b: gV_b[j] = sum_i conj(H[i][j]) V_a[i].
+  load(2 * b);
+#pragma unroll 1
+  for (int j = 0; j < K; ++j) {
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const double2 h = Hs[i * K + j];
+      re += h.x * V[i].x + h.y * V[i].y;
+      im += h.x * V[i].y - h.y * V[i].x;
+    }
+    store(2 * b + 1, j, re, im);
+  }
+}
+
+// gz of a pair's field row (2b + c, l): gz_part + the modulus cotangent h
+// z/|z| of the chain + the S1 cross term root_modulus_vjp(z_c, x) with
+// x = g rn(z_other) for channel a and conj(g) rn(z_other) for channel b,
+// g = s1x[(b kL + l) C + class] (complex).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gz_x(int n, int64_t N, int Bf, const cplx_t<T>* __restrict__ z,
+                                              const cplx_t<T>* __restrict__ gzp, const T* __restrict__ chain,
+                                              const double2* __restrict__ s1x, cplx_t<T>* __restrict__ gz,
+                                              MaskView mask) {
+  const int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (p >= N) return;
+  const int rl = blockIdx.y;  // row kL + l
+  const int row = rl / kL, l = rl % kL, b = row / 2, c = row % 2;
+  const size_t at = size_t(rl) * N + p;
+  const double2 zc = to_d2<T>(z[at]);
+  const double2 zo = to_d2<T>(z[(size_t(row ^ 1) * kL + l) * N + p]);
+  const double mag = hypot(zc.x, zc.y);
+  const double den = mag == 0.0 ? 1.0 : mag;
+  const double h = double(chain[((size_t(n) * Bf + row) * N + p) * kL + l]);
+  double2 g = s1x[(size_t(b) * kL + l) * mask.C + mask_class(mask, p)];
+  if (c == 1) g.y = -g.y;
+  const double2 o = root_normalise(zo);
+  const double2 x = make_double2(g.x * o.x - g.y * o.y, g.x * o.y + g.y * o.x);
+  const double2 r = root_modulus_vjp(zc, x);
+  const double2 gp = to_d2<T>(gzp[at]);
+  gz[at] = make_c<T>(T(gp.x + (h * zc.x) / den + r.x), T(gp.y + (h * zc.y) / den + r.y));
 }
 
 template <typename T>
@@ -1660,6 +1898,68 @@ void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<
 }
 
 template <typename T>
+void launch_gram_x(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, bool moments, int nblk,
+                   double* part, double* part2, const MaskView& mask, hipStream_t s) {
+  const int K = (n + 1) * kL + 1;
+  const int BS = K <= 13 ? 2 : 4;
+  const int KP = (K + BS - 1) / BS * BS, NR = KP / BS;
+  STL_CHECK(NR * NR <= 256, "cross gram: too many levels");
+  const int tpx = gram_tile(n);
+  const size_t lds = std::max(size_t(2) * tpx * KP * sizeof(double2), size_t(256) * (sizeof(double2) + sizeof(double)));
+  STL_CHECK(lds <= 64 * 1024, "cross gram: LDS tile too large");
+  dim3 grid(nblk, B * mask.K * kL);
+  const bool masked = mask.cls != nullptr;
+  const int Bf = 2 * B;
+#define STL_GRAMX(MOM, BSZ)                                                                                         \
+  {                                                                                                                 \
+    if (masked) k_gram_x<T, MOM, BSZ, true><<<grid, 256, lds, s>>>(n, N, Bf, branch, z, tpx, part, part2, mask);   \
+    else k_gram_x<T, MOM, BSZ, false><<<grid, 256, lds, s>>>(n, N, Bf, branch, z, tpx, part, part2, mask);         \
+  }
+  if (BS == 2) {
+    if (moments) STL_GRAMX(true, 2) else STL_GRAMX(false, 2)
+  } else {
+    if (moments) STL_GRAMX(true, 4) else STL_GRAMX(false, 4)
+  }
+#undef STL_GRAMX
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_zcross(int64_t N, int B, const cplx_t<T>* z, int nblk, double* part, const MaskView& mask, hipStream_t s) {
+  k_zcross<T><<<dim3(nblk, B * mask.K * kL), 256, 0, s>>>(N, z, part, mask);
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_gv_x(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, const double2* H,
+                 cplx_t<T>* gbranch, cplx_t<T>* gz_part, const MaskView& mask, hipStream_t s) {
+  dim3 grid(nblocks(N, 256), B * kL);
+  const int K = (n + 1) * kL + 1;
+  const size_t hbytes = size_t(mask.C) * K * K * sizeof(double2);
+  const bool lds = hbytes <= 64 * 1024;
+  const int Bf = 2 * B;
+  switch (n + 1) {
+#define STL_GVX(NL)                                                                                     \
+  case NL:                                                                                              \
+    if (lds) k_gv_x<T, NL, true><<<grid, 256, hbytes, s>>>(N, Bf, branch, z, H, gbranch, gz_part, mask); \
+    else k_gv_x<T, NL, false><<<grid, 256, 0, s>>>(N, Bf, branch, z, H, gbranch, gz_part, mask);         \
+    break;
+    STL_GVX(1) STL_GVX(2) STL_GVX(3) STL_GVX(4) STL_GVX(5) STL_GVX(6) STL_GVX(7) STL_GVX(8) STL_GVX(9)
+    STL_GVX(10) STL_GVX(11) STL_GVX(12)
+#undef STL_GVX
+    default: throw std::runtime_error("gv cross: too many levels");
+  }
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_gz_x(int n, int64_t N, int B, const cplx_t<T>* z, const cplx_t<T>* gz_part, const T* chain,
+                 const double2* s1x, cplx_t<T>* gz, const MaskView& mask, hipStream_t s) {
+  k_gz_x<T><<<dim3(nblocks(N, 256), 2 * B * kL), 256, 0, s>>>(n, N, 2 * B, z, gz_part, chain, s1x, gz, mask);
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
 void launch_zstats(int64_t N, int B, const cplx_t<T>* z, int nblk, double* part, const MaskView& mask,
                    hipStream_t s) {
   k_zstats<T><<<dim3(nblk, B * mask.K * kL), 256, 0, s>>>(N, z, part, mask);
@@ -1754,7 +2054,14 @@ void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s) {
   template void launch_gz<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, const T*, const double*,  \
                              const double*, cplx_t<T>*, const MaskView&, hipStream_t);                        \
   template void launch_meanvar_grad<T>(int64_t, int, const T*, const double*, const double*, T*,              \
-                                       const MaskView&, hipStream_t);
+                                       const MaskView&, hipStream_t);                                         \
+  template void launch_gram_x<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, bool, int, double*,   \
+                                 double*, const MaskView&, hipStream_t);                                      \
+  template void launch_zcross<T>(int64_t, int, const cplx_t<T>*, int, double*, const MaskView&, hipStream_t); \
+  template void launch_gv_x<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, const double2*,         \
+                               cplx_t<T>*, cplx_t<T>*, const MaskView&, hipStream_t);                         \
+  template void launch_gz_x<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, const T*,               \
+                               const double2*, cplx_t<T>*, const MaskView&, hipStream_t);
 
 STL_INSTANTIATE(double)
 STL_INSTANTIATE(float)

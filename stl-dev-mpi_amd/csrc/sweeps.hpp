@@ -131,6 +131,29 @@ void launch_gz(int n, int64_t N, int B, const cplx_t<T>* z, const cplx_t<T>* gz_
 template <typename T>
 void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, T* g0,
                          const MaskView& mask, hipStream_t s);
+// Cross statistics of map pairs (the reference's cross plan, two channels,
+// c1 != c2): B pairs, field rows 2b (channel a) and 2b + 1 (channel b) of
+// the field layouts above.
+// Full K x K Gram block G[i][j] = sum_p V_a,i conj(V_b,j) per (b, mask k, l3):
+// part [nblk][B * K * kL][K * K][2] (+ [K * K] second moments).
+template <typename T>
+void launch_gram_x(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, bool moments, int nblk,
+                   double* part, double* part2, const MaskView& mask, hipStream_t s);
+// S1 cross sums (re, im, |.|^2, 0) of rn(z_a) conj(rn(z_b)), rn(v) = v / sqrt(|v| + 1e-8):
+// part [nblk][B * K * kL][4].
+template <typename T>
+void launch_zcross(int64_t N, int B, const cplx_t<T>* z, int nblk, double* part, const MaskView& mask,
+                   hipStream_t s);
+// gV_a = H V_b, gV_b = H^H V_a with H [B * kL][C][K][K] the cotangent of G (per class).
+template <typename T>
+void launch_gv_x(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, const double2* H,
+                 cplx_t<T>* gbranch, cplx_t<T>* gz_part, const MaskView& mask, hipStream_t s);
+// gz of both rows of every pair: gz_part + chain modulus term + S1 cross term with the
+// complex cotangents s1x [B][kL][C].
+template <typename T>
+void launch_gz_x(int n, int64_t N, int B, const cplx_t<T>* z, const cplx_t<T>* gz_part, const T* chain,
+                 const double2* s1x, cplx_t<T>* gz, const MaskView& mask, hipStream_t s);
+
 // Element-wise precision casts (maps in, gradients out).
 template <typename TO, typename TI>
 void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s);

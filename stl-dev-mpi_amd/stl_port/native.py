@@ -190,6 +190,12 @@ class NativeEngine:
     of Re sum(conj(cotangents) * values) with respect to the maps [batch, npix].
     forward_device / vjp_device take and return device arrays (torch tensors or
     raw device addresses) without host copies of the maps.
+
+    cross=True: the maps are pairs [batch, 2, npix] and the plan holds the cross
+    terms of the two channels only (S1, S2, S3, S4 with c1 != c2; S3/S4 in both
+    orders); see forward_cross / vjp_cross (reference native.py:160-162, 431-447).
+    masks (or set_masks): averages over each mask at the coefficient's level;
+    values, variances and cotangents [batch, K, ncoeff].
     """
 
     def __init__(self, nside, j, batch=1, backend="gpu", kernel=5, orientations=4, block_pixels=4096,
@@ -277,7 +283,10 @@ class NativeEngine:
         return (self.batch, self.nmasks, self.ncoeff) if self.nmasks else (self.batch, self.ncoeff)
 
     def _map_shape(self):
-        return (self.batch, self.npix)
+        return (self.batch, 2, self.npix) if self.cross else (self.batch, self.npix)
+
+    def _map_len(self):
+        return self.batch * (2 if self.cross else 1) * self.npix
 
     def info(self):
         with self._lock:
@@ -344,7 +353,7 @@ class NativeEngine:
         with self._lock:
             self._open()
             key = _key(generation, member, objective)
-            addr, size = _device_address(maps, self.batch * self.npix)
+            addr, size = _device_address(maps, self._map_len())
             values = np.empty(self._shape(), dtype=np.complex128)
             variance = np.empty(self._shape() if moments else (self.batch, 0), dtype=np.float64)
             self._current_key = None
@@ -360,7 +369,7 @@ class NativeEngine:
             self._open()
             key = _key(generation, member, objective)
             c = self._cot(cotangents, key)
-            addr, _ = _device_address(out, self.batch * self.npix)
+            addr, _ = _device_address(out, self._map_len())
             if self._lib.stlp_vjp_device(self._handle, _pointer(c.view(np.float64)), c.size, *key, addr):
                 raise _error(self._lib)
             return out
@@ -375,6 +384,19 @@ class NativeEngine:
             if self._lib.stlp_forward_levels(self._handle, _pointer(x), x.size, _pointer(values.view(np.float64))):
                 raise _error(self._lib)
             return values
+
+    def forward_cross(self, a, b, generation=0, member=0, objective="gaussian", moments=False):
+        """Cross statistics S(a, b) of the pairs a, b [batch, npix] (cross=True)."""
+        if not self.cross:
+            raise RuntimeError("forward_cross needs NativeEngine(..., cross=True)")
+        a = np.asarray(a, dtype=np.float64).reshape(self.batch, self.npix)
+        b = np.asarray(b, dtype=np.float64).reshape(self.batch, self.npix)
+        return self.forward(np.stack([a, b], 1), generation, member, objective, moments)
+
+    def vjp_cross(self, cotangents, generation=0, member=0, objective="gaussian"):
+        """(grad_a, grad_b), each [batch, npix], after forward_cross."""
+        g = self.vjp(cotangents, generation, member, objective)
+        return g[:, 0], g[:, 1]
 
     def checkpoint(self):
         with self._lock:
@@ -406,7 +428,7 @@ class NativeEngine:
         (the reference's stl-synth --ensemble, crates/stl-mpi/src/synthesis.rs:362-470, no scatter term)."""
         with self._lock:
             self._open()
-            addr, size = _device_address(x, self.batch * self.npix)
+            addr, size = _device_address(x, self._map_len())
             shape = (self.ncoeff,) if ensemble else self._shape()
             t = np.ascontiguousarray(np.asarray(target, dtype=np.complex128).reshape(shape))
             w = None if weights is None else np.ascontiguousarray(np.asarray(weights, dtype=np.float64).reshape(-1))

@@ -22,14 +22,14 @@ class _Scattering(torch.autograd.Function):
     def forward(ctx, x, engine):
         g = next(_generation)
         xs = x.detach().to(torch.float64).contiguous()
-        values, _ = engine.forward_device(xs.reshape(engine.batch, engine.npix), generation=g)
+        values, _ = engine.forward_device(xs.reshape(engine._map_shape()), generation=g)
         ctx.engine, ctx.generation, ctx.shape = engine, g, x.shape
         return torch.from_numpy(values).to(x.device)
 
     @staticmethod
     def backward(ctx, gy):
         cot = gy.detach().to(torch.complex128).cpu().numpy()
-        out = torch.empty((ctx.engine.batch, ctx.engine.npix), dtype=torch.float64, device=gy.device)
+        out = torch.empty(ctx.engine._map_shape(), dtype=torch.float64, device=gy.device)
         ctx.engine.vjp_device(cot.reshape(ctx.engine._shape()), out, generation=ctx.generation)
         return out.reshape(ctx.shape), None
 
@@ -55,7 +55,7 @@ class StatsFunction(torch.autograd.Function):
     def backward(ctx, g):
         device = g.device
         gh = g.detach().to(torch.float64).cpu().numpy()
-        out = torch.empty((ctx.engine.batch, ctx.engine.npix), dtype=torch.float64, device=device)
+        out = torch.empty(ctx.engine._map_shape(), dtype=torch.float64, device=device)
         ctx.engine.vjp_device(gh[..., 0] + 1j * gh[..., 1], out, generation=ctx.generation)
         return out, None, None
 

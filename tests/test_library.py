@@ -43,7 +43,7 @@ def test_binding_table_covers_the_reference_binding():
     ('{"nside": 16, "j": 3, "orientations": 2}', "orientations"),
     ('{"nside": 16, "j": 3, "backend": "cpu"}', "backend cpu"),
     ('{"nside": 16, "j": 3, "backend": "tpu"}', "backend must be"),
-    ('{"nside": 16, "j": 3, "cross": true}', "cross"),
+    ('{"nside": 16, "j": 3, "spin": 2}', "spin 2"),
     ('{"nside": 16, "j": 3, "precision": "fp16"}', "precision"),
     ('{"nside": 16, "j": 3, "batch": 0}', "batch"),
     ('{"nside": 16, "j": 3, "conv_variant": 5}', "conv_variant"),
