@@ -38,7 +38,8 @@ void stlp_string_free(char* s);
 
 /* New handle from a JSON configuration (lib.rs:273-351): nside (128), j (7),
  * batch (1), cross (false: maps [batch, 12 nside^2]; true: pairs
- * [batch, 2, 12 nside^2] and their cross plan, lib.rs:238-245), kernel (5),
+ * [batch, 2, 12 nside^2] and their cross plan, lib.rs:238-245), families
+ * (all; "S1,S2" ...: the plan of those families, plan.rs:80-100), kernel (5),
  * orientations (4), block_pixels (accepted, unused:
  * the whole sky is resident), backend ("gpu"; "cpu" is refused), device (0),
  * and the keys of this library: precision ("fp64" | "mixed": FP32 bulk
@@ -47,6 +48,15 @@ void stlp_string_free(char* s);
 StlHandle* stlp_new(const char* config);
 /* Frees a handle (lib.rs:353-359); NULL is ignored. */
 void stlp_free(StlHandle* h);
+/* A handle on the level fields and operators of h (no counterpart symbol in
+ * the reference: its GpuEvaluator::share, crates/stl-gpu/src/engine.rs
+ * :1909-1916, several plans and batches on one engine).  JSON: batch (h's;
+ * at most h's maps, pairs counting 2), cross (h's), families ("mean,var,S1"
+ * ...: a plan of those families only; default all).  Its own identity,
+ * masks and sums; the handles take turns on the fields: a stlp_vjp or
+ * stlp_checkpoint needs the last forward on the fields to be its own handle's.
+ * Either handle may be freed first.  One caller thread for all of them. */
+StlHandle* stlx_share(StlHandle* h, const char* config);
 /* JSON description: version, sizes, plan, dtype, device memory, timers
  * (lib.rs:361-391).  Free with stlp_string_free; NULL on failure. */
 char* stlp_info(StlHandle* h);

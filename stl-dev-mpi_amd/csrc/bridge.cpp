@@ -167,6 +167,28 @@ struct StlEnsemble {
   bool forwarded = false;
 };
 
+// "families": comma-separated family names (mean, var, S1 .. S4); absent: all.
+static std::vector<int> parse_families(const Json& v) {
+  std::vector<int> out;
+  if (!v.has("families")) return out;
+  std::string text = v.string("families", "");
+  size_t at = 0;
+  while (at <= text.size()) {
+    size_t comma = text.find(',', at);
+    std::string name = text.substr(at, comma == std::string::npos ? std::string::npos : comma - at);
+    name.erase(0, name.find_first_not_of(' '));
+    name.erase(name.find_last_not_of(' ') + 1);
+    int f = -1;
+    for (int k = 0; k < 6; ++k)
+      if (name == family_name(k)) f = k;
+    if (f < 0) fail("unknown family \"" + name + "\"");
+    out.push_back(f);
+    if (comma == std::string::npos) break;
+    at = comma + 1;
+  }
+  return out;
+}
+
 extern "C" {
 
 const char* stlp_last_error(void) { return g_error.c_str(); }
@@ -198,6 +220,7 @@ StlHandle* stlp_new(const char* config) {
     opt.J = int(j);
     opt.batch = int(batch);
     opt.cross = v.boolean("cross", false);
+    opt.families = parse_families(v);
     opt.fp64 = precision == "fp64";
     opt.device = int(v.integer("device", 0));
     opt.timing = v.boolean("timing", false);
@@ -217,6 +240,26 @@ StlHandle* stlp_new(const char* config) {
 }
 
 void stlp_free(StlHandle* h) { delete h; }
+
+StlHandle* stlx_share(StlHandle* h, const char* config) {
+  return guard<StlHandle*>(nullptr, [&]() -> StlHandle* {
+    if (!h) fail("null handle");
+    Json v = Json::parse(config ? config : "{}");
+    const long batch = v.integer("batch", h->batch);
+    if (batch <= 0) fail("batch must be positive");
+    const bool cross = v.boolean("cross", h->channels == 2);
+    auto s = std::make_unique<StlHandle>();
+    s->engine = h->engine->share(int(batch), cross, parse_families(v));
+    s->id = g_next_id.fetch_add(1);
+    s->nside = h->nside;
+    s->batch = int(batch);
+    s->channels = cross ? 2 : 1;
+    s->J = h->J;
+    s->npix = h->npix;
+    s->precision = h->precision;
+    return s.release();
+  });
+}
 
 char* stlp_info(StlHandle* h) {
   return guard<char*>(nullptr, [&]() -> char* {

@@ -270,8 +270,23 @@ template <typename T>
 class EngineT final : public Engine {
   using C2 = cplx_t<T>;
 
+  // The level fields (forward state and reverse scratch), shared by the
+  // engines made by share(); holder: the engine whose forward they hold.
+  struct FieldSet {
+    int rows = 0;  // field rows the buffers are sized for
+    uint64_t holder = 0;
+    std::vector<DevArray<T>> maps, M;
+    std::vector<DevArray<C2>> z, br;
+    DevArray<C2> gbr, gzp, gz;
+    DevArray<T> acc[2], mg[2];
+  };
+
  public:
-  explicit EngineT(const EngineOptions& opt) : opt_(opt) {
+  // fields: the field set of the engine this one shares (nullptr: its own).
+  explicit EngineT(const EngineOptions& opt, std::shared_ptr<FieldSet> fields = nullptr)
+      : opt_(opt), f_(std::move(fields)) {
+    static std::atomic<uint64_t> next_id{1};
+    id_ = next_id.fetch_add(1);
     auto t0 = std::chrono::steady_clock::now();
     STL_CHECK(opt.nside >= 1 && (opt.nside & (opt.nside - 1)) == 0, "nside must be a power of two");
     int order0 = 0;
@@ -284,6 +299,15 @@ class EngineT final : public Engine {
     prof_.enable(opt.timing, s_);
     cross_ = opt.cross;
     plan_ = cross_ ? build_cross_plan(opt.J, kL) : build_plan(opt.J, kL);
+    if (!opt.families.empty()) {
+      // A plan of some families only (the reference's build_plan families,
+      // crates/stl-core/src/plan.rs:80-100), in the full plan's order.
+      std::vector<Coefficient> kept;
+      for (const auto& c : plan_)
+        if (std::find(opt.families.begin(), opt.families.end(), c.family) != opt.families.end()) kept.push_back(c);
+      STL_CHECK(!kept.empty(), "no coefficient of the requested families in the plan");
+      plan_.swap(kept);
+    }
     levels_ = opt.J;
     for (auto& c : plan_) slots_.push_back(cross_ ? cross_slot_of(c) : slot_of(c));
     Bs_ = opt.batch;
@@ -297,6 +321,8 @@ class EngineT final : public Engine {
     geometry_bytes_ = device_bytes_live();
     allocate();
     tune();
+    // The timing launches of tune() wrote the (possibly shared) fields.
+    f_->holder = id_;
     STL_HIP(hipStreamSynchronize(s_));
     setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -305,6 +331,20 @@ class EngineT final : public Engine {
     for (auto* cache : {&fwd_graphs_, &vjp_graphs_})
       for (auto& e : *cache) (void)hipGraphExecDestroy(e.exec);
     if (s_) (void)hipStreamDestroy(s_);
+  }
+
+  std::unique_ptr<Engine> share(int batch, bool cross, const std::vector<int>& families) override {
+    EngineOptions o = opt_;
+    o.batch = batch;
+    o.cross = cross;
+    o.families = families;
+    o.conv_variant = opt_.conv_variant;
+    return std::unique_ptr<Engine>(new EngineT<T>(o, f_));
+  }
+
+  // The reverse and the state copy need this engine's forward in the fields.
+  void check_holder() const {
+    STL_CHECK(f_->holder == id_, "gradient without the forward sweep of this evaluator (shared fields)");
   }
 
   const EngineOptions& options() const override { return opt_; }
@@ -383,12 +423,14 @@ class EngineT final : public Engine {
   void reset_timers() override { prof_.reset(); }
 
   void forward_device(const double* x_dev, bool moments, cplx* values, double* variance) override {
+    f_->holder = id_;
     run(fwd_graphs_, x_dev, moments, [&] { enqueue_forward(x_dev, moments); });
     finish_forward(moments, values, variance);
   }
 
   hipGraph_t capture_forward(const double* x_dev, bool moments) override {
     STL_CHECK(!prof_.on(), "graph capture with per-kernel timing");
+    f_->holder = id_;
     return capture([&] { enqueue_forward(x_dev, moments); });
   }
 
@@ -398,14 +440,16 @@ class EngineT final : public Engine {
 
   hipGraph_t capture_vjp(double* grad_dev) override {
     STL_CHECK(!prof_.on(), "graph capture with per-kernel timing");
+    check_holder();
     return capture([&] { sweep_reverse(grad_dev); });
   }
 
   void forward_levels_device(const double* x_dev, cplx* values) override {
+    f_->holder = id_;
     size_t at = 0;
     for (int n = 0; n < levels_; ++n) {
       int64_t len = int64_t(B_) * N_[n];
-      cast_in(x_dev + at, maps_[n].data(), len);
+      cast_in(x_dev + at, f_->maps[n].data(), len);
       at += size_t(len);
     }
     sweep_forward(false);
@@ -415,6 +459,7 @@ class EngineT final : public Engine {
   }
 
   void vjp_device(const cplx* cot, double* grad_dev) override {
+    check_holder();
     stage_cotangent(cot);
     run(vjp_graphs_, grad_dev, false, [&] { sweep_reverse(grad_dev); });
   }
@@ -445,34 +490,34 @@ class EngineT final : public Engine {
       const MaskView mv = mask_view(n);
       const double2* H = nmask_ > 0 ? Hc_[n].data() : dH_[n].data();
       int id = prof_.start();
-      if (cross_) launch_gv_x<T>(n, N, Bs_, br_[n].data(), z_[n].data(), H, gbr_.data(), gzp_.data(), mv, s_);
-      else launch_gv<T>(n, N, B_, br_[n].data(), z_[n].data(), H, gbr_.data(), gzp_.data(), mv, s_);
+      if (cross_) launch_gv_x<T>(n, N, Bs_, f_->br[n].data(), f_->z[n].data(), H, f_->gbr.data(), f_->gzp.data(), mv, s_);
+      else launch_gv<T>(n, N, B_, f_->br[n].data(), f_->z[n].data(), H, f_->gbr.data(), f_->gzp.data(), mv, s_);
       prof_.stop(id, tag("gv", n), double((n + 2) * B_ * kL) * N * kL * 2 * sizeof(T) * 2);
 
-      T* chain = acc_[n % 2].data();
-      T* chain_prev = acc_[(n + 1) % 2].data();
-      conv_adj(n, kL, gbr_.data(), (n + 1) * B_, chain);
+      T* chain = f_->acc[n % 2].data();
+      T* chain_prev = f_->acc[(n + 1) % 2].data();
+      conv_adj(n, kL, f_->gbr.data(), (n + 1) * B_, chain);
       if (n + 1 < levels_) dec_adj(n, kL, chain_prev, chain, (n + 1) * B_, chain);
 
       id = prof_.start();
       const double* s1 = nmask_ > 0 ? s1c_[n].data() : ds1_[n].data();
       const double* s2 = nmask_ > 0 ? s2c_[n].data() : ds2_[n].data();
       if (cross_)
-        launch_gz_x<T>(n, N, Bs_, z_[n].data(), gzp_.data(), chain, reinterpret_cast<const double2*>(s1), gz_.data(),
+        launch_gz_x<T>(n, N, Bs_, f_->z[n].data(), f_->gzp.data(), chain, reinterpret_cast<const double2*>(s1), f_->gz.data(),
                        mv, s_);
       else
-        launch_gz<T>(n, N, B_, z_[n].data(), gzp_.data(), chain, s1, s2, gz_.data(), mv, s_);
+        launch_gz<T>(n, N, B_, f_->z[n].data(), f_->gzp.data(), chain, s1, s2, f_->gz.data(), mv, s_);
       prof_.stop(id, tag("gz", n), double(B_) * kL * N * (6 * sizeof(T)));
 
-      T* mg = mg_[n % 2].data();
-      T* mg_prev = mg_[(n + 1) % 2].data();
-      conv_adj(n, 1, gz_.data(), B_, mg);
+      T* mg = f_->mg[n % 2].data();
+      T* mg_prev = f_->mg[(n + 1) % 2].data();
+      conv_adj(n, 1, f_->gz.data(), B_, mg);
       if (n + 1 < levels_) dec_adj(n, 1, mg_prev, mg, B_, mg);
     }
     if (!cross_)
-      launch_meanvar_grad<T>(N_[0], B_, maps_[0].data(), nmask_ > 0 ? gmc_.data() : dgm_.data(),
-                             nmask_ > 0 ? gvc_.data() : dgv_.data(), mg_[0].data(), mask_view(0), s_);
-    cast_out(mg_[0].data(), grad_dev, int64_t(B_) * N_[0]);
+      launch_meanvar_grad<T>(N_[0], B_, f_->maps[0].data(), nmask_ > 0 ? gmc_.data() : dgm_.data(),
+                             nmask_ > 0 ? gvc_.data() : dgv_.data(), f_->mg[0].data(), mask_view(0), s_);
+    cast_out(f_->mg[0].data(), grad_dev, int64_t(B_) * N_[0]);
   }
 
   void apply_operator(int kind, int n, int rows, const double* in, double* out) override {
@@ -506,6 +551,7 @@ class EngineT final : public Engine {
   }
 
   std::vector<char> save_state() const override {
+    check_holder();
     std::vector<char> out;
     auto put = [&](const void* d, size_t bytes) {
       size_t at = out.size();
@@ -513,15 +559,16 @@ class EngineT final : public Engine {
       STL_HIP(hipMemcpy(out.data() + at, d, bytes, hipMemcpyDeviceToHost));
     };
     for (int n = 0; n < levels_; ++n) {
-      put(maps_[n].data(), maps_[n].bytes());
-      put(z_[n].data(), z_[n].bytes());
-      put(M_[n].data(), M_[n].bytes());
-      put(br_[n].data(), br_[n].bytes());
+      put(f_->maps[n].data(), f_->maps[n].bytes());
+      put(f_->z[n].data(), f_->z[n].bytes());
+      put(f_->M[n].data(), f_->M[n].bytes());
+      put(f_->br[n].data(), f_->br[n].bytes());
     }
     return out;
   }
 
   void load_state(const std::vector<char>& s) override {
+    f_->holder = id_;
     size_t at = 0;
     auto get = [&](void* d, size_t bytes) {
       STL_CHECK(at + bytes <= s.size(), "checkpoint size mismatch");
@@ -529,38 +576,46 @@ class EngineT final : public Engine {
       at += bytes;
     };
     for (int n = 0; n < levels_; ++n) {
-      get(maps_[n].data(), maps_[n].bytes());
-      get(z_[n].data(), z_[n].bytes());
-      get(M_[n].data(), M_[n].bytes());
-      get(br_[n].data(), br_[n].bytes());
+      get(f_->maps[n].data(), f_->maps[n].bytes());
+      get(f_->z[n].data(), f_->z[n].bytes());
+      get(f_->M[n].data(), f_->M[n].bytes());
+      get(f_->br[n].data(), f_->br[n].bytes());
     }
     STL_CHECK(at == s.size(), "checkpoint size mismatch");
   }
 
  private:
   void allocate() {
-    maps_.resize(levels_);
-    z_.resize(levels_);
-    M_.resize(levels_);
-    br_.resize(levels_);
-    for (int n = 0; n < levels_; ++n) {
-      int64_t N = N_[n];
-      maps_[n].resize(size_t(B_) * N);
-      z_[n].resize(size_t(B_) * kL * N);
-      M_[n].resize(size_t(n + 1) * B_ * N * kL);
-      br_[n].resize(size_t(n + 1) * B_ * kL * N * kL);
-    }
-    gbr_.resize(br_[0].size());
-    for (int n = 1; n < levels_; ++n) gbr_.resize(std::max(gbr_.size(), br_[n].size()));
-    gzp_.resize(z_[0].size());
-    gz_.resize(z_[0].size());
-    acc_[0].resize(M_[0].size());
-    acc_[1].resize(std::max(M_[0].size(), levels_ > 1 ? M_[1].size() : size_t(0)));
-    mg_[0].resize(maps_[0].size());
-    mg_[1].resize(maps_[0].size());
     mC_.assign(levels_, 1);
     mcls_.resize(levels_);
     mpat_.resize(levels_);
+    if (f_) {
+      STL_CHECK(f_->rows >= B_ && int(f_->maps.size()) == levels_,
+                "share: more maps than the shared fields hold (or another pyramid)");
+      alloc_sums();
+      return;
+    }
+    f_ = std::make_shared<FieldSet>();
+    f_->rows = B_;
+    f_->maps.resize(levels_);
+    f_->z.resize(levels_);
+    f_->M.resize(levels_);
+    f_->br.resize(levels_);
+    for (int n = 0; n < levels_; ++n) {
+      int64_t N = N_[n];
+      f_->maps[n].resize(size_t(B_) * N);
+      f_->z[n].resize(size_t(B_) * kL * N);
+      f_->M[n].resize(size_t(n + 1) * B_ * N * kL);
+      f_->br[n].resize(size_t(n + 1) * B_ * kL * N * kL);
+    }
+    f_->gbr.resize(f_->br[0].size());
+    for (int n = 1; n < levels_; ++n) f_->gbr.resize(std::max(f_->gbr.size(), f_->br[n].size()));
+    f_->gzp.resize(f_->z[0].size());
+    f_->gz.resize(f_->z[0].size());
+    f_->acc[0].resize(f_->M[0].size());
+    f_->acc[1].resize(std::max(f_->M[0].size(), levels_ > 1 ? f_->M[1].size() : size_t(0)));
+    f_->mg[0].resize(f_->maps[0].size());
+    f_->mg[1].resize(f_->maps[0].size());
     alloc_sums();
   }
 
@@ -652,11 +707,11 @@ class EngineT final : public Engine {
       launch_cast<double, float>(n, reinterpret_cast<const float*>(g), out, s_);
   }
 
-  void load_maps(const double* x_dev) { cast_in(x_dev, maps_[0].data(), int64_t(B_) * N_[0]); }
+  void load_maps(const double* x_dev) { cast_in(x_dev, f_->maps[0].data(), int64_t(B_) * N_[0]); }
 
   void dec_maps(int n) {
     int id = prof_.start();
-    launch_dec_fwd<T>(dec_[n - 1]->store.view(), 1, maps_[n - 1].data(), B_, maps_[n].data(), s_);
+    launch_dec_fwd<T>(dec_[n - 1]->store.view(), 1, f_->maps[n - 1].data(), B_, f_->maps[n].data(), s_);
     prof_.stop(id, tag("dec_fwd", n), dec_bytes(n - 1, 1, B_));
   }
 
@@ -689,16 +744,16 @@ class EngineT final : public Engine {
     for (int n = 0; n < levels_; ++n) {
       const int64_t N = N_[n];
       int id = prof_.start();
-      launch_conv_fwd<T>(conv_[n]->store.view(), 1, maps_[n].data(), B_, z_[n].data(),
-                         M_[n].data() + size_t(n) * B_ * N * kL, s_, conv_var_[n][0]);
+      launch_conv_fwd<T>(conv_[n]->store.view(), 1, f_->maps[n].data(), B_, f_->z[n].data(),
+                         f_->M[n].data() + size_t(n) * B_ * N * kL, s_, conv_var_[n][0]);
       prof_.stop(id, tag("conv_fwd_z", n), conv_bytes(n, 1, B_, false) + double(B_) * N * kL * sizeof(T));
       if (n > 0) {
         id = prof_.start();
-        launch_dec_fwd<T>(dec_[n - 1]->store.view(), kL, M_[n - 1].data(), n * B_, M_[n].data(), s_);
+        launch_dec_fwd<T>(dec_[n - 1]->store.view(), kL, f_->M[n - 1].data(), n * B_, f_->M[n].data(), s_);
         prof_.stop(id, tag("dec_fwd", n), dec_bytes(n - 1, kL, n * B_));
       }
       id = prof_.start();
-      launch_conv_fwd<T>(conv_[n]->store.view(), kL, M_[n].data(), (n + 1) * B_, br_[n].data(), nullptr, s_,
+      launch_conv_fwd<T>(conv_[n]->store.view(), kL, f_->M[n].data(), (n + 1) * B_, f_->br[n].data(), nullptr, s_,
                          conv_var_[n][1]);
       prof_.stop(id, tag("conv_fwd_branch", n), conv_bytes(n, kL, (n + 1) * B_, false));
       id = prof_.start();
@@ -708,20 +763,20 @@ class EngineT final : public Engine {
       const int K = gram_k(n);
       int64_t m = rows * kL * (cross_ ? K * K : gram_e(n));
       if (cross_)
-        launch_gram_x<T>(n, N, Bs_, br_[n].data(), z_[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(),
+        launch_gram_x<T>(n, N, Bs_, f_->br[n].data(), f_->z[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(),
                          mv, s_);
       else
-        launch_gram<T>(n, N, B_, br_[n].data(), z_[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(), mv,
+        launch_gram<T>(n, N, B_, f_->br[n].data(), f_->z[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(), mv,
                        s_);
       launch_reduce_parts(gpart_[n].data(), nb, m * 2, gsum_[n].data(), s_);
       if (moments) launch_reduce_parts(gpart2_[n].data(), nb, m, gsum2_[n].data(), s_);
       prof_.stop(id, tag("gram", n), double(rows) * kL * N * (gram_k(n)) * 2 * sizeof(T));
       id = prof_.start();
-      if (cross_) launch_zcross<T>(N, Bs_, z_[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
-      else launch_zstats<T>(N, B_, z_[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
+      if (cross_) launch_zcross<T>(N, Bs_, f_->z[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
+      else launch_zstats<T>(N, B_, f_->z[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
       launch_reduce_parts(zpart_[n].data(), nblk_z_[n], rows * kL * 4, zsum_[n].data(), s_);
       if (n == 0 && !cross_) {
-        launch_meanvar<T>(N, B_, maps_[0].data(), nblk_mv_, mvpart_.data(), mv, s_);
+        launch_meanvar<T>(N, B_, f_->maps[0].data(), nblk_mv_, mvpart_.data(), mv, s_);
         launch_reduce_parts(mvpart_.data(), nblk_mv_, rows * 4, mvsum_.data(), s_);
       }
       prof_.stop(id, tag("zstats", n), double(rows) * kL * N * 2 * sizeof(T));
@@ -830,13 +885,13 @@ class EngineT final : public Engine {
         }
         auto timed = [&](int v) {
           if (!adj) {
-            T* x = NI == 1 ? maps_[n].data() : M_[n].data();
-            C2* y = NI == 1 ? z_[n].data() : br_[n].data();
-            T* mod = NI == 1 ? M_[n].data() + size_t(n) * B_ * N_[n] * kL : nullptr;
+            T* x = NI == 1 ? f_->maps[n].data() : f_->M[n].data();
+            C2* y = NI == 1 ? f_->z[n].data() : f_->br[n].data();
+            T* mod = NI == 1 ? f_->M[n].data() + size_t(n) * B_ * N_[n] * kL : nullptr;
             return time_launch([&] { launch_conv_fwd<T>(op, NI, x, rows, y, mod, s_, v); });
           }
-          const C2* gy = NI == 1 ? gz_.data() : gbr_.data();
-          T* gx = NI == 1 ? mg_[0].data() : acc_[0].data();
+          const C2* gy = NI == 1 ? f_->gz.data() : f_->gbr.data();
+          T* gx = NI == 1 ? f_->mg[0].data() : f_->acc[0].data();
           return time_launch([&] { launch_conv_adj<T>(op, NI, gy, rows, gx, s_, v); });
         };
         // Two interleaved rounds, each candidate's best: no bias from the
@@ -993,17 +1048,14 @@ class EngineT final : public Engine {
   std::vector<std::shared_ptr<const SharedDec<T>>> dec_;
   size_t geometry_bytes_ = 0;
   double setup_s_ = 0.0;
-  // forward state
-  std::vector<DevArray<T>> maps_, M_;
-  std::vector<DevArray<C2>> z_, br_;
+  std::shared_ptr<FieldSet> f_;
+  uint64_t id_ = 0;
   // reductions
   std::vector<int> nblk_gram_, nblk_z_;
   int nblk_mv_ = 1;
   std::vector<DevArray<double>> gpart_, gpart2_, gsum_, gsum2_, zpart_, zsum_;
   DevArray<double> mvpart_, mvsum_;
   // reverse
-  DevArray<C2> gbr_, gzp_, gz_;
-  DevArray<T> acc_[2], mg_[2];
   std::vector<DevArray<double2>> dH_;
   std::vector<DevArray<double>> ds1_, ds2_;
   DevArray<double> dgm_, dgv_;

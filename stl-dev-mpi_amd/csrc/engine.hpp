@@ -37,6 +37,7 @@ struct EngineOptions {
   int J = 7;
   int batch = 1;
   bool cross = false;    // map pairs [batch][2][12 nside^2] and the cross plan
+  std::vector<int> families;  // plan of these families only (empty: all)
   bool fp64 = true;      // false: FP32 bulk fields and operators, FP64 sums
   int device = 0;
   bool timing = false;   // per-kernel HIP-event timers
@@ -66,6 +67,13 @@ class Engine {
   virtual ~Engine() = default;
   static std::unique_ptr<Engine> create(const EngineOptions& opt);
 
+  // Another engine on this one's level fields and operators (the reference's
+  // GpuEvaluator::share, crates/stl-gpu/src/engine.rs:1909-1916): its own
+  // batch (at most this engine's field rows), cross flag and families, no
+  // device memory beyond its sums.  Engines sharing fields take turns: a
+  // VJP (or state copy) needs the last forward in the fields to be its own.
+  // One caller thread for all of them.
+  virtual std::unique_ptr<Engine> share(int batch, bool cross, const std::vector<int>& families) = 0;
   virtual const EngineOptions& options() const = 0;
   virtual const std::vector<Coefficient>& plan() const = 0;
   virtual int levels() const = 0;

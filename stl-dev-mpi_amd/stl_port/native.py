@@ -99,6 +99,7 @@ SIGNATURES = {
     "stlx_ensemble_ncoef": ([_VOID, _SIZE], C.c_int),
     "stlx_ensemble_forward_device": ([_VOID, _UINT64, C.c_size_t, C.c_bool, _DOUBLE, _DOUBLE], C.c_int),
     "stlx_ensemble_vjp_device": ([_VOID, _DOUBLE, C.c_size_t, _UINT64, C.c_size_t], C.c_int),
+    "stlx_share": ([_VOID, C.c_char_p], _VOID),
     "stlx_map_len": ([C.c_char_p, _SIZE], C.c_int),
     "stlx_read_map": ([C.c_char_p, _DOUBLE, C.c_size_t], C.c_int),
     "stlx_read_map_range": ([C.c_char_p, C.c_size_t, C.c_size_t, _DOUBLE], C.c_int),
@@ -201,7 +202,7 @@ class NativeEngine:
     def __init__(self, nside, j, batch=1, backend="gpu", kernel=5, orientations=4, block_pixels=4096,
                  cache_bytes=1 << 30, threads=1, cache="transposition", device=0, library=None, cross=False,
                  masks=None, spin=0, spin_kernel=5, spin_pyramid="scalar", spin_a=2, precision="fp64",
-                 timing=False, conv_variant=None):
+                 timing=False, conv_variant=None, families=None):
         self._lib = _library(library)
         self.library_sha256 = self._lib._stlp_binary_sha256
         self._handle = None
@@ -228,16 +229,48 @@ class NativeEngine:
                            cross=self.cross, spin=self.spin, precision=precision, timing=bool(timing))
         if conv_variant is not None:
             self.config["conv_variant"] = int(conv_variant)
+        if families is not None:
+            self.config["families"] = ",".join(families)
         self._handle = self._lib.stlp_new(json.dumps(self.config).encode())
         if not self._handle:
             raise _error(self._lib)
+        self._describe()
+        if masks is not None:
+            self.set_masks(masks)
+
+    def _describe(self):
         self.plan = self.info()["plan"]
         self.plan_digest = hashlib.sha256(json.dumps(self.plan, sort_keys=True).encode()).hexdigest()
         self.ncoeff = len(self.plan)
         self._current_key = None
         self.nmasks = 0
-        if masks is not None:
-            self.set_masks(masks)
+
+    def share(self, batch=None, cross=None, families=None):
+        """Another engine on this one's level fields and operators (the reference's
+        GpuEvaluator::share, crates/stl-gpu/src/engine.rs:1909-1916): its own batch (at most
+        this engine's maps), cross flag, plan families and masks, no field memory of its own.
+        The engines take turns: a vjp needs the last forward on the fields to be its own."""
+        with self._lock:
+            self._open()
+            cfg = {}
+            if batch is not None:
+                cfg["batch"] = int(batch)
+            if cross is not None:
+                cfg["cross"] = bool(cross)
+            if families is not None:
+                cfg["families"] = ",".join(families)
+            h = self._lib.stlx_share(self._handle, json.dumps(cfg).encode())
+            if not h:
+                raise _error(self._lib)
+            other = object.__new__(NativeEngine)
+            other.__dict__.update({k: v for k, v in self.__dict__.items() if k not in ("_handle", "_lock")})
+            other._lock = threading.RLock()
+            other._handle = h
+            other.batch = int(batch) if batch is not None else self.batch
+            other.cross = bool(cross) if cross is not None else self.cross
+            other.config = dict(self.config, batch=other.batch, cross=other.cross)
+            other._describe()
+            return other
 
     def _open(self):
         if not self._handle:
