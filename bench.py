@@ -228,6 +228,25 @@ def cpu_baseline(seconds_budget=30.0):
                       f"scaled to nside 1024 by the pixel ratio {scale}"}
 
 
+def copy_bandwidth(gib=2, reps=5):
+    """Measured HBM bandwidth of a device-to-device copy (bytes read + written / time)."""
+    import torch
+    n = (gib << 30) // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda:0").uniform_()
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return round(2.0 * n * 4 / (ms * 1e-3) / 1e9, 1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +257,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-2048", action="store_true", help="skip the nside 2048 (J=10) single-map rate")
     ap.add_argument("--synthesis-iters", type=int, default=20,
                     help="L-BFGS iterations of the synthesis measurement at the bench nside (0: skip)")
     args = ap.parse_args()
@@ -269,6 +289,18 @@ def main():
     synthesis = run_synthesis(args.nside, args.J, args.synthesis_iters) if args.synthesis_iters > 0 else None
     if synthesis:
         print(f"[bench] synthesis: {synthesis['ms_per_evaluation']} ms/evaluation", file=sys.stderr, flush=True)
+    big = None
+    if not args.no_2048:
+        # The largest single-map configuration: nside 2048, J = 10, mixed precision.
+        x2, t2 = lognormal_map(2048, 3), lognormal_map(2048, 4)
+        r2 = run_engine("mixed", 2048, 10, x2, t2, 5, 1, timing=False)
+        big = {"nside": 2048, "J": 10, "precision": "mixed", "evals_per_s": round(5 / r2["elapsed"], 3),
+               "ms_per_eval": round(1e3 * r2["elapsed"] / 5, 3), "setup_s": round(r2["setup_s"], 3),
+               "device_bytes": r2["device_bytes"], "coefficients": len(r2["plan"])}
+        print(f"[bench] nside 2048: {big['evals_per_s']} evals/s", file=sys.stderr, flush=True)
+        del x2, t2, r2
+    copy_gbps = copy_bandwidth()
+    print(f"[bench] copy bandwidth: {copy_gbps} GB/s", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
 
     value = args.steps / mixed["elapsed"]
@@ -296,7 +328,10 @@ def main():
         "ms_per_step_with_kernel_events": round(1e3 * mixed_t["elapsed"] / tsteps, 3),
         "ensemble": ensemble,
         "synthesis": synthesis,
+        "nside2048": big,
     }
+    line["roofline"]["copy_GBps_measured"] = copy_gbps
+    line["roofline"]["frac_of_measured_copy"] = round(line["roofline"]["achieved"] / copy_gbps, 4)
     if fp64 is not None:
         v64 = args.steps / fp64["elapsed"]
         line["fp64"] = {"value": round(v64, 3), "ms_per_step": round(1e3 * fp64["elapsed"] / args.steps, 3),
