@@ -648,7 +648,7 @@ class EngineT final : public Engine {
       gsum2_[n].resize(rows * kL * E);
       int nz = int(std::min<int64_t>((N + 1023) / 1024, 1024));  // ~4 pixels per thread
       nblk_z_.push_back(nz);
-      zpart_[n].resize(size_t(nz) * rows * kL * 4);
+      zpart_[n].resize(size_t(std::max(nz, nb)) * rows * kL * 4);
       zsum_[n].resize(rows * kL * 4);
       dH_[n].resize(rows * kL * K * K);
       ds1_[n].resize(rows * kL * (cross_ ? 2 : 1));  // cross: complex S1 cotangents
@@ -766,15 +766,16 @@ class EngineT final : public Engine {
         launch_gram_x<T>(n, N, Bs_, f_->br[n].data(), f_->z[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(),
                          mv, s_);
       else
-        launch_gram<T>(n, N, B_, f_->br[n].data(), f_->z[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(), mv,
-                       s_);
+        launch_gram<T>(n, N, B_, f_->br[n].data(), f_->z[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(),
+                       zpart_[n].data(), mv, s_);
       launch_reduce_parts(gpart_[n].data(), nb, m * 2, gsum_[n].data(), s_);
       if (moments) launch_reduce_parts(gpart2_[n].data(), nb, m, gsum2_[n].data(), s_);
       prof_.stop(id, tag("gram", n), double(rows) * kL * N * (gram_k(n)) * 2 * sizeof(T));
       id = prof_.start();
+      // The S1/S2 sums came with the Gram pass (one partial per Gram block);
+      // the cross S1 sums take their own pass.
       if (cross_) launch_zcross<T>(N, Bs_, f_->z[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
-      else launch_zstats<T>(N, B_, f_->z[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
-      launch_reduce_parts(zpart_[n].data(), nblk_z_[n], rows * kL * 4, zsum_[n].data(), s_);
+      launch_reduce_parts(zpart_[n].data(), cross_ ? nblk_z_[n] : nb, rows * kL * 4, zsum_[n].data(), s_);
       if (n == 0 && !cross_) {
         launch_meanvar<T>(N, B_, f_->maps[0].data(), nblk_mv_, mvpart_.data(), mv, s_);
         launch_reduce_parts(mvpart_.data(), nblk_mv_, rows * 4, mvsum_.data(), s_);

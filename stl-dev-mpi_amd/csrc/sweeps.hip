@@ -1023,6 +1023,19 @@ __device__ inline double2 to_d2(cplx_t<T> v) {
   return make_double2(double(v.x), double(v.y));
 }
 
+// Fixed-order block tree of 256 values (4 per thread).
+__device__ inline void block_sum4(double v[4], double* sh) {
+  int tid = threadIdx.x;
+  for (int c = 0; c < 4; ++c) sh[c * 256 + tid] = v[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w)
+      for (int c = 0; c < 4; ++c) sh[c * 256 + tid] += sh[c * 256 + tid + w];
+    __syncthreads();
+  }
+  for (int c = 0; c < 4; ++c) v[c] = sh[c * 256];
+}
+
 __device__ inline bool in_mask(const MaskView& m, int64_t p, int k) {
   return m.cls == nullptr || ((m.pat[m.cls[p]] >> k) & 1ull) != 0;
 }
@@ -1037,10 +1050,12 @@ __device__ inline int mask_class(const MaskView& m, int64_t p) { return m.cls ==
 template <typename T, int KT, bool MOM, bool MASKED>
 __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
                                                   const cplx_t<T>* __restrict__ z, double* __restrict__ part,
-                                                  double* __restrict__ part2, MaskView mask) {
+                                                  double* __restrict__ part2, double* __restrict__ zpart,
+                                                  MaskView mask) {
   constexpr int E = KT * (KT + 1) / 2;
   constexpr int ND = (KT - 1) / kL;  // branch rows d = 0..n
   __shared__ double red[4][MOM ? 3 : 2][E];
+  __shared__ double zred[4][4];
   const int bl = blockIdx.y;  // (b K + k) kL + l3
   const int l3 = bl % kL, mk = (bl / kL) % mask.K, b = bl / (kL * mask.K);
   const int tid = threadIdx.x;
@@ -1050,6 +1065,8 @@ __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const
     are[e] = aim[e] = 0.0;
     if (MOM) asq[e] = 0.0;
   }
+  // The S1/S2 sums of z (sum |z|, |z|^2, re^2 + im^2, |z|^4) ride along.
+  double zs[4] = {0.0, 0.0, 0.0, 0.0};
   const cplx_t<T>* zr = z + (size_t(b) * kL + l3) * N;
   for (int64_t p = blockIdx.x * int64_t(256) + tid; p < N; p += int64_t(gridDim.x) * 256) {
     if (MASKED && !in_mask(mask, p, mk)) continue;
@@ -1061,6 +1078,13 @@ __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const
       for (int l1 = 0; l1 < kL; ++l1) v[d * kL + l1] = to_d2<T>(src[l1]);
     }
     v[KT - 1] = to_d2<T>(zr[p]);
+    {
+      const double h = hypot(v[KT - 1].x, v[KT - 1].y);
+      zs[0] += h;
+      zs[1] += h * h;
+      zs[2] += v[KT - 1].x * v[KT - 1].x + v[KT - 1].y * v[KT - 1].y;
+      zs[3] += (h * h) * (h * h);
+    }
     int e = 0;
 #pragma unroll
     for (int i = 0; i < KT; ++i)
@@ -1090,6 +1114,13 @@ __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const
       if (MOM) red[w][MOM ? 2 : 1][e] = q;
     }
   }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    double a = zs[c];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) a += __shfl_xor(a, off);
+    if (lane == 0) zred[w][c] = a;
+  }
   __syncthreads();
   const size_t base = (size_t(blockIdx.x) * gridDim.y + bl) * E;
   for (int e = tid; e < E; e += 256) {
@@ -1104,6 +1135,8 @@ __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const
     part[(base + e) * 2 + 1] = c;
     if (MOM) part2[base + e] = q;
   }
+  if (tid < 4)
+    zpart[(size_t(blockIdx.x) * gridDim.y + bl) * 4 + tid] = zred[0][tid] + zred[1][tid] + zred[2][tid] + zred[3][tid];
 }
 
 // Gram products of one (b, l3) per workgroup row.  A tile of V [TPX][KP]
@@ -1115,10 +1148,13 @@ __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const
 template <typename T, bool MOM, int BS, bool MASKED>
 __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
                                               const cplx_t<T>* __restrict__ z, int TPX, double* __restrict__ part,
-                                              double* __restrict__ part2, MaskView mask) {
+                                              double* __restrict__ part2, double* __restrict__ zpart,
+                                              MaskView mask) {
   extern __shared__ double2 smem[];
   const int K = (n + 1) * kL + 1;
   const int KP = (K + BS - 1) / BS * BS, NR = KP / BS, NBLK = NR * (NR + 1) / 2;
+  // The S1/S2 sums of z, accumulated by the threads that stage z.
+  double zs[4] = {0.0, 0.0, 0.0, 0.0};
   const int E = K * (K + 1) / 2;
   double2* Vs = smem;
   const int bl = blockIdx.y;  // (b K + k) kL + l3
@@ -1148,9 +1184,16 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
             !MASKED || in_mask(mask, p0 + p, mk) ? to_d2<T>(src[idx]) : make_double2(0.0, 0.0);
       }
     }
-    const cplx_t<T>* zs = z + (size_t(b) * kL + l3) * N + p0;
-    for (int p = tid; p < tn; p += blockDim.x)
-      Vs[p * KP + K - 1] = !MASKED || in_mask(mask, p0 + p, mk) ? to_d2<T>(zs[p]) : make_double2(0.0, 0.0);
+    const cplx_t<T>* zt = z + (size_t(b) * kL + l3) * N + p0;
+    for (int p = tid; p < tn; p += blockDim.x) {
+      const double2 v = !MASKED || in_mask(mask, p0 + p, mk) ? to_d2<T>(zt[p]) : make_double2(0.0, 0.0);
+      Vs[p * KP + K - 1] = v;
+      const double h = hypot(v.x, v.y);
+      zs[0] += h;
+      zs[1] += h * h;
+      zs[2] += v.x * v.x + v.y * v.y;
+      zs[3] += (h * h) * (h * h);
+    }
     const int pad = KP - K;
     for (int idx = tid; idx < tn * pad; idx += blockDim.x) Vs[(idx / pad) * KP + K + idx % pad] = make_double2(0.0, 0.0);
     __syncthreads();
@@ -1213,20 +1256,12 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
     part[(base + e) * 2 + 1] = fin[k].y;
     if (MOM) part2[base + e] = fin2[k];
   }
+  __syncthreads();
+  block_sum4(zs, reinterpret_cast<double*>(smem));
+  if (tid == 0)
+    for (int c = 0; c < 4; ++c) zpart[(size_t(blockIdx.x) * gridDim.y + bl) * 4 + c] = zs[c];
 }
 
-// Fixed-order block tree of 256 values (4 per thread).
-__device__ inline void block_sum4(double v[4], double* sh) {
-  int tid = threadIdx.x;
-  for (int c = 0; c < 4; ++c) sh[c * 256 + tid] = v[c];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (tid < w)
-      for (int c = 0; c < 4; ++c) sh[c * 256 + tid] += sh[c * 256 + tid + w];
-    __syncthreads();
-  }
-  for (int c = 0; c < 4; ++c) v[c] = sh[c * 256];
-}
 
 // Cross Gram products of a map pair: G[i][j] = sum_p V_a,i conj(V_b,j), the
 // full K x K block between the V of field row 2b (channel a) and of row
@@ -1860,14 +1895,14 @@ int gram_tile(int n) {
 
 template <typename T>
 void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, bool moments, int nblk,
-                 double* part, double* part2, const MaskView& mask, hipStream_t s) {
+                 double* part, double* part2, double* zpart, const MaskView& mask, hipStream_t s) {
   const int K = (n + 1) * kL + 1;
   dim3 grid(nblk, B * mask.K * kL);
   const bool masked = mask.cls != nullptr;
 #define STL_GRAM_REG(KT, MOM)                                                                      \
   {                                                                                                \
-    if (masked) k_gram_reg<T, KT, MOM, true><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2, mask);  \
-    else k_gram_reg<T, KT, MOM, false><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2, mask);        \
+    if (masked) k_gram_reg<T, KT, MOM, true><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2, zpart, mask);  \
+    else k_gram_reg<T, KT, MOM, false><<<grid, 256, 0, s>>>(n, N, B, branch, z, part, part2, zpart, mask);        \
     STL_HIP(hipGetLastError());                                                                    \
     return;                                                                                        \
   }
@@ -1883,10 +1918,11 @@ void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<
   const int tpx = gram_tile(n);
   size_t lds = std::max(size_t(tpx) * KP * sizeof(double2),
                         (256 + 128) * sizeof(double2) + size_t(NBLK) * BS * BS * (sizeof(double2) + sizeof(double)));
+  lds = std::max(lds, size_t(4 * 256) * sizeof(double));  // the S1/S2 block sums
 #define STL_GRAM(MOM, BSZ)                                                                          \
   {                                                                                                 \
-    if (masked) k_gram<T, MOM, BSZ, true><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2, mask);  \
-    else k_gram<T, MOM, BSZ, false><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2, mask);       \
+    if (masked) k_gram<T, MOM, BSZ, true><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2, zpart, mask);  \
+    else k_gram<T, MOM, BSZ, false><<<grid, 256, lds, s>>>(n, N, B, branch, z, tpx, part, part2, zpart, mask);       \
   }
   if (BS == 2) {
     if (moments) STL_GRAM(true, 2) else STL_GRAM(false, 2)
@@ -2046,7 +2082,7 @@ void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s) {
   template void launch_dec_fwd<T>(const DecOp<T>&, int, const T*, int, T*, hipStream_t);                      \
   template void launch_dec_adj<T>(const DecOp<T>&, int, const T*, const T*, int, T*, hipStream_t);            \
   template void launch_gram<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, bool, int, double*,     \
-                               double*, const MaskView&, hipStream_t);                                        \
+                               double*, double*, const MaskView&, hipStream_t);                               \
   template void launch_zstats<T>(int64_t, int, const cplx_t<T>*, int, double*, const MaskView&, hipStream_t); \
   template void launch_meanvar<T>(int64_t, int, const T*, int, double*, const MaskView&, hipStream_t);        \
   template void launch_gv<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, const double2*,           \

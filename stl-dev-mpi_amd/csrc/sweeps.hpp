@@ -68,10 +68,11 @@ int gram_tile(int n);
 // Per-block partial sums of the Gram products V_i conj(V_j), i <= j, of
 // V = [branch(d)[l1][l3] for d <= n, l1 < kL; z[l3]] per (b, mask k, l3)
 // over the pixels of the mask: part [nblk][B * K * kL][E][2] (+ [E] second
-// moments |.|^2 when moments).
+// moments |.|^2 when moments), and in the same pass the S1/S2 sums of z
+// (as launch_zstats) to zpart [nblk][B * K * kL][4].
 template <typename T>
 void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, bool moments, int nblk,
-                 double* part, double* part2, const MaskView& mask, hipStream_t s);
+                 double* part, double* part2, double* zpart, const MaskView& mask, hipStream_t s);
 // S1/S2 sums of z: part [nblk][B * K * kL][4] = (sum |z|, sum |z|^2, sum re^2+im^2, sum |z|^4).
 template <typename T>
 void launch_zstats(int64_t N, int B, const cplx_t<T>* z, int nblk, double* part, const MaskView& mask,
