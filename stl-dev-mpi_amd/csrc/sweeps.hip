@@ -1676,6 +1676,58 @@ __global__ __launch_bounds__(256) void k_gz_x(int n, int64_t N, int Bf, const cp
   gz[at] = make_c<T>(T(gp.x + (h * zc.x) / den + r.x), T(gp.y + (h * zc.y) / den + r.y));
 }
 
+// gV = H V at the small levels (few pixels, large K): a workgroup takes 32
+// pixels, stages their V and the row's H in LDS, and its 8 groups of 32
+// lanes split the K outputs (output i in group i mod 8), so each lane forms
+// K / 8 outputs instead of K.  One mask class (unmasked).
+constexpr int kGvSmallPix = 32;
+template <typename T, int NL>
+__global__ __launch_bounds__(256) void k_gv_small(int64_t N, int B, const cplx_t<T>* __restrict__ br,
+                                                  const cplx_t<T>* __restrict__ z, const double2* __restrict__ H,
+                                                  cplx_t<T>* __restrict__ gbr, cplx_t<T>* __restrict__ gzp) {
+  constexpr int K = NL * kL + 1;
+  __shared__ double2 Hs[K * K];
+  __shared__ double2 Vs[kGvSmallPix * K];
+  const int bl = blockIdx.y;
+  const int b = bl / kL, l3 = bl % kL;
+  const int64_t p0 = int64_t(blockIdx.x) * kGvSmallPix;
+  for (int e = threadIdx.x; e < K * K; e += blockDim.x) Hs[e] = H[size_t(bl) * K * K + e];
+  for (int e = threadIdx.x; e < kGvSmallPix * K; e += blockDim.x) {
+    const int px = e / K, j = e % K;
+    const int64_t p = p0 + px;
+    double2 v = make_double2(0.0, 0.0);
+    if (p < N) {
+      if (j < K - 1) {
+        const int d = j / kL, l1 = j % kL;
+        v = to_d2<T>(br[(((size_t(d) * B + b) * kL + l3) * N + p) * kL + l1]);
+      } else {
+        v = to_d2<T>(z[(size_t(b) * kL + l3) * N + p]);
+      }
+    }
+    Vs[e] = v;
+  }
+  __syncthreads();
+  const int px = threadIdx.x % kGvSmallPix, grp = threadIdx.x / kGvSmallPix;
+  const int64_t p = p0 + px;
+  if (p >= N) return;
+  const double2* V = Vs + px * K;
+  for (int i = grp; i < K; i += 256 / kGvSmallPix) {
+    double re = 0.0, im = 0.0;
+#pragma unroll 8
+    for (int j = 0; j < K; ++j) {
+      const double2 h = Hs[i * K + j], v = V[j];
+      re += h.x * v.x - h.y * v.y;
+      im += h.x * v.y + h.y * v.x;
+    }
+    if (i < K - 1) {
+      const int d = i / kL, l1 = i % kL;
+      gbr[(((size_t(d) * B + b) * kL + l3) * N + p) * kL + l1] = make_c<T>(T(re), T(im));
+    } else {
+      gzp[(size_t(b) * kL + l3) * N + p] = make_c<T>(T(re), T(im));
+    }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_gz(int n, int64_t N, int B, const cplx_t<T>* __restrict__ z,
                                             const cplx_t<T>* __restrict__ gzp, const T* __restrict__ chain,
@@ -2039,6 +2091,19 @@ void launch_gv(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>
                cplx_t<T>* gbranch, cplx_t<T>* gz_part, const MaskView& mask, hipStream_t s) {
   dim3 grid(nblocks(N, 256), B * kL);
   const int K = (n + 1) * kL + 1;
+  if (mask.C == 1 && n >= 3 && N <= 49152) {
+    // Small levels: split the K outputs over lanes (k_gv_small).
+    dim3 sgrid(unsigned((N + kGvSmallPix - 1) / kGvSmallPix), B * kL);
+    switch (n + 1) {
+#define STL_GVS(NL) \
+  case NL: k_gv_small<T, NL><<<sgrid, 256, 0, s>>>(N, B, branch, z, H, gbranch, gz_part); break;
+      STL_GVS(4) STL_GVS(5) STL_GVS(6) STL_GVS(7) STL_GVS(8) STL_GVS(9) STL_GVS(10) STL_GVS(11) STL_GVS(12)
+#undef STL_GVS
+      default: throw std::runtime_error("gv: too many levels");
+    }
+    STL_HIP(hipGetLastError());
+    return;
+  }
   const size_t hbytes = size_t(mask.C) * K * K * sizeof(double2);
   const bool lds = hbytes <= 64 * 1024;
   switch (n + 1) {
