@@ -14,6 +14,8 @@
 //   decimation.rs Gaussian-cone decimation rows, CSR, transpose   (decimation.rs:62-219)
 //   plan.rs       coefficient enumeration                         (plan.rs:80-158)
 //   engine.rs     forward / reverse sweeps, term sums, cotangents (engine.rs:485-806, 1191-1330)
+//   mask.rs       masked averages: the mask pyramid of parents and the
+//                 masked branches of forward / reverse            (mask.rs:14-70, engine.rs:500-724)
 // The sweeps are evaluated on the whole sky, level by level (the reference's
 // blocks and halos only bound its memory: its block rows are exact full-sky
 // rows).  Partial sums follow the reference order: pixels of a block of
@@ -21,7 +23,10 @@
 //
 // Parity status of each part is pinned by the reference fixtures (see the
 // tests under tests/test_oracle_*.py); the neighbourhood order of cdshealpix
-// is not reproduced (only its sets are used by the reference, sorted).
+// is not reproduced (only its sets are used by the reference, sorted).  The
+// masks, which no reference fixture covers, are pinned by identities: the
+// whole-sky mask, partitions of the sky by base faces, closed forms of mean
+// and var over a mask, and central differences (tests/test_oracle_masks.py).
 
 #include <algorithm>
 #include <array>
