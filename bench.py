@@ -197,6 +197,31 @@ def roofline(timers, steps, workload=None):
             "eval_GBps": round(total_bytes / (total_ms * 1e-3) / 1e9, 1)}
 
 
+def compulsory_bytes(nside, J, elem_bytes, L=4):
+    """Compulsory HBM bytes of one forward + VJP, independent of how the operators are stored:
+    fields 33.5 N0 elements, the convolution coefficients (44.1 used offsets x 2L values per
+    row of every level, read forward and reverse) and the decimation weights (30 per coarse row,
+    7.5 per fine row transposed, with a 4-byte index), all at elem_bytes per value.  It gives
+    51.05 GB (FP32) and 101.1 GB (FP64) at nside 1024, J = 9."""
+    N = [12 * (nside >> n) ** 2 for n in range(J)]
+    fields = elem_bytes * 33.5 * N[0]
+    conv = 2 * elem_bytes * sum(N) * 44.1 * 2 * L
+    dec = sum((30 * N[n + 1] + 7.5 * N[n]) * (4 + elem_bytes) for n in range(J - 1))
+    return fields + conv + dec
+
+
+def eval_roofline(nside, J, elem_bytes, evals_per_s, copy_gbps):
+    """Whole-evaluation rate against the compulsory bytes: achieved GB/s and its fractions of
+    the 8 TB/s spec and of the measured device copy."""
+    gb = compulsory_bytes(nside, J, elem_bytes) / 1e9
+    achieved = gb * evals_per_s
+    out = {"compulsory_GB": round(gb, 2), "achieved_GBps": round(achieved, 1),
+           "frac_of_peak": round(achieved / HBM_PEAK_GBPS, 4)}
+    if copy_gbps:
+        out["frac_of_measured_copy"] = round(achieved / copy_gbps, 4)
+    return out
+
+
 def cpu_baseline(seconds_budget=30.0):
     """The FP64 oracle (C++ restatement of the reference CPU engine, geometry
     rebuilt at every application like its per-block rebuild) on the host cores,
@@ -332,11 +357,15 @@ def main():
     }
     line["roofline"]["copy_GBps_measured"] = copy_gbps
     line["roofline"]["frac_of_measured_copy"] = round(line["roofline"]["achieved"] / copy_gbps, 4)
+    line["eval_roofline"] = eval_roofline(args.nside, args.J, 4, value, copy_gbps)
+    if big is not None:
+        big["eval_roofline"] = eval_roofline(2048, 10, 4, big["evals_per_s"], copy_gbps)
     if fp64 is not None:
         v64 = args.steps / fp64["elapsed"]
         line["fp64"] = {"value": round(v64, 3), "ms_per_step": round(1e3 * fp64["elapsed"] / args.steps, 3),
                         "roofline": roofline(fp64_t["timers"], tsteps), "setup_s": round(fp64["setup_s"], 3),
-                        "device_bytes": fp64["device_bytes"]}
+                        "device_bytes": fp64["device_bytes"],
+                        "eval_roofline": eval_roofline(args.nside, args.J, 8, v64, copy_gbps)}
         line["mixed_over_fp64"] = round(value / v64, 3)
         line["mixed_error_vs_fp64"] = family_errors(mixed["values"], fp64["values"], mixed["plan"])
         g64 = fp64["grad"]
