@@ -681,16 +681,22 @@ __device__ inline int lds_index(int wi) {
 }
 
 // GP gauges of the cotangent staged per pass.
-template <typename T, int NI, int TS, int GP>
-__global__ __launch_bounds__(TS * TS) __attribute__((amdgpu_waves_per_eu(tile32_waves<T, TS>())))
+// RW rows (input maps) per workgroup share each slot's offset and
+// coefficient loads (rows a multiple of RW).
+template <int RW>
+constexpr int adj_rows_waves() { return RW == 2 ? 4 : 1; }
+template <typename T, int NI, int TS, int GP, int RW = 1>
+__global__ __launch_bounds__(TS * TS) __attribute__((amdgpu_waves_per_eu(
+    RW == 1 ? tile32_waves<T, TS>() : adj_rows_waves<RW>())))
 void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
                                                            T* __restrict__ gx) {
   // Window of SW x SW positions, planes of SW rows of pitch PW.
   constexpr int SW = TS + 2 * kWin, PW = lds_pitch(TS), NW = SW * SW, SA = SW * PW;
-  // Planar [gg][j][pixel]: consecutive pixels in consecutive banks.
-  __shared__ cplx_t<T> gs[GP * NI * SA];
-  int row;
-  const TileInfo ti = tile_info<TS, true>(op.order, rows, row);
+  // Planar [r][gg][j][pixel]: consecutive pixels in consecutive banks.
+  __shared__ cplx_t<T> gs[RW * GP * NI * SA];
+  int rowg;
+  const TileInfo ti = tile_info<TS, true>(op.order, rows / RW, rowg);
+  const int row = rowg * RW;
   const int64_t N = 12 * op.npf;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
   const int li = threadIdx.x, rl = rep_lane<TS>(ti.m, li);
@@ -711,9 +717,11 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
     spix[i] = idx < NW ? int32_t(staged_pixel(ti.face, ti.m, ti.x0 - kWin + wi % SW, ti.y0 - kWin + wi / SW, op.order))
                        : -2;
   }
-  T acc[NI];
+  T acc[RW][NI];
 #pragma unroll
-  for (int j = 0; j < NI; ++j) acc[j] = T(0);
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) acc[r][j] = T(0);
   for (int pass = 0; pass < kL / GP; ++pass) {
     // A face of odd shift k reads gauge g from coefficient plane g ^ 2: it
     // walks its gauges in that order, so that the 4 faces of a rep tile
@@ -733,11 +741,13 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
         // the same products.
         const bool cj = gauge_conj(ti.k, g0 + gg) != ti.m;
 #pragma unroll
-        for (int j = 0; j < NI; ++j) {
-          cplx_t<T> v = p >= 0 ? gr[(size_t(g0 + gg) * N + p) * NI + j] : make_c<T>(T(0), T(0));
-          if (cj) v.y = -v.y;
-          gs[(gg * NI + j) * SA + pi] = v;
-        }
+        for (int r = 0; r < RW; ++r)
+#pragma unroll
+          for (int j = 0; j < NI; ++j) {
+            cplx_t<T> v = p >= 0 ? gr[((size_t(r) * kL + g0 + gg) * N + p) * NI + j] : make_c<T>(T(0), T(0));
+            if (cj) v.y = -v.y;
+            gs[((r * GP + gg) * NI + j) * SA + pi] = v;
+          }
       }
     }
     __syncthreads();
@@ -749,16 +759,20 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
         const int g = g0 + gg;
         const cplx_t<T> a = plane_load(tc + (size_t(gauge_src(ti.k, g)) * op.Tn + s) * op.R, qoc);
 #pragma unroll
-        for (int j = 0; j < NI; ++j) {
-          cplx_t<T> v = gs[(gg * NI + j) * SA + src];
-          acc[j] = fma(a.y, v.y, fma(a.x, v.x, acc[j]));
-        }
+        for (int r = 0; r < RW; ++r)
+#pragma unroll
+          for (int j = 0; j < NI; ++j) {
+            cplx_t<T> v = gs[((r * GP + gg) * NI + j) * SA + src];
+            acc[r][j] = fma(a.y, v.y, fma(a.x, v.x, acc[r][j]));
+          }
       }
     }
   }
   const int64_t p = tile_out<TS>(ti.face, ti.m, ti.base, op.order) + li;
 #pragma unroll
-  for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + p) * NI + j] = acc[j];
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) gx[(size_t(row + r) * N + p) * NI + j] = acc[r][j];
 }
 
 // Adjoint convolution of FW rotated faces of one rep tile per workgroup.  In
@@ -1782,8 +1796,28 @@ void conv_adj_tiled(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, 
     // 42 rows of pitch 44: one gauge per pass for NI = kL (59 KB FP32,
     // 118 KB FP64), 4 (FP32) or 2 (FP64) gauges for NI = 1.
     constexpr int GP1 = sizeof(T) == 8 ? 2 : 4;
-    if (NI == 1) k_conv_adj_tiled<T, 1, TS, GP1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
-    else k_conv_adj_tiled<T, kL, TS, 1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
+    if (NI == 1) {
+      k_conv_adj_tiled<T, 1, TS, GP1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
+      return;
+    }
+    if constexpr (sizeof(T) == 4) {
+      // Two rows per workgroup (118 KB of LDS, one workgroup per CU). An odd
+      // last row goes to the one-row kernel where the faces hold at least
+      // 256 x 256 pixels; below that the second launch costs more than the
+      // shared loads save.
+      if (rows % 2 == 0 || (rows > 2 && op.npf >= 65536)) {
+        const int even = rows & ~1;
+        const unsigned per_row = grid.x / unsigned(rows);
+        k_conv_adj_tiled<T, kL, TS, 1, 2><<<dim3(per_row * even / 2), TS * TS, 0, s>>>(op, gy, even, gx);
+        if (even < rows) {
+          const size_t N = size_t(12) * op.npf;
+          k_conv_adj_tiled<T, kL, TS, 1><<<dim3(per_row), TS * TS, 0, s>>>(
+              op, gy + size_t(even) * kL * N * kL, 1, gx + size_t(even) * N * kL);
+        }
+        return;
+      }
+    }
+    k_conv_adj_tiled<T, kL, TS, 1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
   } else {
     constexpr int GP4 = sizeof(T) == 8 ? 1 : 2;  // NI = kL: 47 KB of LDS per pass
     if (NI == 1) k_conv_adj_tiled<T, 1, TS, 4><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
