@@ -225,7 +225,7 @@ StlHandle* stlp_new(const char* config) {
     opt.device = int(v.integer("device", 0));
     opt.timing = v.boolean("timing", false);
     opt.conv_variant = int(v.integer("conv_variant", -1));
-    if (opt.conv_variant < -1 || opt.conv_variant > 4) fail("conv_variant must be -1 or 0 to 4");
+    if (opt.conv_variant < -1 || opt.conv_variant > 5) fail("conv_variant must be -1 or 0 to 5");
     auto h = std::make_unique<StlHandle>();
     h->engine = stl::Engine::create(opt);
     h->id = g_next_id.fetch_add(1);

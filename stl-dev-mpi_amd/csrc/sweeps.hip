@@ -1886,9 +1886,26 @@ bool conv_adj_faces(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, 
   return false;
 }
 
+// Multi-face adjoint on 32 x 32 tiles (118 KB of LDS at most, one
+// workgroup per CU): NI = kL: 2 faces, 1 gauge per pass (FP32 only); NI = 1:
+// 4 faces, 1 gauge per pass.
+template <typename T>
+void conv_adj_faces32(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
+  if (NI == 1) {
+    dim3 grid(unsigned(3 * op.npf / (32 * 32) * rows));
+    k_conv_adj_faces<T, 1, 32, 4, 1><<<grid, 32 * 32, 0, s>>>(op, gy, rows, gx);
+    return;
+  }
+  if constexpr (sizeof(T) == 4) {
+    dim3 grid(unsigned(3 * op.npf / (32 * 32) * 2 * rows));
+    k_conv_adj_faces<T, kL, 32, 2, 1><<<grid, 32 * 32, 0, s>>>(op, gy, rows, gx);
+  }
+}
+
 template <typename T>
 bool conv_adj_has_variant(int NI, int variant) {
-  return variant != kConvFaces || NI == 1 || sizeof(T) == 4;
+  if (variant == kConvFaces || variant == kConvFaces32) return NI == 1 || sizeof(T) == 4;
+  return true;
 }
 
 template <typename T>
@@ -1904,6 +1921,7 @@ void conv_adj_gather(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows,
   if (op.tiled) {
     const int TS = op.order >= 4 ? 16 : 8;
     if (variant == kConvTiled32 && op.order >= 5) conv_adj_tiled<T, 32>(op, NI, gy, rows, gx, s);
+    else if (variant == kConvFaces32 && op.order >= 5) conv_adj_faces32<T>(op, NI, gy, rows, gx, s);
     else if (variant == kConvFaces && TS == 16) conv_adj_faces<T, 16>(op, NI, gy, rows, gx, s);
     else if (variant == kConvFaces) conv_adj_faces<T, 8>(op, NI, gy, rows, gx, s);
     else if (TS == 16 && variant != kConvTiled8) conv_adj_tiled<T, 16>(op, NI, gy, rows, gx, s);

@@ -33,9 +33,11 @@ using cplx_t = typename Complex2<T>::type;
 // Kernel variants of the convolutions: 16 x 16 LDS tiles with one face per
 // workgroup (the global gather at levels too small to tile), several rotated
 // faces per workgroup (one coefficient load for all of them), 8 x 8 tiles,
-// 32 x 32 tiles (1024 threads, smaller halo share; nside >= 32), or the
-// global gather with each output's slots spread over 8 lanes (small levels).
-constexpr int kConvTiled = 0, kConvFaces = 1, kConvTiled8 = 2, kConvTiled32 = 3, kConvSplit = 4;
+// 32 x 32 tiles (1024 threads, smaller halo share; nside >= 32), the
+// global gather with each output's slots spread over 8 lanes (small levels),
+// or (adjoint only) several rotated faces per workgroup on 32 x 32 tiles.
+constexpr int kConvTiled = 0, kConvFaces = 1, kConvTiled8 = 2, kConvTiled32 = 3, kConvSplit = 4,
+              kConvFaces32 = 5;
 template <typename T>
 void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* modulus_out,
                      hipStream_t s, int variant = kConvTiled);

@@ -288,14 +288,15 @@ def test_graph_replays_match_eager_launches():
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_forward_conv_variants_agree(precision):
     """The convolution kernel variants (16 x 16 tiles, several faces per
-    workgroup, 8 x 8 tiles, 32 x 32 tiles, slots split over lanes; chosen per
-    level and kernel by timing unless forced) agree to rounding."""
+    workgroup, 8 x 8 tiles, 32 x 32 tiles, slots split over lanes, several
+    faces per workgroup on 32 x 32 tiles for the adjoint; chosen per level and
+    kernel by timing unless forced) agree to rounding."""
     from stl_port import NativeEngine
     nside, J = 64, 5
     rng = np.random.default_rng(31)
     x = rng.standard_normal((1, 12 * nside * nside))
     out = []
-    for v in (0, 1, 2, 3, 4, None):
+    for v in (0, 1, 2, 3, 4, 5, None):
         print(f"[gpu] conv_variant={v} precision={precision}", flush=True)
         with NativeEngine(nside, J, precision=precision, conv_variant=v) as e:
             vals, _ = e.forward(x)
@@ -312,3 +313,5 @@ def test_forward_conv_variants_agree(precision):
     # the split gather serves every level of nside <= 64
     assert all(row == [1, 1, 1, 1 if precision == "mixed" else 0] for row in out[1][2][:4])
     assert all(row == [4, 4, 4, 4] for row in out[4][2])
+    # 32 x 32 multi-face adjoints at nside >= 32 (none for FP64's branch adjoint)
+    assert out[5][2][:2] == [[0, 0, 5, 5 if precision == "mixed" else 0]] * 2

@@ -46,7 +46,7 @@ def test_binding_table_covers_the_reference_binding():
     ('{"nside": 16, "j": 3, "spin": 2}', "spin 2"),
     ('{"nside": 16, "j": 3, "precision": "fp16"}', "precision"),
     ('{"nside": 16, "j": 3, "batch": 0}', "batch"),
-    ('{"nside": 16, "j": 3, "conv_variant": 5}', "conv_variant"),
+    ('{"nside": 16, "j": 3, "conv_variant": 6}', "conv_variant"),
     ('{"nside": 16, "j": 3, "conv_variant": -2}', "conv_variant"),
 ])
 def test_bad_configuration_is_an_error(config, message):
