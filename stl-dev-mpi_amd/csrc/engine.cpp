@@ -869,7 +869,8 @@ class EngineT final : public Engine {
         if (op.tiled && (!adj || conv_adj_has_variant<T>(NI, kConvFaces))) cands.push_back(kConvFaces);
         if (op.tiled && op.order >= 4) cands.push_back(kConvTiled8);
         if (op.tiled && op.order >= 5) cands.push_back(kConvTiled32);
-        if (adj && op.tiled && op.order >= 5 && conv_adj_has_variant<T>(NI, kConvFaces32))
+        if (op.tiled && op.order >= 5 &&
+            (adj ? conv_adj_has_variant<T>(NI, kConvFaces32) : conv_fwd_has_variant<T>(NI, kConvFaces32)))
           cands.push_back(kConvFaces32);
         if (op.order <= 6) cands.push_back(kConvSplit);
         if (opt_.conv_variant >= 0) {

@@ -779,8 +779,12 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
 // pass P every face reads coefficient planes P * GP .. P * GP + GP - 1 (face k
 // takes gauge plane ^ 2 (k & 1) from them, conjugated by its shift), so each
 // plane entry is loaded once for the FW faces.
-template <typename T, int NI, int TS, int FW, int GP>
-__global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
+// Slot loop unrolled 4 times on 32 x 32 tiles (A/B on MI355X: deeper
+// unrolling, or pinning the 4-face z adjoint to 64 VGPRs for two workgroups
+// per CU, was slower), twice on smaller tiles.
+template <typename T, int NI, int TS, int FW, int GP, int kSlotUnroll = TS == 32 ? 4 : 2>
+__global__ __launch_bounds__(TS * TS)
+void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
                                                            int rows, T* __restrict__ gx) {
   // Window of SW x SW positions, planes of SW rows of pitch PW.
   constexpr int SW = TS + 2 * kWin, PW = lds_pitch(TS), NW = SW * SW, SA = SW * PW;
@@ -843,7 +847,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_adj_faces(ConvOp<T> op, const 
       }
     }
     __syncthreads();
-#pragma unroll 2
+#pragma unroll kSlotUnroll
     for (int s = 0; s < cnt; ++s) {
       const int src = pos + tslot_off<TS>(op, s, qo2);
 #pragma unroll
@@ -1838,10 +1842,32 @@ void conv_fwd_faces(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>
   }
 }
 
+// Multi-face forward on 32 x 32 tiles: NI = 1: 4 faces (28 KB of LDS FP32,
+// 56 KB FP64); NI = kL: 2 faces (56 KB, FP32 only: FP64 would need twice the
+// accumulator registers).
+template <typename T>
+void conv_fwd_faces32(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s) {
+  if (NI == 1) {
+    dim3 grid(unsigned(3 * op.npf / (32 * 32) * rows));
+    k_conv_fwd_faces<T, 1, 32, 4><<<grid, 32 * 32, 0, s>>>(op, x, rows, y, mod);
+    return;
+  }
+  if constexpr (sizeof(T) == 4) {
+    dim3 grid(unsigned(3 * op.npf / (32 * 32) * 2 * rows));
+    k_conv_fwd_faces<T, kL, 32, 2><<<grid, 32 * 32, 0, s>>>(op, x, rows, y, nullptr);
+  }
+}
+
+template <typename T>
+bool conv_fwd_has_variant(int NI, int variant) {
+  return variant != kConvFaces32 || NI == 1 || sizeof(T) == 4;
+}
+
 template <typename T>
 void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s,
                      int variant) {
   STL_CHECK(NI == 1 || NI == kL, "conv: unsupported inner width");
+  STL_CHECK(conv_fwd_has_variant<T>(NI, variant), "conv: no such variant");
   if (variant == kConvSplit) {
     constexpr int LP = 8;
     dim3 grid(nblocks(12 * op.npf * LP, 256), rows);
@@ -1853,6 +1879,7 @@ void launch_conv_fwd(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T
   if (op.tiled) {
     const int TS = op.order >= 4 ? 16 : 8;
     if (variant == kConvTiled32 && op.order >= 5) conv_fwd_tiled<T, 32>(op, NI, x, rows, y, mod, s);
+    else if (variant == kConvFaces32 && op.order >= 5) conv_fwd_faces32<T>(op, NI, x, rows, y, mod, s);
     else if (variant == kConvFaces && TS == 16) conv_fwd_faces<T, 16>(op, NI, x, rows, y, mod, s);
     else if (variant == kConvFaces) conv_fwd_faces<T, 8>(op, NI, x, rows, y, mod, s);
     else if (TS == 16 && variant != kConvTiled8) conv_fwd_tiled<T, 16>(op, NI, x, rows, y, mod, s);
@@ -2196,6 +2223,7 @@ void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s) {
   template void launch_conv_fwd<T>(const ConvOp<T>&, int, const T*, int, cplx_t<T>*, T*, hipStream_t, int);        \
   template void launch_conv_adj<T>(const ConvOp<T>&, int, const cplx_t<T>*, int, T*, hipStream_t, int);     \
   template bool conv_adj_has_variant<T>(int, int);            \
+  template bool conv_fwd_has_variant<T>(int, int);            \
   template void launch_dec_fwd<T>(const DecOp<T>&, int, const T*, int, T*, hipStream_t);                      \
   template void launch_dec_adj<T>(const DecOp<T>&, int, const T*, const T*, int, T*, hipStream_t);            \
   template void launch_gram<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, bool, int, double*,     \

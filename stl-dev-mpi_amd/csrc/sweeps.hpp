@@ -35,7 +35,7 @@ using cplx_t = typename Complex2<T>::type;
 // faces per workgroup (one coefficient load for all of them), 8 x 8 tiles,
 // 32 x 32 tiles (1024 threads, smaller halo share; nside >= 32), the
 // global gather with each output's slots spread over 8 lanes (small levels),
-// or (adjoint only) several rotated faces per workgroup on 32 x 32 tiles.
+// or several rotated faces per workgroup on 32 x 32 tiles.
 constexpr int kConvTiled = 0, kConvFaces = 1, kConvTiled8 = 2, kConvTiled32 = 3, kConvSplit = 4,
               kConvFaces32 = 5;
 template <typename T>
@@ -47,6 +47,8 @@ void launch_conv_adj(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows,
 // Whether the adjoint has this variant for the precision and inner width.
 template <typename T>
 bool conv_adj_has_variant(int NI, int variant);
+template <typename T>
+bool conv_fwd_has_variant(int NI, int variant);
 template <typename T>
 void launch_dec_fwd(const DecOp<T>& op, int NI, const T* x, int rows, T* y, hipStream_t s);
 template <typename T>

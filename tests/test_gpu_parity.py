@@ -289,8 +289,8 @@ def test_graph_replays_match_eager_launches():
 def test_forward_conv_variants_agree(precision):
     """The convolution kernel variants (16 x 16 tiles, several faces per
     workgroup, 8 x 8 tiles, 32 x 32 tiles, slots split over lanes, several
-    faces per workgroup on 32 x 32 tiles for the adjoint; chosen per level and
-    kernel by timing unless forced) agree to rounding."""
+    faces per workgroup on 32 x 32 tiles; chosen per level and kernel by
+    timing unless forced) agree to rounding."""
     from stl_port import NativeEngine
     nside, J = 64, 5
     rng = np.random.default_rng(31)
@@ -313,5 +313,5 @@ def test_forward_conv_variants_agree(precision):
     # the split gather serves every level of nside <= 64
     assert all(row == [1, 1, 1, 1 if precision == "mixed" else 0] for row in out[1][2][:4])
     assert all(row == [4, 4, 4, 4] for row in out[4][2])
-    # 32 x 32 multi-face adjoints at nside >= 32 (none for FP64's branch adjoint)
-    assert out[5][2][:2] == [[0, 0, 5, 5 if precision == "mixed" else 0]] * 2
+    # 32 x 32 multi-face kernels at nside >= 32 (none for FP64's branch kernels)
+    assert out[5][2][:2] == [[5, 5, 5, 5] if precision == "mixed" else [5, 0, 5, 0]] * 2
