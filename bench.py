@@ -85,7 +85,8 @@ def run_engine(precision, nside, J, x, t, steps, warmup, timing):
             timers[name] = {k: v[k] - v0[k] for k in ("ms", "launches", "bytes")}
     out = dict(elapsed=elapsed, values=values.copy(), grad=grad.cpu().numpy().ravel(), plan=eng.plan,
                setup_s=info1["setup_seconds"], device_bytes=info1["device_memory"]["total_bytes"],
-               peak_bytes=info1["device_memory"]["peak_bytes"], timers=timers)
+               peak_bytes=info1["device_memory"]["peak_bytes"], timers=timers,
+               conv_variants=info1["conv_variants"])
     eng.close()
     del xd, td, grad
     torch.cuda.empty_cache()
@@ -351,6 +352,8 @@ def main():
         "device_bytes": mixed["device_bytes"],
         "peak_device_bytes": mixed["peak_bytes"],
         "ms_per_step_with_kernel_events": round(1e3 * mixed_t["elapsed"] / tsteps, 3),
+        # per level: forward z, forward branch, adjoint z, adjoint branch kernel variant
+        "conv_variants": mixed["conv_variants"],
         "ensemble": ensemble,
         "synthesis": synthesis,
         "nside2048": big,
