@@ -315,3 +315,32 @@ def test_forward_conv_variants_agree(precision):
     assert all(row == [4, 4, 4, 4] for row in out[4][2])
     # 32 x 32 multi-face kernels at nside >= 32 (none for FP64's branch kernels)
     assert out[5][2][:2] == [[5, 5, 5, 5] if precision == "mixed" else [5, 0, 5, 0]] * 2
+
+
+def test_two_row_adjoint_with_odd_rows_agrees():
+    """The FP32 32 x 32 branch adjoint takes two input rows per workgroup; an odd
+    row count sends the last row to the one-row kernel on faces of >= 256 x 256
+    pixels. Three maps at nside 256 give 3 rows at level 0 and 6 at level 1:
+    forced to that variant, the mixed engine agrees with the 16 x 16 tiles and
+    each map with its own one-map evaluation."""
+    from stl_port import NativeEngine
+    nside, J = 256, 3
+    rng = np.random.default_rng(41)
+    x = rng.standard_normal((3, 12 * nside * nside))
+    out = {}
+    for v in (0, 3):
+        print(f"[gpu] two-row adjoint nside={nside} batch=3 conv_variant={v}", flush=True)
+        with NativeEngine(nside, J, batch=3, precision="mixed", conv_variant=v) as e:
+            assert e.info()["conv_variants"][0][3] == v
+            vals, _ = e.forward(x)
+            cot = np.conj(vals) * 0.5 + 1.0
+            out[v] = (vals, e.vjp(cot))
+            plan = e.plan
+    assert max(family_errors(out[3][0], out[0][0], plan).values()) <= 1e-6
+    assert rel(out[3][1], out[0][1]) <= 1e-5
+    with NativeEngine(nside, J, batch=1, precision="mixed", conv_variant=3) as one:
+        for b in range(3):
+            vb, _ = one.forward(x[b:b + 1])
+            assert max(family_errors(vb, out[3][0][b:b + 1], plan).values()) <= 1e-6
+            gb = one.vjp(np.conj(vb) * 0.5 + 1.0)
+            assert rel(gb, out[3][1][b:b + 1]) <= 1e-5
