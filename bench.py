@@ -98,8 +98,8 @@ ENSEMBLE = [(128, 8), (256, 4), (512, 2)]  # (nside, maps), J = log2(nside) - 1
 
 def run_ensemble(steps, warmup):
     """Forward + VJP of 8 maps at nside 128, 4 at 256 and 2 at 512 (lognormal
-    recipe, per-map seeds), mixed precision: as one ensemble (one HIP graph
-    launch per forward and per VJP over all groups) and, for comparison, as
+    recipe, per-map seeds), mixed precision: as one ensemble (each group's HIP
+    graph on its own stream, the groups concurrent) and, for comparison, as
     three standalone batched engines run one after the other."""
     import torch
     from stl_port import NativeEngine, NativeEnsemble

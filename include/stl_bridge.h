@@ -176,8 +176,8 @@ int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* ta
 /* Ensembles of maps of uneven nside (no counterpart in the reference; its
  * ensemble driver loops over NativeEngine handles): groups g of batch[g] maps
  * at nside[g] with j[g] scales, evaluated together, each forward and each VJP
- * one HIP graph launch over all groups (operators shared between the groups'
- * levels of equal nside).  options: precision ("mixed" | "fp64"), device. */
+ * one HIP graph per group on concurrent streams (operators shared between the
+ * groups' levels of equal nside).  options: precision ("mixed" | "fp64"), device. */
 typedef struct StlEnsemble StlEnsemble;
 StlEnsemble* stlx_ensemble_new(const int32_t* nside, const int32_t* j, const int32_t* batch, size_t groups,
                                const char* options);

@@ -31,14 +31,26 @@ Ensemble::Ensemble(const std::vector<EnsembleGroup>& groups, bool fp64, int devi
     o.timing = false;
     engines_[g] = Engine::create(o);
   }
-  STL_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+  // Default-priority streams, one per group: the groups' graphs run
+  // concurrently (their small levels overlap the others' large kernels),
+  // launched largest group first.
+  streams_.assign(groups.size(), nullptr);
+  order_.resize(groups.size());
+  for (size_t g = 0; g < groups.size(); ++g) order_[g] = g;
+  auto work = [&](size_t g) { return double(groups[g].batch) * double(groups[g].nside) * groups[g].nside; };
+  std::stable_sort(order_.begin(), order_.end(), [&](size_t a, size_t b) { return work(a) > work(b); });
+  for (auto& st : streams_) STL_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 Ensemble::~Ensemble() {
+  for (auto& st : streams_)
+    if (st) (void)hipStreamSynchronize(st);
   for (auto* c : {&fwd_, &vjp_})
-    for (auto& e : *c) (void)hipGraphExecDestroy(e.exec);
-  if (s_) (void)hipStreamDestroy(s_);
+    for (auto& e : *c)
+      for (auto x : e.execs) (void)hipGraphExecDestroy(x);
+  for (auto& st : streams_)
+    if (st) (void)hipStreamDestroy(st);
 }
 
 size_t Ensemble::values() const {
@@ -48,42 +60,49 @@ size_t Ensemble::values() const {
 }
 
 template <typename Capture>
-hipGraphExec_t Ensemble::get(std::vector<Cached>& cache, const std::vector<const void*>& ptrs, bool flag,
-                             Capture&& capture) {
+const std::vector<hipGraphExec_t>& Ensemble::get(std::vector<Cached>& cache, const std::vector<const void*>& ptrs,
+                                                 bool flag, Capture&& capture) {
   for (auto& c : cache)
-    if (c.ptrs == ptrs && c.flag == flag) return c.exec;
-  hipGraph_t graph = nullptr;
-  STL_HIP(hipGraphCreate(&graph, 0));
+    if (c.ptrs == ptrs && c.flag == flag) return c.execs;
+  std::vector<hipGraphExec_t> execs;
   try {
     for (size_t g = 0; g < engines_.size(); ++g) {
-      hipGraph_t child = capture(g);
-      hipGraphNode_t node;
-      hipError_t err = hipGraphAddChildGraphNode(&node, graph, nullptr, 0, child);
-      (void)hipGraphDestroy(child);
+      hipGraph_t graph = capture(g);
+      hipGraphExec_t exec = nullptr;
+      hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
       STL_HIP(err);
+      execs.push_back(exec);
     }
   } catch (...) {
-    (void)hipGraphDestroy(graph);
+    for (auto x : execs) (void)hipGraphExecDestroy(x);
     throw;
   }
-  hipGraphExec_t exec = nullptr;
-  hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  STL_HIP(err);
   if (cache.size() == kMaxGraphs) {
-    (void)hipGraphExecDestroy(cache.front().exec);
+    for (auto x : cache.front().execs) (void)hipGraphExecDestroy(x);
     cache.erase(cache.begin());
   }
-  cache.push_back(Cached{ptrs, flag, exec});
-  return exec;
+  cache.push_back(Cached{ptrs, flag, std::move(execs)});
+  return cache.back().execs;
+}
+
+void Ensemble::launch(const std::vector<hipGraphExec_t>& execs) {
+  hipError_t first = hipSuccess;
+  for (size_t i = 0; i < execs.size() && first == hipSuccess; ++i) {
+    const size_t g = order_[i];
+    first = hipGraphLaunch(execs[g], streams_[g]);
+  }
+  // Join every stream, also after a failed launch, before reporting.
+  for (auto& st : streams_) {
+    hipError_t err = hipStreamSynchronize(st);
+    if (first == hipSuccess) first = err;
+  }
+  STL_HIP(first);
 }
 
 void Ensemble::forward(const double* const* x_dev, bool moments, cplx* values, double* variance) {
   std::vector<const void*> ptrs(x_dev, x_dev + engines_.size());
-  hipGraphExec_t exec =
-      get(fwd_, ptrs, moments, [&](size_t g) { return engines_[g]->capture_forward(x_dev[g], moments); });
-  STL_HIP(hipGraphLaunch(exec, s_));
-  STL_HIP(hipStreamSynchronize(s_));
+  launch(get(fwd_, ptrs, moments, [&](size_t g) { return engines_[g]->capture_forward(x_dev[g], moments); }));
   size_t at = 0;
   for (auto& e : engines_) {
     e->finish_forward(moments, values + at, variance ? variance + at : nullptr);
@@ -98,9 +117,7 @@ void Ensemble::vjp(const cplx* cot, double* const* grad_dev) {
     e->stage_cotangent(cot + at);
     at += size_t(e->options().batch) * e->plan().size();
   }
-  hipGraphExec_t exec = get(vjp_, ptrs, false, [&](size_t g) { return engines_[g]->capture_vjp(grad_dev[g]); });
-  STL_HIP(hipGraphLaunch(exec, s_));
-  STL_HIP(hipStreamSynchronize(s_));
+  launch(get(vjp_, ptrs, false, [&](size_t g) { return engines_[g]->capture_vjp(grad_dev[g]); }));
 }
 
 }  // namespace stl

@@ -34,16 +34,21 @@ class Ensemble {
   double setup_seconds() const { return setup_s_; }
 
  private:
+  // One executable graph per group, keyed by the groups' device pointers.
   struct Cached {
     std::vector<const void*> ptrs;
     bool flag;
-    hipGraphExec_t exec;
+    std::vector<hipGraphExec_t> execs;
   };
   template <typename Capture>
-  hipGraphExec_t get(std::vector<Cached>& cache, const std::vector<const void*>& ptrs, bool flag, Capture&& capture);
+  const std::vector<hipGraphExec_t>& get(std::vector<Cached>& cache, const std::vector<const void*>& ptrs,
+                                         bool flag, Capture&& capture);
+  // Each group's graph on its own stream, then all streams joined.
+  void launch(const std::vector<hipGraphExec_t>& execs);
 
   std::vector<std::unique_ptr<Engine>> engines_;
-  hipStream_t s_ = nullptr;
+  std::vector<hipStream_t> streams_;
+  std::vector<size_t> order_;  // launch order: most pixels first
   std::vector<Cached> fwd_, vjp_;
   double setup_s_ = 0.0;
 };

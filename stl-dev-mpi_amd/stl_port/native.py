@@ -509,7 +509,7 @@ class NativeEngine:
 class NativeEnsemble:
     """Maps of uneven nside evaluated together on one GPU: groups of
     (nside, batch, j) (j defaults to log2(nside) - 1), each forward and each
-    VJP one HIP graph launch over all groups.  forward_device takes one float64
+    VJP one HIP graph per group, the groups on concurrent streams.  forward_device takes one float64
     device tensor [batch, 12 nside^2] per group and returns one complex
     statistics array [batch, ncoeff_g] per group; vjp_device takes the
     cotangents in the same shapes and writes the gradients into `out`."""
