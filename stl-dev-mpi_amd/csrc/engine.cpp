@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -277,8 +278,11 @@ class EngineT final : public Engine {
     uint64_t holder = 0;
     std::vector<DevArray<T>> maps, M;
     std::vector<DevArray<C2>> z, br;
-    DevArray<C2> gbr, gzp, gz;
-    DevArray<T> acc[2], mg[2];
+    // Reverse buffers: gbr / gzp / acc[0] hold level 0, gbr1 / gzp1 and
+    // acc[1 + (n & 1)] the levels n >= 1, so that level 0's gV and branch
+    // adjoint can run beside the coarser levels' reverse.
+    DevArray<C2> gbr, gzp, gz, gbr1, gzp1;
+    DevArray<T> acc[3], mg[2];
   };
 
  public:
@@ -296,6 +300,10 @@ class EngineT final : public Engine {
     STL_CHECK(opt.batch >= 1, "batch must be positive");
     STL_HIP(hipSetDevice(opt.device));
     STL_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    // Default priority, as the engine stream.
+    STL_HIP(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking));
+    STL_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    STL_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     prof_.enable(opt.timing, s_);
     cross_ = opt.cross;
     plan_ = cross_ ? build_cross_plan(opt.J, kL) : build_plan(opt.J, kL);
@@ -329,7 +337,11 @@ class EngineT final : public Engine {
 
   ~EngineT() override {
     for (auto* cache : {&fwd_graphs_, &vjp_graphs_})
-      for (auto& e : *cache) (void)hipGraphExecDestroy(e.exec);
+      for (auto& e : *cache)
+        for (int i = 0; i < e.count; ++i) (void)hipGraphExecDestroy(e.exec[i]);
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (s2_) (void)hipStreamDestroy(s2_);
     if (s_) (void)hipStreamDestroy(s_);
   }
 
@@ -424,7 +436,20 @@ class EngineT final : public Engine {
 
   void forward_device(const double* x_dev, bool moments, cplx* values, double* variance) override {
     f_->holder = id_;
-    run(fwd_graphs_, x_dev, moments, [&] { enqueue_forward(x_dev, moments); });
+    run_split(fwd_graphs_, x_dev, moments,
+              Phases{[&] {
+                       load_maps(x_dev);
+                       for (int n = 1; n < levels_; ++n) dec_maps(n);
+                       fwd_z(0);
+                     },
+                     [&] { fwd_sums(0, moments); },
+                     [&] {
+                       for (int n = 1; n < levels_; ++n) {
+                         fwd_z(n);
+                         fwd_sums(n, moments);
+                       }
+                     },
+                     [&] { fwd_finish(moments); }});
     finish_forward(moments, values, variance);
   }
 
@@ -461,10 +486,36 @@ class EngineT final : public Engine {
   void vjp_device(const cplx* cot, double* grad_dev) override {
     check_holder();
     stage_cotangent(cot);
-    run(vjp_graphs_, grad_dev, false, [&] { sweep_reverse(grad_dev); });
+    run_split(vjp_graphs_, grad_dev, false,
+              Phases{[&] { rev_prelude(); },
+                     [&] { rev_branch(0); },
+                     [&] {
+                       for (int n = levels_ - 1; n >= 1; --n) {
+                         rev_branch(n);
+                         rev_chain(n);
+                       }
+                     },
+                     [&] {
+                       rev_chain(0);
+                       rev_finish(grad_dev);
+                     }});
   }
 
+  // The reverse sweep in phases: the prelude (cotangents onto the sums'
+  // operands), each level's gV and branch adjoint (rev_branch), each level's
+  // chain, z cotangent and z adjoint (rev_chain), then the map gradient.
+  // Level 0's rev_branch touches only level-0 buffers, so it may run beside
+  // levels J - 1 .. 1; rev_chain(0) needs both.
   void sweep_reverse(double* grad_dev) {
+    rev_prelude();
+    for (int n = levels_ - 1; n >= 0; --n) {
+      rev_branch(n);
+      rev_chain(n);
+    }
+    rev_finish(grad_dev);
+  }
+
+  void rev_prelude() {
     const size_t nv = nvals();
     STL_HIP(hipMemcpyAsync(dcot_.data(), hcot_.data(), nv * sizeof(double2), hipMemcpyHostToDevice, s_));
     launch_scatter_cot(nrt_, Bs_, Km_, levels_, int(plan_.size()), drt_.data(), drk_.data(), dnd_.data(),
@@ -485,35 +536,47 @@ class EngineT final : public Engine {
         launch_mask_combine(Bs_, Km_, 1, mC_[0], 1, mpat_[0].data(), dgv_.data(), gvc_.data(), s_);
       }
     }
-    for (int n = levels_ - 1; n >= 0; --n) {
-      const int64_t N = N_[n];
-      const MaskView mv = mask_view(n);
-      const double2* H = nmask_ > 0 ? Hc_[n].data() : dH_[n].data();
-      int id = prof_.start();
-      if (cross_) launch_gv_x<T>(n, N, Bs_, f_->br[n].data(), f_->z[n].data(), H, f_->gbr.data(), f_->gzp.data(), mv, s_);
-      else launch_gv<T>(n, N, B_, f_->br[n].data(), f_->z[n].data(), H, f_->gbr.data(), f_->gzp.data(), mv, s_);
-      prof_.stop(id, tag("gv", n), double((n + 2) * B_ * kL) * N * kL * 2 * sizeof(T) * 2);
+  }
 
-      T* chain = f_->acc[n % 2].data();
-      T* chain_prev = f_->acc[(n + 1) % 2].data();
-      conv_adj(n, kL, f_->gbr.data(), (n + 1) * B_, chain);
-      if (n + 1 < levels_) dec_adj(n, kL, chain_prev, chain, (n + 1) * B_, chain);
+  // Reverse buffers of level n (level 0 apart from the others).
+  C2* gbr_of(int n) { return n == 0 ? f_->gbr.data() : f_->gbr1.data(); }
+  C2* gzp_of(int n) { return n == 0 ? f_->gzp.data() : f_->gzp1.data(); }
+  T* chain_of(int n) { return f_->acc[n == 0 ? 0 : 1 + (n & 1)].data(); }
 
-      id = prof_.start();
-      const double* s1 = nmask_ > 0 ? s1c_[n].data() : ds1_[n].data();
-      const double* s2 = nmask_ > 0 ? s2c_[n].data() : ds2_[n].data();
-      if (cross_)
-        launch_gz_x<T>(n, N, Bs_, f_->z[n].data(), f_->gzp.data(), chain, reinterpret_cast<const double2*>(s1), f_->gz.data(),
-                       mv, s_);
-      else
-        launch_gz<T>(n, N, B_, f_->z[n].data(), f_->gzp.data(), chain, s1, s2, f_->gz.data(), mv, s_);
-      prof_.stop(id, tag("gz", n), double(B_) * kL * N * (6 * sizeof(T)));
+  void rev_branch(int n) {
+    const int64_t N = N_[n];
+    const MaskView mv = mask_view(n);
+    const double2* H = nmask_ > 0 ? Hc_[n].data() : dH_[n].data();
+    int id = prof_.start();
+    if (cross_) launch_gv_x<T>(n, N, Bs_, f_->br[n].data(), f_->z[n].data(), H, gbr_of(n), gzp_of(n), mv, s_);
+    else launch_gv<T>(n, N, B_, f_->br[n].data(), f_->z[n].data(), H, gbr_of(n), gzp_of(n), mv, s_);
+    prof_.stop(id, tag("gv", n), double((n + 2) * B_ * kL) * N * kL * 2 * sizeof(T) * 2);
+    conv_adj(n, kL, gbr_of(n), (n + 1) * B_, chain_of(n));
+  }
 
-      T* mg = f_->mg[n % 2].data();
-      T* mg_prev = f_->mg[(n + 1) % 2].data();
-      conv_adj(n, 1, f_->gz.data(), B_, mg);
-      if (n + 1 < levels_) dec_adj(n, 1, mg_prev, mg, B_, mg);
-    }
+  void rev_chain(int n) {
+    const int64_t N = N_[n];
+    const MaskView mv = mask_view(n);
+    T* chain = chain_of(n);
+    if (n + 1 < levels_) dec_adj(n, kL, chain_of(n + 1), chain, (n + 1) * B_, chain);
+
+    int id = prof_.start();
+    const double* s1 = nmask_ > 0 ? s1c_[n].data() : ds1_[n].data();
+    const double* s2 = nmask_ > 0 ? s2c_[n].data() : ds2_[n].data();
+    if (cross_)
+      launch_gz_x<T>(n, N, Bs_, f_->z[n].data(), gzp_of(n), chain, reinterpret_cast<const double2*>(s1), f_->gz.data(),
+                     mv, s_);
+    else
+      launch_gz<T>(n, N, B_, f_->z[n].data(), gzp_of(n), chain, s1, s2, f_->gz.data(), mv, s_);
+    prof_.stop(id, tag("gz", n), double(B_) * kL * N * (6 * sizeof(T)));
+
+    T* mg = f_->mg[n % 2].data();
+    T* mg_prev = f_->mg[(n + 1) % 2].data();
+    conv_adj(n, 1, f_->gz.data(), B_, mg);
+    if (n + 1 < levels_) dec_adj(n, 1, mg_prev, mg, B_, mg);
+  }
+
+  void rev_finish(double* grad_dev) {
     if (!cross_)
       launch_meanvar_grad<T>(N_[0], B_, f_->maps[0].data(), nmask_ > 0 ? gmc_.data() : dgm_.data(),
                              nmask_ > 0 ? gvc_.data() : dgv_.data(), f_->mg[0].data(), mask_view(0), s_);
@@ -608,12 +671,19 @@ class EngineT final : public Engine {
       f_->M[n].resize(size_t(n + 1) * B_ * N * kL);
       f_->br[n].resize(size_t(n + 1) * B_ * kL * N * kL);
     }
+    // Level 0 has the largest branch, z and modulus fields: (n + 1) / 4^n <= 1.
     f_->gbr.resize(f_->br[0].size());
-    for (int n = 1; n < levels_; ++n) f_->gbr.resize(std::max(f_->gbr.size(), f_->br[n].size()));
     f_->gzp.resize(f_->z[0].size());
     f_->gz.resize(f_->z[0].size());
     f_->acc[0].resize(f_->M[0].size());
-    f_->acc[1].resize(std::max(f_->M[0].size(), levels_ > 1 ? f_->M[1].size() : size_t(0)));
+    if (levels_ > 1) {
+      size_t g1 = 0;
+      for (int n = 1; n < levels_; ++n) g1 = std::max(g1, f_->br[n].size());
+      f_->gbr1.resize(g1);
+      f_->gzp1.resize(f_->z[1].size());
+      f_->acc[1].resize(f_->M[1].size());
+      f_->acc[2].resize(f_->M[1].size());
+    }
     f_->mg[0].resize(f_->maps[0].size());
     f_->mg[1].resize(f_->maps[0].size());
     alloc_sums();
@@ -688,7 +758,8 @@ class EngineT final : public Engine {
 
   void drop_graphs() {
     for (auto* cache : {&fwd_graphs_, &vjp_graphs_}) {
-      for (auto& e : *cache) (void)hipGraphExecDestroy(e.exec);
+      for (auto& e : *cache)
+        for (int i = 0; i < e.count; ++i) (void)hipGraphExecDestroy(e.exec[i]);
       cache->clear();
     }
   }
@@ -740,48 +811,65 @@ class EngineT final : public Engine {
     prof_.stop(id, tag("dec_adj", n), dec_bytes(n, NI, rows, true));
   }
 
+  // The forward sweep in phases: per level, the z convolution with its
+  // modulus and the coarser moduli (fwd_z), then the branch convolution and
+  // the sums (fwd_sums); then the statistics.  Levels >= 1 need only level
+  // 0's fwd_z, so level 0's fwd_sums may run beside them.
   void sweep_forward(bool moments) {
     for (int n = 0; n < levels_; ++n) {
-      const int64_t N = N_[n];
-      int id = prof_.start();
-      launch_conv_fwd<T>(conv_[n]->store.view(), 1, f_->maps[n].data(), B_, f_->z[n].data(),
-                         f_->M[n].data() + size_t(n) * B_ * N * kL, s_, conv_var_[n][0]);
-      prof_.stop(id, tag("conv_fwd_z", n), conv_bytes(n, 1, B_, false) + double(B_) * N * kL * sizeof(T));
-      if (n > 0) {
-        id = prof_.start();
-        launch_dec_fwd<T>(dec_[n - 1]->store.view(), kL, f_->M[n - 1].data(), n * B_, f_->M[n].data(), s_);
-        prof_.stop(id, tag("dec_fwd", n), dec_bytes(n - 1, kL, n * B_));
-      }
-      id = prof_.start();
-      launch_conv_fwd<T>(conv_[n]->store.view(), kL, f_->M[n].data(), (n + 1) * B_, f_->br[n].data(), nullptr, s_,
-                         conv_var_[n][1]);
-      prof_.stop(id, tag("conv_fwd_branch", n), conv_bytes(n, kL, (n + 1) * B_, false));
-      id = prof_.start();
-      int nb = nblk_gram_[n];
-      const MaskView mv = mask_view(n);
-      const int64_t rows = int64_t(Bs_) * Km_;
-      const int K = gram_k(n);
-      int64_t m = rows * kL * (cross_ ? K * K : gram_e(n));
-      if (cross_)
-        launch_gram_x<T>(n, N, Bs_, f_->br[n].data(), f_->z[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(),
-                         mv, s_);
-      else
-        launch_gram<T>(n, N, B_, f_->br[n].data(), f_->z[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(),
-                       zpart_[n].data(), mv, s_);
-      launch_reduce_parts(gpart_[n].data(), nb, m * 2, gsum_[n].data(), s_);
-      if (moments) launch_reduce_parts(gpart2_[n].data(), nb, m, gsum2_[n].data(), s_);
-      prof_.stop(id, tag("gram", n), double(rows) * kL * N * (gram_k(n)) * 2 * sizeof(T));
-      id = prof_.start();
-      // The S1/S2 sums came with the Gram pass (one partial per Gram block);
-      // the cross S1 sums take their own pass.
-      if (cross_) launch_zcross<T>(N, Bs_, f_->z[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
-      launch_reduce_parts(zpart_[n].data(), cross_ ? nblk_z_[n] : nb, rows * kL * 4, zsum_[n].data(), s_);
-      if (n == 0 && !cross_) {
-        launch_meanvar<T>(N, B_, f_->maps[0].data(), nblk_mv_, mvpart_.data(), mv, s_);
-        launch_reduce_parts(mvpart_.data(), nblk_mv_, rows * 4, mvsum_.data(), s_);
-      }
-      prof_.stop(id, tag("zstats", n), double(rows) * kL * N * 2 * sizeof(T));
+      fwd_z(n);
+      fwd_sums(n, moments);
     }
+    fwd_finish(moments);
+  }
+
+  void fwd_z(int n) {
+    const int64_t N = N_[n];
+    int id = prof_.start();
+    launch_conv_fwd<T>(conv_[n]->store.view(), 1, f_->maps[n].data(), B_, f_->z[n].data(),
+                       f_->M[n].data() + size_t(n) * B_ * N * kL, s_, conv_var_[n][0]);
+    prof_.stop(id, tag("conv_fwd_z", n), conv_bytes(n, 1, B_, false) + double(B_) * N * kL * sizeof(T));
+    if (n > 0) {
+      id = prof_.start();
+      launch_dec_fwd<T>(dec_[n - 1]->store.view(), kL, f_->M[n - 1].data(), n * B_, f_->M[n].data(), s_);
+      prof_.stop(id, tag("dec_fwd", n), dec_bytes(n - 1, kL, n * B_));
+    }
+  }
+
+  void fwd_sums(int n, bool moments) {
+    const int64_t N = N_[n];
+    int id = prof_.start();
+    launch_conv_fwd<T>(conv_[n]->store.view(), kL, f_->M[n].data(), (n + 1) * B_, f_->br[n].data(), nullptr, s_,
+                       conv_var_[n][1]);
+    prof_.stop(id, tag("conv_fwd_branch", n), conv_bytes(n, kL, (n + 1) * B_, false));
+    id = prof_.start();
+    int nb = nblk_gram_[n];
+    const MaskView mv = mask_view(n);
+    const int64_t rows = int64_t(Bs_) * Km_;
+    const int K = gram_k(n);
+    int64_t m = rows * kL * (cross_ ? K * K : gram_e(n));
+    if (cross_)
+      launch_gram_x<T>(n, N, Bs_, f_->br[n].data(), f_->z[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(),
+                       mv, s_);
+    else
+      launch_gram<T>(n, N, B_, f_->br[n].data(), f_->z[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(),
+                     zpart_[n].data(), mv, s_);
+    launch_reduce_parts(gpart_[n].data(), nb, m * 2, gsum_[n].data(), s_);
+    if (moments) launch_reduce_parts(gpart2_[n].data(), nb, m, gsum2_[n].data(), s_);
+    prof_.stop(id, tag("gram", n), double(rows) * kL * N * (gram_k(n)) * 2 * sizeof(T));
+    id = prof_.start();
+    // The S1/S2 sums came with the Gram pass (one partial per Gram block);
+    // the cross S1 sums take their own pass.
+    if (cross_) launch_zcross<T>(N, Bs_, f_->z[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
+    launch_reduce_parts(zpart_[n].data(), cross_ ? nblk_z_[n] : nb, rows * kL * 4, zsum_[n].data(), s_);
+    if (n == 0 && !cross_) {
+      launch_meanvar<T>(N, B_, f_->maps[0].data(), nblk_mv_, mvpart_.data(), mv, s_);
+      launch_reduce_parts(mvpart_.data(), nblk_mv_, rows * 4, mvsum_.data(), s_);
+    }
+    prof_.stop(id, tag("zstats", n), double(rows) * kL * N * 2 * sizeof(T));
+  }
+
+  void fwd_finish(bool moments) {
     const size_t nv = nvals();
     launch_assemble(int(plan_.size()), Bs_, Km_, levels_, dstat_.data(), dsrc_.data(), dnd_.data(), moments,
                     dvalues_.data(), dvar_.data(), s_);
@@ -810,7 +898,8 @@ class EngineT final : public Engine {
   struct GraphEntry {
     const void* ptr;
     bool flag;
-    hipGraphExec_t exec;
+    std::array<hipGraphExec_t, 4> exec{};  // one graph, or the 4 phases of a split sweep
+    int count = 0;
   };
   static constexpr size_t kMaxGraphs = 8;
 
@@ -829,6 +918,31 @@ class EngineT final : public Engine {
   }
 
   template <typename Body>
+  hipGraphExec_t instantiate(Body&& body) {
+    hipGraph_t graph = capture(body);
+    hipGraphExec_t exec = nullptr;
+    hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    STL_HIP(err);
+    return exec;
+  }
+
+  GraphEntry* cached(std::vector<GraphEntry>& cache, const void* ptr, bool flag) {
+    for (auto& e : cache)
+      if (e.ptr == ptr && e.flag == flag) return &e;
+    return nullptr;
+  }
+
+  GraphEntry* remember(std::vector<GraphEntry>& cache, const GraphEntry& entry) {
+    if (cache.size() == kMaxGraphs) {
+      for (int i = 0; i < cache.front().count; ++i) (void)hipGraphExecDestroy(cache.front().exec[i]);
+      cache.erase(cache.begin());
+    }
+    cache.push_back(entry);
+    return &cache.back();
+  }
+
+  template <typename Body>
   void run(std::vector<GraphEntry>& cache, const void* ptr, bool flag, Body&& body) {
     if (prof_.on()) {
       body();
@@ -836,22 +950,60 @@ class EngineT final : public Engine {
       prof_.collect();
       return;
     }
-    hipGraphExec_t exec = nullptr;
-    for (auto& e : cache)
-      if (e.ptr == ptr && e.flag == flag) exec = e.exec;
-    if (!exec) {
-      hipGraph_t graph = capture(body);
-      hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      STL_HIP(err);
-      if (cache.size() == kMaxGraphs) {
-        (void)hipGraphExecDestroy(cache.front().exec);
-        cache.erase(cache.begin());
-      }
-      cache.push_back(GraphEntry{ptr, flag, exec});
+    GraphEntry* e = cached(cache, ptr, flag);
+    if (!e) {
+      GraphEntry n{ptr, flag};
+      n.exec[0] = instantiate(body);
+      n.count = 1;
+      e = remember(cache, n);
     }
-    STL_HIP(hipGraphLaunch(exec, s_));
+    STL_HIP(hipGraphLaunch(e->exec[0], s_));
     STL_HIP(hipStreamSynchronize(s_));
+  }
+
+  // A sweep in 4 phases: `first` on the engine stream, then `side` on the
+  // second stream beside `rest` on the engine stream, then `last` once both
+  // are done.  Each phase is a graph of its own (captured at the first call
+  // with these arguments); with timers, or a single level, the phases run in
+  // order on the engine stream.
+  struct Phases {
+    std::function<void()> first, side, rest, last;
+  };
+
+  void run_split(std::vector<GraphEntry>& cache, const void* ptr, bool flag, const Phases& ph) {
+    if (prof_.on() || levels_ < 2) {
+      run(cache, ptr, flag, [&] {
+        ph.first();
+        ph.side();
+        ph.rest();
+        ph.last();
+      });
+      return;
+    }
+    GraphEntry* e = cached(cache, ptr, flag);
+    if (!e) {
+      GraphEntry n{ptr, flag};
+      try {
+        for (const auto* body : {&ph.first, &ph.side, &ph.rest, &ph.last}) n.exec[n.count++] = instantiate(*body);
+      } catch (...) {
+        for (int i = 0; i < n.count; ++i) (void)hipGraphExecDestroy(n.exec[i]);
+        throw;
+      }
+      e = remember(cache, n);
+    }
+    hipError_t err = hipGraphLaunch(e->exec[0], s_);
+    if (err == hipSuccess) err = hipEventRecord(ev_fork_, s_);
+    if (err == hipSuccess) err = hipStreamWaitEvent(s2_, ev_fork_, 0);
+    if (err == hipSuccess) err = hipGraphLaunch(e->exec[1], s2_);
+    if (err == hipSuccess) err = hipGraphLaunch(e->exec[2], s_);
+    if (err == hipSuccess) err = hipEventRecord(ev_join_, s2_);
+    if (err == hipSuccess) err = hipStreamWaitEvent(s_, ev_join_, 0);
+    if (err == hipSuccess) err = hipGraphLaunch(e->exec[3], s_);
+    // Both streams drained before any error is reported.
+    const hipError_t e2 = hipStreamSynchronize(s2_), e1 = hipStreamSynchronize(s_);
+    STL_HIP(err);
+    STL_HIP(e2);
+    STL_HIP(e1);
   }
 
   // Convolution kernel variant per level and kernel (forward z, forward
@@ -1038,6 +1190,8 @@ class EngineT final : public Engine {
 
   EngineOptions opt_;
   hipStream_t s_ = nullptr;
+  hipStream_t s2_ = nullptr;  // the split sweeps' second stream
+  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
   std::vector<std::array<int, 4>> conv_var_;
   std::vector<TuneRecord> tune_ms_;
   std::vector<GraphEntry> fwd_graphs_, vjp_graphs_;
