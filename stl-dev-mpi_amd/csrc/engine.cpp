@@ -278,11 +278,12 @@ class EngineT final : public Engine {
     uint64_t holder = 0;
     std::vector<DevArray<T>> maps, M;
     std::vector<DevArray<C2>> z, br;
-    // Reverse buffers: gbr / gzp / acc[0] hold level 0, gbr1 / gzp1 and
-    // acc[1 + (n & 1)] the levels n >= 1, so that level 0's gV and branch
-    // adjoint can run beside the coarser levels' reverse.
-    DevArray<C2> gbr, gzp, gz, gbr1, gzp1;
-    DevArray<T> acc[3], mg[2];
+    // Reverse buffers: gbr / gzp / acc[0] hold level 0, gbr1 / gzp1 / acc[1]
+    // level 1, gbr2 / gzp2 and acc[2 + (n & 1)] the levels n >= 2, so that
+    // level 0's and level 1's gV and branch adjoints can run beside the
+    // coarser levels' reverse.
+    DevArray<C2> gbr, gzp, gz, gbr1, gzp1, gbr2, gzp2;
+    DevArray<T> acc[4], mg[2];
   };
 
  public:
@@ -300,10 +301,10 @@ class EngineT final : public Engine {
     STL_CHECK(opt.batch >= 1, "batch must be positive");
     STL_HIP(hipSetDevice(opt.device));
     STL_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
-    // Default priority, as the engine stream.
+    // Side streams of the split sweeps, default priority as the engine stream.
     STL_HIP(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking));
-    STL_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
-    STL_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+    STL_HIP(hipStreamCreateWithFlags(&s3_, hipStreamNonBlocking));
+    for (auto& ev : ev_) STL_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     prof_.enable(opt.timing, s_);
     cross_ = opt.cross;
     plan_ = cross_ ? build_cross_plan(opt.J, kL) : build_plan(opt.J, kL);
@@ -339,8 +340,9 @@ class EngineT final : public Engine {
     for (auto* cache : {&fwd_graphs_, &vjp_graphs_})
       for (auto& e : *cache)
         for (int i = 0; i < e.count; ++i) (void)hipGraphExecDestroy(e.exec[i]);
-    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
-    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    for (auto& ev : ev_)
+      if (ev) (void)hipEventDestroy(ev);
+    if (s3_) (void)hipStreamDestroy(s3_);
     if (s2_) (void)hipStreamDestroy(s2_);
     if (s_) (void)hipStreamDestroy(s_);
   }
@@ -436,20 +438,7 @@ class EngineT final : public Engine {
 
   void forward_device(const double* x_dev, bool moments, cplx* values, double* variance) override {
     f_->holder = id_;
-    run_split(fwd_graphs_, x_dev, moments,
-              Phases{[&] {
-                       load_maps(x_dev);
-                       for (int n = 1; n < levels_; ++n) dec_maps(n);
-                       fwd_z(0);
-                     },
-                     [&] { fwd_sums(0, moments); },
-                     [&] {
-                       for (int n = 1; n < levels_; ++n) {
-                         fwd_z(n);
-                         fwd_sums(n, moments);
-                       }
-                     },
-                     [&] { fwd_finish(moments); }});
+    run_steps(fwd_graphs_, x_dev, moments, forward_steps(x_dev, moments));
     finish_forward(moments, values, variance);
   }
 
@@ -486,19 +475,7 @@ class EngineT final : public Engine {
   void vjp_device(const cplx* cot, double* grad_dev) override {
     check_holder();
     stage_cotangent(cot);
-    run_split(vjp_graphs_, grad_dev, false,
-              Phases{[&] { rev_prelude(); },
-                     [&] { rev_branch(0); },
-                     [&] {
-                       for (int n = levels_ - 1; n >= 1; --n) {
-                         rev_branch(n);
-                         rev_chain(n);
-                       }
-                     },
-                     [&] {
-                       rev_chain(0);
-                       rev_finish(grad_dev);
-                     }});
+    run_steps(vjp_graphs_, grad_dev, false, reverse_steps(grad_dev));
   }
 
   // The reverse sweep in phases: the prelude (cotangents onto the sums'
@@ -538,10 +515,10 @@ class EngineT final : public Engine {
     }
   }
 
-  // Reverse buffers of level n (level 0 apart from the others).
-  C2* gbr_of(int n) { return n == 0 ? f_->gbr.data() : f_->gbr1.data(); }
-  C2* gzp_of(int n) { return n == 0 ? f_->gzp.data() : f_->gzp1.data(); }
-  T* chain_of(int n) { return f_->acc[n == 0 ? 0 : 1 + (n & 1)].data(); }
+  // Reverse buffers of level n (levels 0 and 1 apart from the others).
+  C2* gbr_of(int n) { return n == 0 ? f_->gbr.data() : n == 1 ? f_->gbr1.data() : f_->gbr2.data(); }
+  C2* gzp_of(int n) { return n == 0 ? f_->gzp.data() : n == 1 ? f_->gzp1.data() : f_->gzp2.data(); }
+  T* chain_of(int n) { return f_->acc[n < 2 ? n : 2 + (n & 1)].data(); }
 
   void rev_branch(int n) {
     const int64_t N = N_[n];
@@ -677,12 +654,17 @@ class EngineT final : public Engine {
     f_->gz.resize(f_->z[0].size());
     f_->acc[0].resize(f_->M[0].size());
     if (levels_ > 1) {
-      size_t g1 = 0;
-      for (int n = 1; n < levels_; ++n) g1 = std::max(g1, f_->br[n].size());
-      f_->gbr1.resize(g1);
+      f_->gbr1.resize(f_->br[1].size());
       f_->gzp1.resize(f_->z[1].size());
       f_->acc[1].resize(f_->M[1].size());
-      f_->acc[2].resize(f_->M[1].size());
+    }
+    if (levels_ > 2) {
+      size_t g2 = 0;
+      for (int n = 2; n < levels_; ++n) g2 = std::max(g2, f_->br[n].size());
+      f_->gbr2.resize(g2);
+      f_->gzp2.resize(f_->z[2].size());
+      f_->acc[2].resize(f_->M[2].size());
+      f_->acc[3].resize(f_->M[2].size());
     }
     f_->mg[0].resize(f_->maps[0].size());
     f_->mg[1].resize(f_->maps[0].size());
@@ -891,17 +873,17 @@ class EngineT final : public Engine {
     if (moments && variance) std::memcpy(variance, hvar_.data(), nv * sizeof(double));
   }
 
-  // Runs body's stream work: eagerly with per-kernel timers, else as a HIP
-  // graph captured at the first call with these arguments (kernel arguments
-  // are fixed in a graph, so it is keyed by the device pointer and the flag)
-  // and replayed afterwards; returns after the stream has completed.
+  // Executable graphs of a sweep, captured at the first call with these
+  // arguments (kernel arguments are fixed in a graph, so they are keyed by
+  // the device pointer and the flag) and replayed afterwards.
   struct GraphEntry {
     const void* ptr;
     bool flag;
-    std::array<hipGraphExec_t, 4> exec{};  // one graph, or the 4 phases of a split sweep
+    std::array<hipGraphExec_t, 8> exec{};  // one graph, or the steps of a split sweep
     int count = 0;
   };
   static constexpr size_t kMaxGraphs = 8;
+  static constexpr int kMaxSteps = 8;
 
   template <typename Body>
   hipGraph_t capture(Body&& body) {
@@ -942,10 +924,21 @@ class EngineT final : public Engine {
     return &cache.back();
   }
 
-  template <typename Body>
-  void run(std::vector<GraphEntry>& cache, const void* ptr, bool flag, Body&& body) {
+  // A sweep as a list of steps, each a graph of its own (captured at the
+  // first call with these arguments): step i runs on stream `on` (0: the
+  // engine stream, 1 and 2: side streams) after the earlier steps of that
+  // stream and the steps listed in `after`.  With timers the steps run in
+  // list order on the engine stream, so the list is a valid sequential order.
+  struct Step {
+    std::function<void()> body;
+    int on;
+    std::vector<int> after;
+  };
+
+  void run_steps(std::vector<GraphEntry>& cache, const void* ptr, bool flag, const std::vector<Step>& steps) {
+    STL_CHECK(!steps.empty() && steps.size() <= size_t(kMaxSteps), "sweep: bad step list");
     if (prof_.on()) {
-      body();
+      for (const auto& st : steps) st.body();
       STL_HIP(hipStreamSynchronize(s_));
       prof_.collect();
       return;
@@ -953,57 +946,80 @@ class EngineT final : public Engine {
     GraphEntry* e = cached(cache, ptr, flag);
     if (!e) {
       GraphEntry n{ptr, flag};
-      n.exec[0] = instantiate(body);
-      n.count = 1;
-      e = remember(cache, n);
-    }
-    STL_HIP(hipGraphLaunch(e->exec[0], s_));
-    STL_HIP(hipStreamSynchronize(s_));
-  }
-
-  // A sweep in 4 phases: `first` on the engine stream, then `side` on the
-  // second stream beside `rest` on the engine stream, then `last` once both
-  // are done.  Each phase is a graph of its own (captured at the first call
-  // with these arguments); with timers, or a single level, the phases run in
-  // order on the engine stream.
-  struct Phases {
-    std::function<void()> first, side, rest, last;
-  };
-
-  void run_split(std::vector<GraphEntry>& cache, const void* ptr, bool flag, const Phases& ph) {
-    if (prof_.on() || levels_ < 2) {
-      run(cache, ptr, flag, [&] {
-        ph.first();
-        ph.side();
-        ph.rest();
-        ph.last();
-      });
-      return;
-    }
-    GraphEntry* e = cached(cache, ptr, flag);
-    if (!e) {
-      GraphEntry n{ptr, flag};
       try {
-        for (const auto* body : {&ph.first, &ph.side, &ph.rest, &ph.last}) n.exec[n.count++] = instantiate(*body);
+        for (const auto& st : steps) n.exec[n.count++] = instantiate(st.body);
       } catch (...) {
         for (int i = 0; i < n.count; ++i) (void)hipGraphExecDestroy(n.exec[i]);
         throw;
       }
       e = remember(cache, n);
     }
-    hipError_t err = hipGraphLaunch(e->exec[0], s_);
-    if (err == hipSuccess) err = hipEventRecord(ev_fork_, s_);
-    if (err == hipSuccess) err = hipStreamWaitEvent(s2_, ev_fork_, 0);
-    if (err == hipSuccess) err = hipGraphLaunch(e->exec[1], s2_);
-    if (err == hipSuccess) err = hipGraphLaunch(e->exec[2], s_);
-    if (err == hipSuccess) err = hipEventRecord(ev_join_, s2_);
-    if (err == hipSuccess) err = hipStreamWaitEvent(s_, ev_join_, 0);
-    if (err == hipSuccess) err = hipGraphLaunch(e->exec[3], s_);
-    // Both streams drained before any error is reported.
-    const hipError_t e2 = hipStreamSynchronize(s2_), e1 = hipStreamSynchronize(s_);
+    const hipStream_t streams[3] = {s_, s2_, s3_};
+    hipError_t err = hipSuccess;
+    for (size_t i = 0; i < steps.size() && err == hipSuccess; ++i) {
+      const hipStream_t st = streams[steps[i].on];
+      for (int d : steps[i].after)
+        if (err == hipSuccess) err = hipStreamWaitEvent(st, ev_[d], 0);
+      if (err == hipSuccess) err = hipGraphLaunch(e->exec[i], st);
+      if (err == hipSuccess) err = hipEventRecord(ev_[i], st);
+    }
+    // Every stream drained before any error is reported.
+    const hipError_t e3 = hipStreamSynchronize(s3_), e2 = hipStreamSynchronize(s2_), e1 = hipStreamSynchronize(s_);
     STL_HIP(err);
+    STL_HIP(e3);
     STL_HIP(e2);
     STL_HIP(e1);
+  }
+
+  // The forward's steps.  After level n's z convolution (and the moduli it
+  // completes), levels > n need nothing more from level n, and level n's
+  // branch convolution and sums need nothing from them: levels 0 and 1's
+  // sums run on side streams beside the coarser levels.
+  std::vector<Step> forward_steps(const double* x_dev, bool moments) {
+    auto first = [this, x_dev] {
+      load_maps(x_dev);
+      for (int n = 1; n < levels_; ++n) dec_maps(n);
+      fwd_z(0);
+    };
+    auto coarser = [this, moments](int from) {
+      return [this, moments, from] {
+        for (int n = from; n < levels_; ++n) {
+          fwd_z(n);
+          fwd_sums(n, moments);
+        }
+      };
+    };
+    auto sums = [this, moments](int n) { return [this, moments, n] { fwd_sums(n, moments); }; };
+    auto finish = [this, moments] { fwd_finish(moments); };
+    if (levels_ == 1) return {{[first, sums, finish] { first(); sums(0)(); finish(); }, 0, {}}};
+    if (levels_ == 2) return {{first, 0, {}}, {sums(0), 1, {0}}, {coarser(1), 0, {}}, {finish, 0, {1}}};
+    return {{first, 0, {}},        {sums(0), 1, {0}},    {[this] { fwd_z(1); }, 0, {}},
+            {sums(1), 2, {2}},     {coarser(2), 0, {}}, {finish, 0, {1, 3}}};
+  }
+
+  // The reverse's steps.  Level n's gV and branch adjoint read only level-n
+  // fields and the prelude's operands, and write level-n buffers: levels 0
+  // and 1's run on side streams beside the coarser levels; their chains
+  // (which need the coarser chain) follow on the engine stream.
+  std::vector<Step> reverse_steps(double* grad_dev) {
+    auto coarser = [this](int down_to) {
+      return [this, down_to] {
+        for (int n = levels_ - 1; n >= down_to; --n) {
+          rev_branch(n);
+          rev_chain(n);
+        }
+      };
+    };
+    auto last = [this, grad_dev] {
+      rev_chain(0);
+      rev_finish(grad_dev);
+    };
+    auto prelude = [this] { rev_prelude(); };
+    if (levels_ == 1) return {{[this, grad_dev] { sweep_reverse(grad_dev); }, 0, {}}};
+    if (levels_ == 2)
+      return {{prelude, 0, {}}, {[this] { rev_branch(0); }, 1, {0}}, {coarser(1), 0, {}}, {last, 0, {1}}};
+    return {{prelude, 0, {}},        {[this] { rev_branch(0); }, 1, {0}}, {[this] { rev_branch(1); }, 2, {0}},
+            {coarser(2), 0, {}},     {[this] { rev_chain(1); }, 0, {2}},  {last, 0, {1}}};
   }
 
   // Convolution kernel variant per level and kernel (forward z, forward
@@ -1190,8 +1206,8 @@ class EngineT final : public Engine {
 
   EngineOptions opt_;
   hipStream_t s_ = nullptr;
-  hipStream_t s2_ = nullptr;  // the split sweeps' second stream
-  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+  hipStream_t s2_ = nullptr, s3_ = nullptr;  // the split sweeps' side streams
+  std::array<hipEvent_t, kMaxSteps> ev_{};     // one per step of a split sweep
   std::vector<std::array<int, 4>> conv_var_;
   std::vector<TuneRecord> tune_ms_;
   std::vector<GraphEntry> fwd_graphs_, vjp_graphs_;
