@@ -253,12 +253,16 @@ def test_torch_autograd_matches_vjp_and_finite_differences():
         assert abs(fd - exact) <= 1e-6 * abs(exact), (fd, exact)
 
 
-def test_graph_replays_match_eager_launches():
+@pytest.mark.parametrize("J", [1, 2, 3, 4])
+def test_graph_replays_match_eager_launches(J):
     """The forward and the VJP run as HIP graphs (captured per device pointer
-    and moments flag) unless per-kernel timing is on; replays, interleaved with
-    other work and other inputs, give the eager launches' bits."""
+    and moments flag; with 2 levels or more, levels 0 and 1's branch work on
+    side streams) unless per-kernel timing is on; replays, interleaved with
+    other work and other inputs, give the eager launches' bits (J = 1: one
+    graph; J = 2: level 0 on a side stream; J >= 3: levels 0 and 1)."""
     from stl_port import NativeEngine
-    nside, J = 32, 4
+    nside = 32
+    print(f"[gpu] graph vs eager nside={nside} J={J}", flush=True)
     rng = np.random.default_rng(21)
     xs = [rng.standard_normal((1, 12 * nside * nside)) for _ in range(2)]
     with NativeEngine(nside, J, precision="mixed") as g, NativeEngine(nside, J, precision="mixed",
