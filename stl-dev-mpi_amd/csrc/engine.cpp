@@ -976,10 +976,13 @@ class EngineT final : public Engine {
   // branch convolution and sums need nothing from them: levels 0 and 1's
   // sums run on side streams beside the coarser levels.
   std::vector<Step> forward_steps(const double* x_dev, bool moments) {
+    // The maps' decimations are not needed by level 0: they follow the fork.
     auto first = [this, x_dev] {
       load_maps(x_dev);
-      for (int n = 1; n < levels_; ++n) dec_maps(n);
       fwd_z(0);
+    };
+    auto dec_all = [this] {
+      for (int n = 1; n < levels_; ++n) dec_maps(n);
     };
     auto coarser = [this, moments](int from) {
       return [this, moments, from] {
@@ -992,9 +995,25 @@ class EngineT final : public Engine {
     auto sums = [this, moments](int n) { return [this, moments, n] { fwd_sums(n, moments); }; };
     auto finish = [this, moments] { fwd_finish(moments); };
     if (levels_ == 1) return {{[first, sums, finish] { first(); sums(0)(); finish(); }, 0, {}}};
-    if (levels_ == 2) return {{first, 0, {}}, {sums(0), 1, {0}}, {coarser(1), 0, {}}, {finish, 0, {1}}};
-    return {{first, 0, {}},        {sums(0), 1, {0}},    {[this] { fwd_z(1); }, 0, {}},
-            {sums(1), 2, {2}},     {coarser(2), 0, {}}, {finish, 0, {1, 3}}};
+    if (levels_ == 2)
+      return {{first, 0, {}},
+              {sums(0), 1, {0}},
+              {[dec_all, coarser] {
+                 dec_all();
+                 coarser(1)();
+               },
+               0, {}},
+              {finish, 0, {1}}};
+    return {{first, 0, {}},
+            {sums(0), 1, {0}},
+            {[this, dec_all] {
+               dec_all();
+               fwd_z(1);
+             },
+             0, {}},
+            {sums(1), 2, {2}},
+            {coarser(2), 0, {}},
+            {finish, 0, {1, 3}}};
   }
 
   // The reverse's steps.  Level n's gV and branch adjoint read only level-n
