@@ -571,7 +571,7 @@ __device__ inline void tile_group(int order, int rows, int64_t& tr, int& fg, int
 // Forward convolution of FW rotated faces of one rep tile per workgroup:
 // every slot's coefficients are loaded once and applied to the FW faces'
 // staged inputs (faces k = fg * FW + f).
-template <typename T, int NI, int TS, int FW>
+template <typename T, int NI, int TS, int FW, int kSlotUnroll = TS == 32 ? 4 : 2>
 __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const T* __restrict__ x, int rows,
                                                            cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
   constexpr int SW = TS + 2 * kWin, SA = SW * SW;
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
   // The mirror's conjugation, on the coefficients: 4 products per slot for
   // all the faces (at the outputs it would hold more registers across the loop).
   const T sg = m ? T(-1) : T(1);
-#pragma unroll 2
+#pragma unroll kSlotUnroll
   for (int s = 0; s < cnt; ++s) {
     size_t at = size_t(s) * op.R + r;
     int16_t o = op.off[at];
