@@ -155,7 +155,8 @@ int stlp_train_loss(void* h, uint32_t f, bool b, double* out);
 int stlp_train_step(void* h, uint32_t f, double lr, double* out);
 int stlp_train_live(void* h, uint32_t f, const uint64_t* a, const double* b, size_t n, const uint64_t* c,
                     const double* d, size_t m, bool e, double w, double* out);
-void* stlp_train_lbfgs(void* h, const uint32_t* f, size_t n, const char* options);
+/* Returns a JSON report string in the reference (crates/stl-bridge/src/unet.rs:1477). */
+char* stlp_train_lbfgs(void* h, const uint32_t* f, size_t n, const char* options);
 int stlp_train_set_joint(void* h, const double* a, const double* b, uint32_t c, uint32_t d, const char* e, double w);
 int stlp_train_front(void* h, const double* x, size_t n, bool b, double* out);
 
@@ -164,11 +165,13 @@ int stlp_train_front(void* h, const double* x, size_t n, bool b, double* out);
  * :207-268 with crates/stl-core/src/lbfgs.rs:356-465, run on device vectors):
  * L-BFGS of loss = sum_k w_k |S_k(x) - target_k|^2 (cotangent 2 w (S - target))
  * from the device maps x_dev (float64, n = batch * 12 nside^2, updated in
- * place), target interleaved (re, im) [batch, ncoef], weights [ncoef] or NULL
- * (all 1), options as stlp_lbfgs.  Option "ensemble": true makes the batch
- * one ensemble (the reference's stl-synth --ensemble, synthesis.rs:362-470
- * without its scatter term): loss = sum_k w_k |mean over the members of S_k -
- * target_k|^2 with target one row [ncoef].  The loss of every evaluation goes
+ * place), target interleaved (re, im) [batch, K, ncoef] (K = max(masks, 1)),
+ * weights [K, ncoef] (one per mask and coefficient, the layout of
+ * stlp_native_pixels; the same for every map) or NULL (all 1), options as
+ * stlp_lbfgs.  Option "ensemble": true makes the batch one ensemble (the
+ * reference's stl-synth --ensemble, synthesis.rs:362-470 without its scatter
+ * term): loss = sum_k w_k |mean over the members of S_k - target_k|^2 with
+ * target one row [K, ncoef].  The loss of every evaluation goes
  * to loss_history (up to cap).  Returns the number of evaluations or -1. */
 int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* target, const double* weights,
                         const char* options, double* loss_history, size_t cap);

@@ -802,7 +802,10 @@ STL_NI_INT(stlp_train_loss, void*, uint32_t, bool, double*)
 STL_NI_INT(stlp_train_step, void*, uint32_t, double, double*)
 STL_NI_INT(stlp_train_live, void*, uint32_t, const uint64_t*, const double*, size_t, const uint64_t*, const double*,
            size_t, bool, double, double*)
-STL_NI_PTR(stlp_train_lbfgs, void*, const uint32_t*, size_t, const char*)
+char* stlp_train_lbfgs(void*, const uint32_t*, size_t, const char*) {
+  g_error = "not implemented: stlp_train_lbfgs";
+  return nullptr;
+}
 STL_NI_INT(stlp_train_set_joint, void*, const double*, const double*, uint32_t, uint32_t, const char*, double)
 STL_NI_INT(stlp_train_front, void*, const double*, size_t, bool, double*)
 

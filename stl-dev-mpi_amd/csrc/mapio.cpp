@@ -75,12 +75,22 @@ struct NpyLayout {
   int size = 8;     // bytes per element
   bool big = false;
   std::vector<size_t> shape;
+  // Element count; a product that overflows size_t is refused.
   size_t count() const {
     size_t n = 1;
-    for (size_t s : shape) n *= s;
+    for (size_t s : shape) {
+      if (s != 0 && n > SIZE_MAX / s) throw std::runtime_error(".npy shape overflows the address space");
+      n *= s;
+    }
     return n;
   }
 };
+
+uint64_t file_size(const std::string& path) {
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  if (!f) fail(path, "cannot open");
+  return uint64_t(f.tellg());
+}
 
 // Value of a header dict field: the text after "'key':".
 bool dict_field(const std::string& dict, const std::string& key, std::string& out) {
@@ -144,6 +154,11 @@ NpyLayout npy_layout(const std::string& path) {
     if (comma == std::string::npos) break;
     pos = comma + 1;
   }
+  // The data the header announces must be in the file (a malformed shape would
+  // otherwise size a read beyond it).
+  const size_t n = L.count();
+  if (n > SIZE_MAX / size_t(L.size) || L.offset + uint64_t(n) * uint64_t(L.size) > file_size(path))
+    fail(path, "shape exceeds the file");
   return L;
 }
 
@@ -454,7 +469,24 @@ void sliced_create(const std::string& path, size_t n, const std::string& column,
 
 void sliced_write(const std::string& path, bool dbl, size_t start, const double* values, size_t n) {
   const bool fits = is_fits(path);
-  const size_t offset = fits ? 2 * kBlock : npy_header({0}).size();
+  // The map length and the data offset come from the file's own header, so a
+  // slice beyond the map is refused instead of growing or corrupting the file.
+  uint64_t offset;
+  size_t len;
+  if (fits) {
+    const FitsLayout L = fits_layout(path);
+    if (L.repeat != 1 || L.width != (dbl ? 8u : 4u)) fail(path, "not a sliced map of this element type");
+    offset = L.offset;
+    len = L.rows;
+  } else {
+    const NpyLayout L = npy_layout(path);
+    if (L.kind != 'f' || L.size != 8 || L.big || L.shape.size() != 1) fail(path, "not a sliced float64 map");
+    offset = L.offset;
+    len = L.shape[0];
+  }
+  if (start > len || n > len - start)
+    fail(path, "slice " + std::to_string(start) + ".." + std::to_string(start + n) + " beyond the map of " +
+                   std::to_string(len) + " pixels");
   std::vector<char> bytes;
   bytes.reserve(n * 8);
   size_t width = 8;

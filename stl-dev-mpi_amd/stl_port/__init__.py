@@ -1,3 +1,3 @@
 """MI355X ScatCov engine: Python binding of libstl_bridge.so (reference stl_port API)."""
-from .native import (NativeEngine, NativeEnsemble, Snapshot, ergodic_weights, library_path,  # noqa: F401
-                     native_lbfgs)
+from .native import (BridgeError, NativeEngine, NativeEnsemble, Snapshot, bridge, ergodic_weights,  # noqa: F401
+                     library_path, native_lbfgs)

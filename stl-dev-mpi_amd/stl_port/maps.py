@@ -15,7 +15,13 @@ import os
 
 import numpy as np
 
-from .native import _SIZE, _error, _library, _pointer
+from .native import bridge
+
+_SIZE = C.POINTER(C.c_size_t)
+
+
+def _pointer(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
 def _p(path):
@@ -33,64 +39,57 @@ def nside_of(npix):
 
 def map_len(path, library=None):
     """Pixels of a map file, from its header (io.rs:370-379)."""
-    lib = _library(library)
+    lib = bridge(library)
     out = C.c_size_t(0)
-    if lib.stlx_map_len(_p(path), C.byref(out)):
-        raise _error(lib)
+    lib.call('stlx_map_len', _p(path), C.byref(out))
     return int(out.value)
 
 
 def read_map(path, library=None):
     """The whole NESTED map of a `.npy` / `.fits` file as float64 (io.rs:227-233)."""
-    lib = _library(library)
+    lib = bridge(library)
     out = np.empty(map_len(path, library))
-    if lib.stlx_read_map(_p(path), _pointer(out), out.size):
-        raise _error(lib)
+    lib.call('stlx_read_map', _p(path), _pointer(out), out.size)
     return out
 
 
 def read_map_range(path, start, end, library=None):
     """NESTED pixels [start, end) read at their offset (io.rs:387-416)."""
-    lib = _library(library)
+    lib = bridge(library)
     start, end = int(start), int(end)
     if start < 0 or end < start:
         raise ValueError(f"bad pixel range {start}..{end}")
     out = np.empty(end - start)
-    if lib.stlx_read_map_range(_p(path), start, end, _pointer(out)):
-        raise _error(lib)
+    lib.call('stlx_read_map_range', _p(path), start, end, _pointer(out))
     return out
 
 
 def read_npy(path, library=None):
     """A `.npy` array as float64 with its shape (io.rs:27-49)."""
-    lib = _library(library)
+    lib = bridge(library)
     ndim = C.c_size_t(32)
     shape = np.zeros(32, dtype=np.uintp)
-    if lib.stlx_read_npy(_p(path), None, 0, shape.ctypes.data_as(_SIZE), C.byref(ndim)):
-        raise _error(lib)
+    lib.call('stlx_read_npy', _p(path), None, 0, shape.ctypes.data_as(_SIZE), C.byref(ndim))
     dims = tuple(int(d) for d in shape[:ndim.value])
     out = np.empty(dims)
-    if lib.stlx_read_npy(_p(path), _pointer(out), out.size, None, C.byref(ndim)):
-        raise _error(lib)
+    lib.call('stlx_read_npy', _p(path), _pointer(out), out.size, None, C.byref(ndim))
     return out
 
 
 def write_npy(path, values, library=None):
     """Little-endian float64 `.npy` of the array's shape (io.rs:52-78)."""
-    lib = _library(library)
+    lib = bridge(library)
     v = np.ascontiguousarray(values, dtype=np.float64)
     shape = np.asarray(v.shape, dtype=np.uintp)
-    if lib.stlx_write_npy(_p(path), _pointer(v), shape.ctypes.data_as(_SIZE), v.ndim):
-        raise _error(lib)
+    lib.call('stlx_write_npy', _p(path), _pointer(v), shape.ctypes.data_as(_SIZE), v.ndim)
 
 
 def write_map(path, values, column="VALUE", double=False, library=None):
     """A flat map file by extension: HEALPix FITS (float32 `E`, float64 `D` when double) for
     `.fits` / `.fit`, else `.npy` float64 (io.rs:52-55, 99-170)."""
-    lib = _library(library)
+    lib = bridge(library)
     v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
-    if lib.stlx_write_map(_p(path), _pointer(v), v.size, column.encode(), bool(double)):
-        raise _error(lib)
+    lib.call('stlx_write_map', _p(path), _pointer(v), v.size, column.encode(), bool(double))
 
 
 class SlicedMap:
@@ -101,27 +100,24 @@ class SlicedMap:
     def __init__(self, path, double=False, library=None):
         self.path = os.fspath(path)
         self.double = bool(double)
-        self._lib = _library(library)
+        self._lib = bridge(library)
 
     def create(self, n, column=""):
-        if self._lib.stlx_sliced_create(_p(self.path), int(n), column.encode(), self.double):
-            raise _error(self._lib)
+        self._lib.call('stlx_sliced_create', _p(self.path), int(n), column.encode(), self.double)
 
     def write(self, start, values):
         v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
-        if self._lib.stlx_sliced_write(_p(self.path), self.double, int(start), _pointer(v), v.size):
-            raise _error(self._lib)
+        self._lib.call('stlx_sliced_write', _p(self.path), self.double, int(start), _pointer(v), v.size)
 
 
 def pixel_normal(seed, start, end, library=None):
     """Standard normal values of pixels [start, end) for seed: splitmix64 of the seed and the
     pixel id, Box-Muller of two 53-bit uniforms (lib.rs:523-540); the same map whatever the
     decomposition."""
-    lib = _library(library)
+    lib = bridge(library)
     start, end = int(start), int(end)
     out = np.empty(max(end - start, 0))
-    if lib.stlx_pixel_normal(C.c_uint64(int(seed) & (2**64 - 1)), start, out.size, _pointer(out)):
-        raise _error(lib)
+    lib.call('stlx_pixel_normal', C.c_uint64(int(seed) & (2**64 - 1)), start, out.size, _pointer(out))
     return out
 
 

@@ -28,11 +28,27 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
-def test_binding_table_covers_the_reference_binding():
-    # the reference binding (native.py:55-108) binds these eagerly; all must resolve
-    lib = native._library()
-    for name in native.SIGNATURES:
-        assert hasattr(lib, name), name
+def test_library_exports_what_the_reference_binding_binds():
+    # the reference binding (native.py:55-116) sets argtypes on each of these names at load
+    # time, so every one must resolve; the list is data extracted from it
+    # (tests/golden/binding/extract_symbols.py)
+    ref = json.loads((ROOT / "tests" / "golden" / "binding" / "reference_symbols.json").read_text())["symbols"]
+    assert len(ref) == 52
+    lib = C.CDLL(native.library_path())
+    missing = [n for n in ref if not hasattr(lib, n)]
+    assert not missing, missing
+    # and this binding types every one of them
+    assert set(ref) <= set(native.ABI)
+
+
+def test_abi_table_matches_the_header():
+    # every typed entry point is declared in include/stl_bridge.h with as many parameters
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    for name, code in native.ABI.items():
+        m = re.search(r"\b" + name + r"\s*\(([^)]*)\)", text)
+        assert m, name
+        params = [p for p in m.group(1).split(",") if p.strip() and p.strip() != "void"]
+        assert len(params) == len(code) - 1, (name, params, code)
 
 
 @pytest.mark.parametrize("config, message", [
@@ -50,13 +66,13 @@ def test_binding_table_covers_the_reference_binding():
     ('{"nside": 16, "j": 3, "conv_variant": -2}', "conv_variant"),
 ])
 def test_bad_configuration_is_an_error(config, message):
-    lib = native._library()
-    assert not lib.stlp_new(config.encode())
-    assert message in lib.stlp_last_error().decode()
+    lib = native.bridge()
+    with pytest.raises(native.BridgeError, match=message):
+        lib.call("stlp_new", config.encode())
 
 
 def test_other_products_are_not_implemented():
-    lib = native._library()
+    lib = native.bridge().cdll
     assert not lib.stlp_patch_new(b"{}")
     assert "not implemented" in lib.stlp_last_error().decode()
     assert lib.stlp_unet_zero_grad(None) == -1
@@ -67,6 +83,7 @@ def test_identity_key_matches_the_reference_hash():
     # native.py:122-129: first 8 bytes of sha256(objective), little endian
     import hashlib
     g, m, o = native._key(3, 7, "gaussian")
+    assert native.objective_id("gaussian") == o
     assert (g, m) == (3, 7)
     assert o == int.from_bytes(hashlib.sha256(b"gaussian").digest()[:8], "little")
     with pytest.raises(ValueError):
@@ -76,9 +93,9 @@ def test_identity_key_matches_the_reference_hash():
 
 
 def test_lbfgs_failure_value_without_callback_errors():
-    lib = native._library()
+    lib = native.bridge().cdll
     x = np.zeros(4)
-    assert lib.stlp_lbfgs(native._pointer(x), 4, b"{bad", native._CALLBACK(lambda *a: 0.0), None) == -1
+    assert lib.stlp_lbfgs(native._dptr(x), 4, b"{bad", native._CALLBACK(lambda *a: 0.0), None) == -1
 
 
 @pytest.mark.parametrize("nside, j, batch, options, message", [
@@ -96,6 +113,6 @@ def test_bad_ensemble_is_an_error(nside, j, batch, options, message):
 
 
 def test_ensemble_without_groups_is_an_error():
-    lib = native._library()
+    lib = native.bridge().cdll
     assert not lib.stlx_ensemble_new(None, None, None, 0, b"{}")
     assert "at least one group" in lib.stlp_last_error().decode()

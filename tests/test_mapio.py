@@ -180,6 +180,37 @@ def test_errors_name_the_file(tmp_path):
         M.read_map(bad)
 
 
+@pytest.mark.parametrize("name, double", [("s.npy", True), ("s.fits", False), ("s.fits", True)])
+def test_sliced_writes_stay_inside_the_map(tmp_path, name, double):
+    path = tmp_path / name
+    out = M.SlicedMap(path, double)
+    out.create(192, "VALUE")
+    size = path.stat().st_size
+    out.write(100, VALUES[100:192])
+    with pytest.raises(RuntimeError, match="beyond the map"):
+        out.write(100, VALUES[:93])
+    with pytest.raises(RuntimeError, match="beyond the map"):
+        out.write(193, VALUES[:0])
+    assert path.stat().st_size == size
+    out.write(0, VALUES[:100])
+    expect = VALUES if double else SINGLE
+    np.testing.assert_array_equal(M.read_map(path), expect)
+
+
+def test_npy_shape_beyond_the_file_is_refused(tmp_path):
+    # a header whose shape announces more data than the file holds (or overflows the
+    # element count) is refused before anything is read
+    for shape in ("(1000,)", "(4294967296, 4294967296, 16)"):
+        path = tmp_path / "h.npy"
+        header = "{'descr': '<f8', 'fortran_order': False, 'shape': " + shape + ", }"
+        header += " " * ((64 - (10 + len(header) + 1) % 64) % 64) + "\n"
+        path.write_bytes(b"\x93NUMPY\x01\x00" + len(header).to_bytes(2, "little") + header.encode() + bytes(64))
+        with pytest.raises(RuntimeError, match="exceeds the file|overflows"):
+            M.read_npy(path)
+        with pytest.raises(RuntimeError, match="exceeds the file|overflows"):
+            M.map_len(path)
+
+
 def test_takahashi_fixture_reads_as_numpy_does():
     path = GOLDEN / "takahashi" / "takahashi_r000_zs1_nside64_nested.npy"
     a = np.load(path)
