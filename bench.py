@@ -28,6 +28,10 @@ sys.path.insert(0, str(ROOT / "stl-dev-mpi_amd"))
 METRIC = "fwd+VJP evals/s, nside 1024 L=4: mixed vs FP64 GPU at ≤1e-6 rel err; HBM GB/s"
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
 FAMILIES = ["mean", "var", "S1", "S2", "S3", "S4"]
+ERROR_BAR = 1e-6          # the metric's relative-error bar, per statistic family and on the gradient
+SPEED_RATIO_TARGET = 3.0  # mixed / FP64 evaluations per second asked for
+HOST_THREADS = 16         # host threads one GPU's job gets on the GPU boxes
+HEADLINE = ROOT / "tests" / "golden" / "headline"
 
 
 def lognormal_map(nside, seed, base=32, sigma=0.1):
@@ -50,11 +54,13 @@ def family_errors(values, ref, plan):
     return out
 
 
-def run_engine(precision, nside, J, x, t, steps, warmup, timing):
+def run_engine(precision, nside, J, x, t, steps, warmup, timing, fields="auto", extra_cot=None):
+    """Timed forward + VJP steps of one engine; with extra_cot, one more VJP of that cotangent
+    on the last forward (for the parity report, after the timed region)."""
     import torch
     from stl_port import NativeEngine
 
-    eng = NativeEngine(nside, J, batch=1, backend="gpu", precision=precision, timing=timing)
+    eng = NativeEngine(nside, J, batch=1, backend="gpu", precision=precision, timing=timing, fields=fields)
     xd = torch.from_numpy(x).to("cuda:0").reshape(1, -1)
     td = torch.from_numpy(t).to("cuda:0").reshape(1, -1)
     target, _ = eng.forward_device(td, generation=0, objective="target")
@@ -77,6 +83,11 @@ def run_engine(precision, nside, J, x, t, steps, warmup, timing):
         step(warmup + i + 1)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    extra_grad = None
+    if extra_cot is not None:
+        g2 = torch.empty_like(xd)
+        eng.vjp_device(extra_cot.reshape(values.shape), g2, generation=warmup + steps)
+        extra_grad = g2.cpu().numpy().ravel()
     info1 = eng.info()
     timers = {}
     if timing:
@@ -85,8 +96,8 @@ def run_engine(precision, nside, J, x, t, steps, warmup, timing):
             timers[name] = {k: v[k] - v0[k] for k in ("ms", "launches", "bytes")}
     out = dict(elapsed=elapsed, values=values.copy(), grad=grad.cpu().numpy().ravel(), plan=eng.plan,
                setup_s=info1["setup_seconds"], device_bytes=info1["device_memory"]["total_bytes"],
-               peak_bytes=info1["device_memory"]["peak_bytes"], timers=timers,
-               conv_variants=info1["conv_variants"])
+               peak_bytes=info1["device_memory"]["peak_bytes"], fields_bytes=info1["device_memory"]["fields_bytes"],
+               fields=info1["fields"], timers=timers, conv_variants=info1["conv_variants"], extra_grad=extra_grad)
     eng.close()
     del xd, td, grad
     torch.cuda.empty_cache()
@@ -191,11 +202,19 @@ def roofline(timers, steps, workload=None):
             traffic = measured.get(name, {}).get("bytes_per_launch")
     total_bytes = sum(v["bytes"] for v in timers.values()) / steps
     total_ms = sum(v["ms"] for v in timers.values()) / steps
+    by_level = {}
+    for k, v in timers.items():
+        lev = k.rsplit("@L", 1)[-1] if "@L" in k else "-"
+        by_level[lev] = by_level.get(lev, 0.0) + v["ms"] / steps
     return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
             "bytes_per_launch": bytes_per_launch, "ms_per_launch": per_launch_ms,
             "eval_algorithmic_GB": round(total_bytes / 1e9, 3), "eval_kernel_ms": round(total_ms, 3),
-            "eval_GBps": round(total_bytes / (total_ms * 1e-3) / 1e9, 1)}
+            "eval_GBps": round(total_bytes / (total_ms * 1e-3) / 1e9, 1),
+            "kernel_ms_by_level": {k: round(v, 4) for k, v in sorted(by_level.items())},
+            "kernels": {k: {"ms_per_launch": round(v["ms"] / v["launches"], 4),
+                            "GBps": round(v["bytes"] / v["launches"] / (v["ms"] / v["launches"] * 1e-3) / 1e9, 1)}
+                        for k, v in sorted(timers.items()) if v["launches"]}}
 
 
 def compulsory_bytes(nside, J, elem_bytes, L=4):
@@ -223,39 +242,98 @@ def eval_roofline(nside, J, elem_bytes, evals_per_s, copy_gbps):
     return out
 
 
-def cpu_baseline(seconds_budget=30.0):
-    """The FP64 oracle (C++ restatement of the reference CPU engine, geometry
-    rebuilt at every application like its per-block rebuild) on the host cores,
-    one forward + VJP at nside 128 (J = 6, same recipe), scaled to nside 1024
-    by the pixel ratio 64."""
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def level_work(nside, J):
+    """Per-level work terms of one forward + VJP: sum_n N_n, sum_n N_n (n + 1) and
+    sum_n N_n (n + 1)^2 (per-pixel work independent of the level, work per branch row -- the
+    branch convolutions and moduli decimations -- and per pair of branch rows -- the S3/S4
+    products)."""
+    N = [12 * (nside >> n) ** 2 for n in range(J)]
+    return np.array([sum(N), sum(v * (n + 1) for n, v in enumerate(N)),
+                     sum(v * (n + 1) ** 2 for n, v in enumerate(N))], dtype=np.float64)
+
+
+def cpu_baseline():
+    """The FP64 oracle (the C++ restatement of the reference's CPU engine; its geometry rebuilt
+    at every application, like the reference's per-block rebuild, crates/stl-core/src/
+    engine.rs:837-862) on the host: forward + VJP of the lognormal maps timed at nside 64
+    (J 5), 128 (J 6) and 256 (J 7) on HOST_THREADS threads, and a per-level model
+    t = a sum N_n + b sum N_n (n+1) + c sum N_n (n+1)^2 solved on those three points gives the
+    nside 1024, J 9 rate (extrapolated, labelled so).  The reference's own per-core anchor is
+    timed too: nside 64, L = 2, J = 5 on one thread (3.26 s per evaluation on one EPYC 9655
+    core, docs/validation.md:162)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = min(HOST_THREADS, os.cpu_count() or 1)
+
+    def timed(nside, J, L=4, reps=1):
+        x, t = lognormal_map(nside, 1), lognormal_map(nside, 2)
+        e = oracle.Engine(nside, J, L=L, cache_rows=0)
+        e.load(t)
+        target = e.forward()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            e.load(x)
+            v = e.forward()
+            e.gradient(2.0 * (v - target))
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    oracle.set_threads(1)
+    anchor = timed(64, 5, L=2)
     oracle.set_threads(threads)
-    nside, J = 128, 6
-    x, t = lognormal_map(nside, 1), lognormal_map(nside, 2)
-    e = oracle.Engine(nside, J, cache_rows=0)
-    e.load(t)
-    target = e.forward()
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        e.load(x)
-        v = e.forward()
-        e.gradient(2.0 * (v - target))
-        n += 1
-        if time.perf_counter() - t0 > seconds_budget / 3 or n >= 3:
-            break
-    s_per_eval = (time.perf_counter() - t0) / n
-    scale = (1024 // nside) ** 2
-    return {"value": round(1.0 / (s_per_eval * scale), 6), "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"{n} forward+VJP of the FP64 oracle at nside {nside}, J={J} ({s_per_eval:.2f} s each), "
-                      f"scaled to nside 1024 by the pixel ratio {scale}"}
+    points = [(64, 5, timed(64, 5, reps=2)), (128, 6, timed(128, 6)), (256, 7, timed(256, 7))]
+    A = np.stack([level_work(n, j) for n, j, _ in points])
+    coef = np.linalg.solve(A, np.array([p[2] for p in points]))
+    s_eval = float(level_work(1024, 9) @ coef)
+    return {"value": round(1.0 / s_eval, 6), "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"forward+VJP of the FP64 oracle (geometry rebuilt per application) on {threads} threads of "
+                      f"{os.cpu_count()} ({cpu_model()}): measured "
+                      + ", ".join(f"nside {n} J {j}: {t:.2f} s" for n, j, t in points)
+                      + f"; nside 1024 J 9 extrapolated with a per-level model solved on them: {s_eval:.1f} s/eval",
+            "measured": [{"nside": n, "J": j, "s_per_eval": round(t, 3)} for n, j, t in points],
+            "extrapolated_nside1024_s_per_eval": round(s_eval, 2),
+            "cpu_model": cpu_model(), "os_cpu_count": os.cpu_count(),
+            "anchor": {"nside": 64, "L": 2, "J": 5, "threads": 1, "s_per_eval": round(anchor, 3),
+                       "reference_s_per_eval": 3.26, "reference_source": "docs/validation.md:162 (1 EPYC 9655 core)",
+                       "ratio": round(anchor / 3.26, 3)}}
+
+
+def headline_parity(values, grad, plan, nside, J):
+    """Mixed engine against the FP64 oracle's vectors of the same map (tests/golden/headline,
+    written by its make_vectors.py): per-family errors of the statistics and the errors of the
+    gradient of the vectors' cotangent, projected on 8 directions and at a pixel sample."""
+    case = HEADLINE / f"bench_n{nside}"
+    if not (case / "meta.json").is_file():
+        return None
+    meta = json.loads((case / "meta.json").read_text())
+    if meta["J"] != J:
+        return None
+    sys.path.insert(0, str(HEADLINE))
+    import recipe
+    ref = np.load(case / "values.npy")
+    out = {"statistics": family_errors(values.reshape(-1), ref, plan)}
+    proj = np.array([float(grad @ recipe.direction(nside, i)) for i in range(recipe.NDIR)])
+    rproj = np.load(case / "grad_proj.npy")
+    out["gradient_projections"] = float(np.abs(proj - rproj).max() / np.abs(rproj).max())
+    ids = np.load(case / "grad_ids.npy")
+    out["gradient_sample"] = float(np.abs(grad[ids] - np.load(case / "grad_sample.npy")).max() / meta["grad_max_abs"])
+    return out
 
 
 def copy_bandwidth(gib=2, reps=5):
-    """Measured HBM bandwidth of a device-to-device copy (bytes read + written / time)."""
+    """Measured HBM bandwidth of torch's device-to-device copy (bytes read + written / time)."""
     import torch
     n = (gib << 30) // 4
     a = torch.empty(n, dtype=torch.float32, device="cuda:0").uniform_()
@@ -273,6 +351,17 @@ def copy_bandwidth(gib=2, reps=5):
     return round(2.0 * n * 4 / (ms * 1e-3) / 1e9, 1)
 
 
+def run_2048(fields):
+    """The largest single-map configuration: nside 2048, J = 10, mixed precision, 5 timed steps."""
+    x2, t2 = lognormal_map(2048, 3), lognormal_map(2048, 4)
+    r2 = run_engine("mixed", 2048, 10, x2, t2, 5, 1, timing=False, fields=fields)
+    return {"nside": 2048, "J": 10, "precision": "mixed", "fields": r2["fields"],
+            "evals_per_s": round(5 / r2["elapsed"], 3), "ms_per_eval": round(1e3 * r2["elapsed"] / 5, 3),
+            "setup_s": round(r2["setup_s"], 3), "device_bytes": r2["device_bytes"],
+            "peak_device_bytes": r2["peak_bytes"], "fields_bytes": r2["fields_bytes"],
+            "coefficients": len(r2["plan"])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,7 +372,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--no-2048", action="store_true", help="skip the nside 2048 (J=10) single-map rate")
+    ap.add_argument("--no-2048", action="store_true", help="skip the nside 2048 (J=10) single-map rates")
     ap.add_argument("--synthesis-iters", type=int, default=20,
                     help="L-BFGS iterations of the synthesis measurement at the bench nside (0: skip)")
     args = ap.parse_args()
@@ -292,23 +381,28 @@ def main():
               f"MI355X, and the multi-GPU sky split is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
     import torch
+    from stl_port.native import stream_copy_gbps
     assert torch.cuda.is_available(), "bench.py needs a GPU"
 
     x, t = lognormal_map(args.nside, 1), lognormal_map(args.nside, 2)
+    # The oracle vectors' cotangent of this map (for the parity report, outside the timed steps).
+    sys.path.insert(0, str(HEADLINE))
+    import recipe
     # Throughput with the forward and the VJP replayed as HIP graphs; then the
     # same steps launched eagerly with HIP events around every kernel, for the
     # per-kernel durations of the roofline.
     mixed = run_engine("mixed", args.nside, args.J, x, t, args.steps, args.warmup, timing=False)
+    extra = recipe.cotangent(len(mixed["plan"]), 1)
     print(f"[bench] mixed: {args.steps / mixed['elapsed']:.2f} evals/s", file=sys.stderr, flush=True)
     tsteps = max(3, min(args.steps, 10))
-    mixed_t = run_engine("mixed", args.nside, args.J, x, t, tsteps, 1, timing=True)
+    mixed_t = run_engine("mixed", args.nside, args.J, x, t, tsteps, 1, timing=True, extra_cot=extra)
     print(f"[bench] mixed, per-kernel events: {tsteps / mixed_t['elapsed']:.2f} evals/s", file=sys.stderr,
           flush=True)
     fp64 = fp64_t = None
     if not args.no_fp64:
         fp64 = run_engine("fp64", args.nside, args.J, x, t, args.steps, args.warmup, timing=False)
         print(f"[bench] fp64: {args.steps / fp64['elapsed']:.2f} evals/s", file=sys.stderr, flush=True)
-        fp64_t = run_engine("fp64", args.nside, args.J, x, t, tsteps, 1, timing=True)
+        fp64_t = run_engine("fp64", args.nside, args.J, x, t, tsteps, 1, timing=True, extra_cot=extra)
     ensemble = None if args.no_ensemble else run_ensemble(max(3, args.steps // 2), 2)
     if ensemble:
         print(f"[bench] ensemble: {ensemble['evals_per_s']} evals/s", file=sys.stderr, flush=True)
@@ -317,16 +411,15 @@ def main():
         print(f"[bench] synthesis: {synthesis['ms_per_evaluation']} ms/evaluation", file=sys.stderr, flush=True)
     big = None
     if not args.no_2048:
-        # The largest single-map configuration: nside 2048, J = 10, mixed precision.
-        x2, t2 = lognormal_map(2048, 3), lognormal_map(2048, 4)
-        r2 = run_engine("mixed", 2048, 10, x2, t2, 5, 1, timing=False)
-        big = {"nside": 2048, "J": 10, "precision": "mixed", "evals_per_s": round(5 / r2["elapsed"], 3),
-               "ms_per_eval": round(1e3 * r2["elapsed"] / 5, 3), "setup_s": round(r2["setup_s"], 3),
-               "device_bytes": r2["device_bytes"], "coefficients": len(r2["plan"])}
-        print(f"[bench] nside 2048: {big['evals_per_s']} evals/s", file=sys.stderr, flush=True)
-        del x2, t2, r2
+        # cfg: nside 2048 with the pyramid levels recomputed in the reverse; the keep store beside it.
+        big = run_2048("recompute")
+        print(f"[bench] nside 2048 (recompute): {big['evals_per_s']} evals/s", file=sys.stderr, flush=True)
+        big["keep"] = run_2048("keep")
+        print(f"[bench] nside 2048 (keep): {big['keep']['evals_per_s']} evals/s", file=sys.stderr, flush=True)
     copy_gbps = copy_bandwidth()
-    print(f"[bench] copy bandwidth: {copy_gbps} GB/s", file=sys.stderr, flush=True)
+    stream_gbps = round(stream_copy_gbps(), 1)
+    print(f"[bench] copy bandwidth: torch {copy_gbps} GB/s, stream kernel {stream_gbps} GB/s", file=sys.stderr,
+          flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
 
     value = args.steps / mixed["elapsed"]
@@ -345,7 +438,8 @@ def main():
         "data": "synthetic: lognormal HEALPix NESTED maps (x seed 1, target seed 2)",
         "config": {"workload": f"ScatCov forward+VJP of sum|S(x)-S(t)|^2, HEALPix NESTED nside {args.nside}, "
                                f"J={args.J}, L=4, kernel 5, 1 map, {len(mixed['plan'])} coefficients",
-                   "nside": args.nside, "J": args.J, "L": 4, "batch": 1, "precision": "mixed"},
+                   "nside": args.nside, "J": args.J, "L": 4, "batch": 1, "precision": "mixed",
+                   "fields": mixed["fields"]},
         "roofline": roofline(mixed_t["timers"], tsteps, {"nside": args.nside, "J": args.J, "precision": "mixed"}),
         "cpu_baseline": cpu,
         "setup_s": round(mixed["setup_s"], 3),
@@ -358,18 +452,35 @@ def main():
         "synthesis": synthesis,
         "nside2048": big,
     }
-    line["roofline"]["copy_GBps_measured"] = copy_gbps
-    line["roofline"]["frac_of_measured_copy"] = round(line["roofline"]["achieved"] / copy_gbps, 4)
-    line["eval_roofline"] = eval_roofline(args.nside, args.J, 4, value, copy_gbps)
+    r = line["roofline"]
+    r["copy_GBps_measured"] = copy_gbps
+    r["stream_copy_GBps_measured"] = stream_gbps
+    r["frac_of_measured_copy"] = round(r["achieved"] / max(copy_gbps, stream_gbps), 4)
+    attainable = max(copy_gbps, stream_gbps)
+    line["eval_roofline"] = eval_roofline(args.nside, args.J, 4, value, attainable)
     if big is not None:
-        big["eval_roofline"] = eval_roofline(2048, 10, 4, big["evals_per_s"], copy_gbps)
+        big["eval_roofline"] = eval_roofline(2048, 10, 4, big["evals_per_s"], attainable)
+        big["keep"]["eval_roofline"] = eval_roofline(2048, 10, 4, big["keep"]["evals_per_s"], attainable)
+    # Mixed engine against the FP64 oracle's vectors of this map, and the bar of the metric.
+    vs_oracle = headline_parity(mixed_t["values"], mixed_t["extra_grad"], mixed["plan"], args.nside, args.J)
+    if vs_oracle is not None:
+        line["mixed_error_vs_oracle"] = vs_oracle
+        worst_grad = max(vs_oracle["gradient_projections"], vs_oracle["gradient_sample"])
+        line["error_bar"] = {"bar": ERROR_BAR, "against": "FP64 oracle vectors of the bench map",
+                             "statistics_within": max(vs_oracle["statistics"].values()) <= ERROR_BAR,
+                             "gradient_within": worst_grad <= ERROR_BAR}
     if fp64 is not None:
         v64 = args.steps / fp64["elapsed"]
         line["fp64"] = {"value": round(v64, 3), "ms_per_step": round(1e3 * fp64["elapsed"] / args.steps, 3),
                         "roofline": roofline(fp64_t["timers"], tsteps), "setup_s": round(fp64["setup_s"], 3),
                         "device_bytes": fp64["device_bytes"],
-                        "eval_roofline": eval_roofline(args.nside, args.J, 8, v64, copy_gbps)}
+                        "eval_roofline": eval_roofline(args.nside, args.J, 8, v64, attainable)}
+        p64 = headline_parity(fp64_t["values"], fp64_t["extra_grad"], fp64["plan"], args.nside, args.J)
+        if p64 is not None:
+            line["fp64"]["error_vs_oracle"] = p64
         line["mixed_over_fp64"] = round(value / v64, 3)
+        line["mixed_over_fp64_target"] = SPEED_RATIO_TARGET
+        line["mixed_over_fp64_meets_target"] = value / v64 >= SPEED_RATIO_TARGET
         line["mixed_error_vs_fp64"] = family_errors(mixed["values"], fp64["values"], mixed["plan"])
         g64 = fp64["grad"]
         line["mixed_error_vs_fp64"]["gradient"] = float(np.abs(mixed["grad"] - g64).max() / np.abs(g64).max())

@@ -171,8 +171,11 @@ int stlp_train_front(void* h, const double* x, size_t n, bool b, double* out);
  * stlp_lbfgs.  Option "ensemble": true makes the batch one ensemble (the
  * reference's stl-synth --ensemble, synthesis.rs:362-470 without its scatter
  * term): loss = sum_k w_k |mean over the members of S_k - target_k|^2 with
- * target one row [K, ncoef].  The loss of every evaluation goes
- * to loss_history (up to cap).  Returns the number of evaluations or -1. */
+ * target one row [K, ncoef].  Option "history": "evaluations" (default: the
+ * loss of every evaluation goes to loss_history, up to cap) or "iterations"
+ * (loss_history[0] the start's loss, then the loss at each iteration's
+ * accepted step; NaN after the last).  Returns the number of evaluations or
+ * -1. */
 int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* target, const double* weights,
                         const char* options, double* loss_history, size_t cap);
 
@@ -202,6 +205,22 @@ int stlx_ensemble_vjp_device(StlEnsemble* e, const double* cot, size_t n, const 
  * 1 conv adjoint (gy -> gx), 2 decimation forward level -> level + 1,
  * 3 decimation adjoint.  Arrays are float64 on the device. */
 int stlx_apply_operator(StlHandle* h, int kind, int level, int rows, uint64_t in_dev, uint64_t out_dev);
+/* stlx_apply_operator without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * The staging of the operator's input and output lives in a static device
+ * scratch of 64 MiB per GPU (held from the library's load until the process
+ * ends); an application whose staging exceeds it fails with an error naming
+ * the sizes.  Every call on one GPU shares that scratch: calls that overlap
+ * on one GPU race.  Results equal stlx_apply_operator's bit for bit. */
+int stlx_apply_operator_noalloc(StlHandle* h, int kind, int level, int rows, uint64_t in_dev, uint64_t out_dev);
+
+/* Measurement hook (no counterpart in the reference): mean milliseconds of one
+ * copy of `bytes` (a multiple of 16; 16-byte aligned device addresses) from
+ * src_dev to dst_dev by this library's hand-written stream kernel, over `reps`
+ * copies after a warm-up, to *ms.  The attainable HBM rate is
+ * 2 * bytes / ms.  0 or -1. */
+int stlx_stream_copy(uint64_t src_dev, uint64_t dst_dev, size_t bytes, int reps, double* ms);
 
 /* Map files and initial maps (no counterpart in the reference's C ABI: they
  * restate the Rust I/O of its command-line tools, crates/stl-mpi/src/io.rs:18-520,

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <optional>
 #include <sstream>
 #include <string>
@@ -139,6 +140,39 @@ std::string json_opt(int v) { return v < 0 ? "null" : std::to_string(v); }
 
 }  // namespace
 
+// Static device scratch of the allocation-free entry point
+// (stlx_apply_operator_noalloc): the staging of one operator application, at
+// most kScratchBytes.  Each device holds its own copy from the library's load
+// until the process ends; host code uses only the per-device address from
+// hipGetSymbolAddress (scratch_address), never the symbol itself.  Calls that
+// overlap on one device share it and race.
+namespace {
+constexpr size_t kScratchBytes = size_t(64) << 20;
+}
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+// Address of `scratch` on a device, looked up once per device with that device
+// current (at handle creation, the library's set-up of a device).
+void* scratch_address(int device) {
+  static std::mutex lock;
+  static std::map<int, void*> addresses;
+  std::lock_guard<std::mutex> guard_lock(lock);
+  auto it = addresses.find(device);
+  if (it != addresses.end()) return it->second;
+  int current = 0;
+  STL_HIP(hipGetDevice(&current));
+  STL_HIP(hipSetDevice(device));
+  void* ptr = nullptr;
+  const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  STL_HIP(hipSetDevice(current));
+  if (err != hipSuccess || ptr == nullptr)
+    throw std::runtime_error(std::string("scratch address: ") + hipGetErrorString(err));
+  addresses.emplace(device, ptr);
+  return ptr;
+}
+}  // namespace
+
 struct StlHandle {
   std::unique_ptr<stl::Engine> engine;
   uint64_t id = 0;
@@ -224,10 +258,19 @@ StlHandle* stlp_new(const char* config) {
     opt.fp64 = precision == "fp64";
     opt.device = int(v.integer("device", 0));
     opt.timing = v.boolean("timing", false);
+    // The reference's field store key (lib.rs:277, :340-341): auto | keep | recompute.
+    const std::string fields = v.string("fields", "auto");
+    if (fields == "keep") opt.fields = stl::EngineOptions::kKeep;
+    else if (fields == "recompute") opt.fields = stl::EngineOptions::kRecompute;
+    else if (fields == "auto") opt.fields = stl::EngineOptions::kAuto;
+    else fail("fields: auto, keep or recompute");
     opt.conv_variant = int(v.integer("conv_variant", -1));
+    opt.queue_nside = int(v.integer("queue_nside", 32));
+    if (opt.queue_nside < 0) fail("queue_nside must be nonnegative");
     if (opt.conv_variant < -1 || opt.conv_variant > 5) fail("conv_variant must be -1 or 0 to 5");
     auto h = std::make_unique<StlHandle>();
     h->engine = stl::Engine::create(opt);
+    (void)scratch_address(opt.device);
     h->id = g_next_id.fetch_add(1);
     h->nside = int(nside);
     h->batch = int(batch);
@@ -271,9 +314,12 @@ char* stlp_info(StlHandle* h) {
       << ", \"cross\": " << (h->channels == 2 ? "true" : "false") << ", \"fewer\": false, \"backend\": \"gpu\""
       << ", \"precision\": \"" << h->precision << "\", \"dtype\": \""
       << (h->precision == "fp64" ? "float64/complex128" : "float32 fields and operators, float64 sums") << "\""
-      << ", \"masks\": " << h->engine->masks() << ", \"spin\": 0, \"conv\": \"folded gather, 4-fold face symmetry\", \"fields\": \"keep\"";
+      << ", \"masks\": " << h->engine->masks() << ", \"spin\": 0, \"conv\": \"folded gather, 4-fold face symmetry\", \"fields\": \""
+      << (h->engine->field_store() == stl::EngineOptions::kRecompute ? "recompute" : "keep") << "\""
+      << ", \"queue_level\": " << h->engine->queue_level();
     o << ", \"device_memory\": {\"total_bytes\": " << h->engine->device_bytes()
-      << ", \"peak_bytes\": " << stl::device_bytes_peak() << "}";
+      << ", \"peak_bytes\": " << stl::device_bytes_peak() << ", \"fields_bytes\": " << h->engine->field_bytes()
+      << "}";
     o << ", \"setup_seconds\": " << h->engine->setup_seconds();
     o << ", \"conv_variants\": [";
     {
@@ -510,6 +556,13 @@ int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* ta
     Json v = Json::parse(options);
     stl::LbfgsOptions opt = lbfgs_options(v);
     const bool ensemble = v.boolean("ensemble", false);
+    // "history": "evaluations" (the loss of every evaluation) or "iterations"
+    // (the start's loss, then the loss at each iteration's accepted step).
+    const std::string record = v.string("history", "evaluations");
+    if (record != "evaluations" && record != "iterations") fail("history: evaluations or iterations");
+    const bool per_iteration = record == "iterations";
+    if (loss_history && per_iteration)
+      for (size_t i = 0; i < cap; ++i) loss_history[i] = std::nan("");
     h->current.reset();
     const size_t nc = h->nmask_rows() * h->ncoef(), nv = size_t(h->batch) * nc;
     std::vector<stl::cplx> values(nv), cot(nv);
@@ -552,11 +605,15 @@ int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* ta
         if (!std::isfinite(loss)) fail("non-finite loss");
         h->engine->vjp_device(cot.data(), h->gbuf.data());
         STL_HIP(hipMemcpyAsync(g.p, h->gbuf.data(), n * sizeof(double), hipMemcpyDeviceToDevice, st));
-        if (loss_history && evals < cap) loss_history[evals] = loss;
+        if (loss_history && !per_iteration && evals < cap) loss_history[evals] = loss;
         ++evals;
         return loss;
       };
       stl::Lbfgs<stl::VectorOps> lb(ops, opt);
+      if (loss_history && per_iteration)
+        lb.on_iteration = [&](int it, double loss) {
+          if (size_t(it) < cap) loss_history[it] = loss;
+        };
       lb.minimize(x, f);
       STL_HIP(hipMemcpyAsync(reinterpret_cast<double*>(x_dev), x.p, n * sizeof(double), hipMemcpyDeviceToDevice,
                              h->engine->stream()));
@@ -627,6 +684,20 @@ int stlx_ensemble_vjp_device(StlEnsemble* e, const double* cot, size_t n, const 
       g[k] = reinterpret_cast<double*>(grad_dev[k]);
     }
     e->ensemble->vjp(reinterpret_cast<const stl::cplx*>(cot), g.data());
+    return 0;
+  });
+}
+
+int stlx_apply_operator_noalloc(StlHandle* h, int kind, int level, int rows, uint64_t in_dev, uint64_t out_dev) {
+  return guard(-1, [&] {
+    if (!h || !in_dev || !out_dev) fail("null argument");
+    const size_t need = h->engine->operator_workspace_bytes(kind, level, rows);
+    if (need > kScratchBytes)
+      fail("stlx_apply_operator_noalloc: the operator's staging needs " + std::to_string(need) +
+           " bytes, more than the " + std::to_string(kScratchBytes) + "-byte static scratch");
+    void* ws = scratch_address(h->engine->options().device);
+    h->engine->apply_operator_in(kind, level, rows, reinterpret_cast<const double*>(in_dev),
+                                 reinterpret_cast<double*>(out_dev), ws, kScratchBytes);
     return 0;
   });
 }
@@ -738,6 +809,23 @@ int stlx_pixel_normal(uint64_t seed, int64_t first, size_t n, double* out) {
   return guard(-1, [&] {
     if (!out && n) fail("null argument");
     for (size_t i = 0; i < n; ++i) out[i] = stl::io::pixel_normal(seed, first + int64_t(i));
+    return 0;
+  });
+}
+
+int stlx_stream_copy(uint64_t src_dev, uint64_t dst_dev, size_t bytes, int reps, double* ms) {
+  return guard(-1, [&] {
+    if (!src_dev || !dst_dev || !ms) fail("null argument");
+    hipStream_t s = nullptr;
+    STL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    try {
+      *ms = stl::stream_copy_ms(reinterpret_cast<const void*>(src_dev), reinterpret_cast<void*>(dst_dev), bytes,
+                                reps, s);
+    } catch (...) {
+      (void)hipStreamDestroy(s);
+      throw;
+    }
+    STL_HIP(hipStreamDestroy(s));
     return 0;
   });
 }

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -271,11 +272,25 @@ template <typename T>
 class EngineT final : public Engine {
   using C2 = cplx_t<T>;
 
+  // A task list of the work queue on the device, ready to launch.
+  struct QueueRun {
+    DevArray<QueueTask<T>> tasks;
+    int ntasks = 0;
+    int64_t items = 0;
+    size_t lds = 0;
+    double bytes = 0.0;  // algorithmic bytes of its tasks
+    std::string name;
+  };
+
   // The level fields (forward state and reverse scratch), shared by the
   // engines made by share(); holder: the engine whose forward they hold.
   struct FieldSet {
     int rows = 0;  // field rows the buffers are sized for
     uint64_t holder = 0;
+    // Recompute store: z and br hold one level at a time (sized for level 0),
+    // and every level's reverse uses gbr, gzp and acc[n & 1].
+    bool recompute = false;
+    size_t bytes = 0;  // device bytes of the fields and reverse buffers
     std::vector<DevArray<T>> maps, M;
     std::vector<DevArray<C2>> z, br;
     // Reverse buffers: gbr / gzp / acc[0] hold level 0, gbr1 / gzp1 / acc[1]
@@ -284,6 +299,11 @@ class EngineT final : public Engine {
     // coarser levels' reverse.
     DevArray<C2> gbr, gzp, gz, gbr1, gzp1, gbr2, gzp2;
     DevArray<T> acc[4], mg[2];
+    // Reverse buffers of every work-queue level (from level qc on), so that
+    // the queue's levels overlap: gV, z part, chain, gz and map cotangent.
+    int qc = 0;
+    std::vector<DevArray<C2>> qgbr, qgzp, qgz;
+    std::vector<DevArray<T>> qchain, qmg;
   };
 
  public:
@@ -328,6 +348,18 @@ class EngineT final : public Engine {
       if (n + 1 < levels_) dec_.push_back(shared_dec<T>(opt.device, order, s_));
     }
     geometry_bytes_ = device_bytes_live();
+    qc_ = levels_;
+    if (opt.queue_nside > 0)
+      for (int n = 2; n < levels_; ++n)
+        if ((opt.nside >> n) <= opt.queue_nside) {
+          qc_ = n;
+          break;
+        }
+    int cus = 0;
+    STL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, opt.device));
+    q_workgroups_ = std::max(1, 2 * cus);
+    hqerr_.resize(1);
+    hqerr_.data()[0] = 0u;
     allocate();
     tune();
     // The timing launches of tune() wrote the (possibly shared) fields.
@@ -430,6 +462,8 @@ class EngineT final : public Engine {
   }
   hipStream_t stream() const override { return s_; }
   size_t device_bytes() const override { return device_bytes_live(); }
+  int field_store() const override { return f_->recompute ? EngineOptions::kRecompute : EngineOptions::kKeep; }
+  size_t field_bytes() const override { return f_->bytes; }
   double setup_seconds() const override { return setup_s_; }
   std::vector<KernelTimer> timers() const override { return prof_.timers(); }
   std::vector<std::array<int, 4>> conv_variants() const override { return conv_var_; }
@@ -485,11 +519,22 @@ class EngineT final : public Engine {
   // levels J - 1 .. 1; rev_chain(0) needs both.
   void sweep_reverse(double* grad_dev) {
     rev_prelude();
-    for (int n = levels_ - 1; n >= 0; --n) {
+    rev_levels(0);
+    rev_finish(grad_dev);
+  }
+
+  // The reverse of levels J - 1 .. down_to: the work queue's levels in one
+  // launch, then the others level by level.
+  void rev_levels(int down_to) {
+    int n = levels_ - 1;
+    if (queue_on() && qc_ >= down_to) {
+      run_queue(qrev_);
+      n = qc_ - 1;
+    }
+    for (; n >= down_to; --n) {
       rev_branch(n);
       rev_chain(n);
     }
-    rev_finish(grad_dev);
   }
 
   void rev_prelude() {
@@ -516,18 +561,55 @@ class EngineT final : public Engine {
   }
 
   // Reverse buffers of level n (levels 0 and 1 apart from the others).
-  C2* gbr_of(int n) { return n == 0 ? f_->gbr.data() : n == 1 ? f_->gbr1.data() : f_->gbr2.data(); }
-  C2* gzp_of(int n) { return n == 0 ? f_->gzp.data() : n == 1 ? f_->gzp1.data() : f_->gzp2.data(); }
-  T* chain_of(int n) { return f_->acc[n < 2 ? n : 2 + (n & 1)].data(); }
+  // Level-n buffers.  Keep store: z and branches of every level; the reverse
+  // buffers of levels 0 and 1 apart from the others.  Recompute store: one
+  // level's z, branches and reverse buffers at a time.
+  C2* z_of(int n) { return f_->z[f_->recompute ? 0 : n].data(); }
+  C2* br_of(int n) { return f_->br[f_->recompute ? 0 : n].data(); }
+  // The work queue's levels (n >= qc_) have reverse buffers of their own.
+  bool queued(int n) const { return n >= qc_ && queue_on(); }
+  C2* gbr_of(int n) {
+    if (queued(n)) return f_->qgbr[n].data();
+    return n == 0 || f_->recompute ? f_->gbr.data() : n == 1 ? f_->gbr1.data() : f_->gbr2.data();
+  }
+  C2* gzp_of(int n) {
+    if (queued(n)) return f_->qgzp[n].data();
+    return n == 0 || f_->recompute ? f_->gzp.data() : n == 1 ? f_->gzp1.data() : f_->gzp2.data();
+  }
+  T* chain_of(int n) {
+    if (queued(n)) return f_->qchain[n].data();
+    return f_->acc[f_->recompute ? (n & 1) : n < 2 ? n : 2 + (n & 1)].data();
+  }
+  C2* gz_of(int n) { return queued(n) ? f_->qgz[n].data() : f_->gz.data(); }
+  T* mg_of(int n) { return queued(n) ? f_->qmg[n].data() : f_->mg[n % 2].data(); }
+
+  // The work queue serves the keep store's one-channel, unmasked sweeps.
+  bool queue_on() const { return qc_ < levels_ && !cross_ && nmask_ == 0 && !f_->recompute; }
+
+  // Recompute store: level n's z (with its modulus row, the same bits as the
+  // forward's) and branches again, from the kept maps and moduli.
+  void recompute_level(int n) {
+    const int64_t N = N_[n];
+    int id = prof_.start();
+    launch_conv_fwd<T>(conv_[n]->store.view(), 1, f_->maps[n].data(), B_, z_of(n),
+                       f_->M[n].data() + size_t(n) * B_ * N * kL, s_, conv_var_[n][0]);
+    prof_.stop(id, tag("recompute_z", n), conv_bytes(n, 1, B_, false) + double(B_) * N * kL * sizeof(T));
+    id = prof_.start();
+    launch_conv_fwd<T>(conv_[n]->store.view(), kL, f_->M[n].data(), (n + 1) * B_, br_of(n), nullptr, s_,
+                       conv_var_[n][1]);
+    prof_.stop(id, tag("recompute_branch", n), conv_bytes(n, kL, (n + 1) * B_, false));
+  }
 
   void rev_branch(int n) {
     const int64_t N = N_[n];
     const MaskView mv = mask_view(n);
+    if (f_->recompute) recompute_level(n);
     const double2* H = nmask_ > 0 ? Hc_[n].data() : dH_[n].data();
     int id = prof_.start();
-    if (cross_) launch_gv_x<T>(n, N, Bs_, f_->br[n].data(), f_->z[n].data(), H, gbr_of(n), gzp_of(n), mv, s_);
-    else launch_gv<T>(n, N, B_, f_->br[n].data(), f_->z[n].data(), H, gbr_of(n), gzp_of(n), mv, s_);
-    prof_.stop(id, tag("gv", n), double((n + 2) * B_ * kL) * N * kL * 2 * sizeof(T) * 2);
+    if (cross_) launch_gv_x<T>(n, N, Bs_, br_of(n), z_of(n), H, gbr_of(n), gzp_of(n), mv, s_);
+    else launch_gv<T>(n, N, B_, br_of(n), z_of(n), H, gbr_of(n), gzp_of(n), mv, s_);
+    // V (branches (n + 1) L x L and z L complex values per pixel) read, gV written.
+    prof_.stop(id, tag("gv", n), double(B_) * N * ((n + 1) * kL * kL + kL) * 2 * sizeof(T) * 2);
     conv_adj(n, kL, gbr_of(n), (n + 1) * B_, chain_of(n));
   }
 
@@ -541,16 +623,16 @@ class EngineT final : public Engine {
     const double* s1 = nmask_ > 0 ? s1c_[n].data() : ds1_[n].data();
     const double* s2 = nmask_ > 0 ? s2c_[n].data() : ds2_[n].data();
     if (cross_)
-      launch_gz_x<T>(n, N, Bs_, f_->z[n].data(), gzp_of(n), chain, reinterpret_cast<const double2*>(s1), f_->gz.data(),
+      launch_gz_x<T>(n, N, Bs_, z_of(n), gzp_of(n), chain, reinterpret_cast<const double2*>(s1), gz_of(n),
                      mv, s_);
     else
-      launch_gz<T>(n, N, B_, f_->z[n].data(), gzp_of(n), chain, s1, s2, f_->gz.data(), mv, s_);
-    prof_.stop(id, tag("gz", n), double(B_) * kL * N * (6 * sizeof(T)));
+      launch_gz<T>(n, N, B_, z_of(n), gzp_of(n), chain, s1, s2, gz_of(n), mv, s_);
+    // z, its gV part and gz (complex) and the chain row (real), L each per pixel.
+    prof_.stop(id, tag("gz", n), double(B_) * kL * N * (7 * sizeof(T)));
 
-    T* mg = f_->mg[n % 2].data();
-    T* mg_prev = f_->mg[(n + 1) % 2].data();
-    conv_adj(n, 1, f_->gz.data(), B_, mg);
-    if (n + 1 < levels_) dec_adj(n, 1, mg_prev, mg, B_, mg);
+    T* mg = mg_of(n);
+    conv_adj(n, 1, gz_of(n), B_, mg);
+    if (n + 1 < levels_) dec_adj(n, 1, mg_of(n + 1), mg, B_, mg);
   }
 
   void rev_finish(double* grad_dev) {
@@ -560,32 +642,59 @@ class EngineT final : public Engine {
     cast_out(f_->mg[0].data(), grad_dev, int64_t(B_) * N_[0]);
   }
 
-  void apply_operator(int kind, int n, int rows, const double* in, double* out) override {
+  // Staging of apply_operator: the operator's input and output at the
+  // engine's precision, each 256-byte aligned.
+  static size_t aligned(size_t b) { return (b + 255) / 256 * 256; }
+  std::array<size_t, 2> operator_staging(int kind, int n, int rows) const {
     STL_CHECK(n >= 0 && n < levels_, "level out of range");
     STL_CHECK(rows >= 1, "rows must be positive");
+    STL_CHECK(kind >= 0 && kind <= 3, "unknown operator kind");
+    const size_t N = size_t(N_[n]);
+    if (kind == 0) return {size_t(rows) * N * sizeof(T), size_t(rows) * kL * N * sizeof(C2)};
+    if (kind == 1) return {size_t(rows) * kL * N * sizeof(C2), size_t(rows) * N * sizeof(T)};
+    STL_CHECK(n + 1 < levels_, "no coarser level");
+    const size_t Nc = size_t(N_[n + 1]);
+    return {size_t(rows) * (kind == 2 ? N : Nc) * sizeof(T), size_t(rows) * (kind == 2 ? Nc : N) * sizeof(T)};
+  }
+
+  size_t operator_workspace_bytes(int kind, int n, int rows) const override {
+    const auto st = operator_staging(kind, n, rows);
+    return aligned(st[0]) + aligned(st[1]);
+  }
+
+  void apply_operator(int kind, int n, int rows, const double* in, double* out) override {
+    DevArray<unsigned char> ws(operator_workspace_bytes(kind, n, rows));
+    apply_operator_in(kind, n, rows, in, out, ws.data(), ws.size());
+  }
+
+  void apply_operator_in(int kind, int n, int rows, const double* in, double* out, void* workspace,
+                         size_t workspace_bytes) override {
+    const auto st = operator_staging(kind, n, rows);
+    STL_CHECK(workspace != nullptr && reinterpret_cast<uintptr_t>(workspace) % 256 == 0,
+              "operator workspace: null or not 256-byte aligned");
+    STL_CHECK(aligned(st[0]) + aligned(st[1]) <= workspace_bytes, "operator workspace too small");
+    unsigned char* a = static_cast<unsigned char*>(workspace);
+    unsigned char* b = a + aligned(st[0]);
     const int64_t N = N_[n];
     if (kind == 0) {
-      DevArray<T> x(size_t(rows) * N);
-      DevArray<C2> y(size_t(rows) * kL * N);
-      cast_in(in, x.data(), int64_t(rows) * N);
-      launch_conv_fwd<T>(conv_[n]->store.view(), 1, x.data(), rows, y.data(), nullptr, s_);
-      cast_out(reinterpret_cast<const T*>(y.data()), out, int64_t(rows) * kL * N * 2);
+      T* x = reinterpret_cast<T*>(a);
+      C2* y = reinterpret_cast<C2*>(b);
+      cast_in(in, x, int64_t(rows) * N);
+      launch_conv_fwd<T>(conv_[n]->store.view(), 1, x, rows, y, nullptr, s_);
+      cast_out(reinterpret_cast<const T*>(y), out, int64_t(rows) * kL * N * 2);
     } else if (kind == 1) {
-      DevArray<C2> gy(size_t(rows) * kL * N);
-      DevArray<T> gx(size_t(rows) * N);
-      cast_in(in, reinterpret_cast<T*>(gy.data()), int64_t(rows) * kL * N * 2);
-      launch_conv_adj<T>(conv_[n]->store.view(), 1, gy.data(), rows, gx.data(), s_);
-      cast_out(gx.data(), out, int64_t(rows) * N);
-    } else if (kind == 2 || kind == 3) {
-      STL_CHECK(n + 1 < levels_, "no coarser level");
-      const int64_t Nc = N_[n + 1];
-      DevArray<T> a(size_t(rows) * (kind == 2 ? N : Nc)), b(size_t(rows) * (kind == 2 ? Nc : N));
-      cast_in(in, a.data(), int64_t(a.size()));
-      if (kind == 2) launch_dec_fwd<T>(dec_[n]->store.view(), 1, a.data(), rows, b.data(), s_);
-      else launch_dec_adj<T>(dec_[n]->store.view(), 1, a.data(), nullptr, rows, b.data(), s_);
-      cast_out(b.data(), out, int64_t(b.size()));
+      C2* gy = reinterpret_cast<C2*>(a);
+      T* gx = reinterpret_cast<T*>(b);
+      cast_in(in, reinterpret_cast<T*>(gy), int64_t(rows) * kL * N * 2);
+      launch_conv_adj<T>(conv_[n]->store.view(), 1, gy, rows, gx, s_);
+      cast_out(gx, out, int64_t(rows) * N);
     } else {
-      throw std::runtime_error("unknown operator kind");
+      T* x = reinterpret_cast<T*>(a);
+      T* y = reinterpret_cast<T*>(b);
+      cast_in(in, x, int64_t(st[0] / sizeof(T)));
+      if (kind == 2) launch_dec_fwd<T>(dec_[n]->store.view(), 1, x, rows, y, s_);
+      else launch_dec_adj<T>(dec_[n]->store.view(), 1, x, nullptr, rows, y, s_);
+      cast_out(y, out, int64_t(st[1] / sizeof(T)));
     }
     STL_HIP(hipStreamSynchronize(s_));
   }
@@ -598,11 +707,15 @@ class EngineT final : public Engine {
       out.resize(at + bytes);
       STL_HIP(hipMemcpy(out.data() + at, d, bytes, hipMemcpyDeviceToHost));
     };
+    // The recompute store's state is the maps and moduli (z and the branches
+    // are recomputed by the reverse).
     for (int n = 0; n < levels_; ++n) {
       put(f_->maps[n].data(), f_->maps[n].bytes());
-      put(f_->z[n].data(), f_->z[n].bytes());
       put(f_->M[n].data(), f_->M[n].bytes());
-      put(f_->br[n].data(), f_->br[n].bytes());
+      if (!f_->recompute) {
+        put(f_->z[n].data(), f_->z[n].bytes());
+        put(f_->br[n].data(), f_->br[n].bytes());
+      }
     }
     return out;
   }
@@ -617,9 +730,11 @@ class EngineT final : public Engine {
     };
     for (int n = 0; n < levels_; ++n) {
       get(f_->maps[n].data(), f_->maps[n].bytes());
-      get(f_->z[n].data(), f_->z[n].bytes());
       get(f_->M[n].data(), f_->M[n].bytes());
-      get(f_->br[n].data(), f_->br[n].bytes());
+      if (!f_->recompute) {
+        get(f_->z[n].data(), f_->z[n].bytes());
+        get(f_->br[n].data(), f_->br[n].bytes());
+      }
     }
     STL_CHECK(at == s.size(), "checkpoint size mismatch");
   }
@@ -632,11 +747,15 @@ class EngineT final : public Engine {
     if (f_) {
       STL_CHECK(f_->rows >= B_ && int(f_->maps.size()) == levels_,
                 "share: more maps than the shared fields hold (or another pyramid)");
+      if (f_->recompute || int(f_->qmg.size()) != levels_) qc_ = levels_;
+      else qc_ = f_->qc;
       alloc_sums();
       return;
     }
     f_ = std::make_shared<FieldSet>();
     f_->rows = B_;
+    f_->recompute = choose_recompute();
+    const size_t live0 = device_bytes_live();
     f_->maps.resize(levels_);
     f_->z.resize(levels_);
     f_->M.resize(levels_);
@@ -644,15 +763,25 @@ class EngineT final : public Engine {
     for (int n = 0; n < levels_; ++n) {
       int64_t N = N_[n];
       f_->maps[n].resize(size_t(B_) * N);
-      f_->z[n].resize(size_t(B_) * kL * N);
       f_->M[n].resize(size_t(n + 1) * B_ * N * kL);
-      f_->br[n].resize(size_t(n + 1) * B_ * kL * N * kL);
+      if (n == 0 || !f_->recompute) {
+        f_->z[n].resize(size_t(B_) * kL * N);
+        f_->br[n].resize(size_t(n + 1) * B_ * kL * N * kL);
+      }
     }
     // Level 0 has the largest branch, z and modulus fields: (n + 1) / 4^n <= 1.
     f_->gbr.resize(f_->br[0].size());
     f_->gzp.resize(f_->z[0].size());
     f_->gz.resize(f_->z[0].size());
     f_->acc[0].resize(f_->M[0].size());
+    f_->mg[0].resize(f_->maps[0].size());
+    f_->mg[1].resize(f_->maps[0].size());
+    if (f_->recompute) {
+      if (levels_ > 1) f_->acc[1].resize(f_->M[0].size());
+      f_->bytes = device_bytes_live() - live0;
+      alloc_sums();
+      return;
+    }
     if (levels_ > 1) {
       f_->gbr1.resize(f_->br[1].size());
       f_->gzp1.resize(f_->z[1].size());
@@ -666,9 +795,48 @@ class EngineT final : public Engine {
       f_->acc[2].resize(f_->M[2].size());
       f_->acc[3].resize(f_->M[2].size());
     }
-    f_->mg[0].resize(f_->maps[0].size());
-    f_->mg[1].resize(f_->maps[0].size());
+    f_->qc = qc_;
+    f_->qgbr.resize(levels_);
+    f_->qgzp.resize(levels_);
+    f_->qgz.resize(levels_);
+    f_->qchain.resize(levels_);
+    f_->qmg.resize(levels_);
+    for (int n = qc_; n < levels_; ++n) {
+      f_->qgbr[n].resize(f_->br[n].size());
+      f_->qgzp[n].resize(f_->z[n].size());
+      f_->qgz[n].resize(f_->z[n].size());
+      f_->qchain[n].resize(f_->M[n].size());
+      f_->qmg[n].resize(f_->maps[n].size());
+    }
+    f_->bytes = device_bytes_live() - live0;
     alloc_sums();
+  }
+
+  // Bytes of the keep store's fields and reverse buffers (what allocate() makes).
+  size_t keep_field_bytes() const {
+    auto len = [&](int n) { return n < levels_ ? size_t(N_[n]) : size_t(0); };
+    size_t e = 0, c = 0;  // real and complex elements
+    for (int n = 0; n < levels_; ++n) {
+      e += size_t(B_) * len(n) * (1 + (n + 1) * kL);
+      c += size_t(B_) * len(n) * kL * (1 + (n + 1) * kL);
+    }
+    size_t br2 = 0;
+    for (int n = 2; n < levels_; ++n) br2 = std::max(br2, size_t(n + 1) * B_ * kL * len(n) * kL);
+    c += size_t(B_) * kL * kL * len(0) + 2 * size_t(B_) * kL * len(0) + 2 * size_t(B_) * kL * kL * len(1) +
+         size_t(B_) * kL * len(1) + br2 + size_t(B_) * kL * len(2);
+    e += size_t(B_) * kL * (len(0) + 2 * len(1) + 6 * len(2)) + 2 * size_t(B_) * len(0);
+    return e * sizeof(T) + c * sizeof(C2);
+  }
+
+  // The field store: forced, or (auto) keep when the kept fields take at most
+  // half of the free device memory, as the reference's FieldStore::Auto
+  // (crates/stl-gpu/src/engine.rs:574-578).
+  bool choose_recompute() const {
+    if (opt_.fields == EngineOptions::kKeep) return false;
+    if (opt_.fields == EngineOptions::kRecompute) return true;
+    size_t free_b = 0, total_b = 0;
+    STL_HIP(hipMemGetInfo(&free_b, &total_b));
+    return keep_field_bytes() > free_b / 2;
   }
 
   // Sums, reverse operands and statistics staging: [B][Km] rows of each.
@@ -723,6 +891,7 @@ class EngineT final : public Engine {
         nd[size_t(k) * levels_ + n] = nmask_ > 0 ? double(mcount_[size_t(k) * levels_ + n]) : double(N_[n]);
     upload(dnd_, nd);
     build_tables();
+    build_queue();
   }
 
   MaskView mask_view(int n) const {
@@ -778,7 +947,10 @@ class EngineT final : public Engine {
 
   double dec_bytes(int n, int NI, int rows, bool adj = false) const {
     double slots = adj ? dec_[n]->tslots : dec_[n]->slots;
-    return slots * (sizeof(T) + 4) + double(rows) * (N_[n] + N_[n + 1]) * NI * sizeof(T) * (adj ? 1.5 : 1.0);
+    // Forward: the fine rows in, the coarse rows out; adjoint: the coarse
+    // cotangent and the fine term it is added to in, the fine rows out.
+    const double elems = adj ? double(N_[n + 1]) + 2.0 * double(N_[n]) : double(N_[n]) + double(N_[n + 1]);
+    return slots * (sizeof(T) + 4) + double(rows) * elems * NI * sizeof(T);
   }
 
   void conv_adj(int n, int NI, const C2* gy, int rows, T* gx) {
@@ -798,17 +970,27 @@ class EngineT final : public Engine {
   // the sums (fwd_sums); then the statistics.  Levels >= 1 need only level
   // 0's fwd_z, so level 0's fwd_sums may run beside them.
   void sweep_forward(bool moments) {
-    for (int n = 0; n < levels_; ++n) {
+    fwd_levels(0, moments);
+    fwd_finish(moments);
+  }
+
+  // The forward of levels from .. J - 1 (maps decimated): level by level, and
+  // the work queue's levels in one launch.
+  void fwd_levels(int from, bool moments) {
+    for (int n = from; n < levels_; ++n) {
+      if (queue_on() && n == qc_) {
+        run_queue(qfwd_[moments ? 1 : 0]);
+        return;
+      }
       fwd_z(n);
       fwd_sums(n, moments);
     }
-    fwd_finish(moments);
   }
 
   void fwd_z(int n) {
     const int64_t N = N_[n];
     int id = prof_.start();
-    launch_conv_fwd<T>(conv_[n]->store.view(), 1, f_->maps[n].data(), B_, f_->z[n].data(),
+    launch_conv_fwd<T>(conv_[n]->store.view(), 1, f_->maps[n].data(), B_, z_of(n),
                        f_->M[n].data() + size_t(n) * B_ * N * kL, s_, conv_var_[n][0]);
     prof_.stop(id, tag("conv_fwd_z", n), conv_bytes(n, 1, B_, false) + double(B_) * N * kL * sizeof(T));
     if (n > 0) {
@@ -821,7 +1003,7 @@ class EngineT final : public Engine {
   void fwd_sums(int n, bool moments) {
     const int64_t N = N_[n];
     int id = prof_.start();
-    launch_conv_fwd<T>(conv_[n]->store.view(), kL, f_->M[n].data(), (n + 1) * B_, f_->br[n].data(), nullptr, s_,
+    launch_conv_fwd<T>(conv_[n]->store.view(), kL, f_->M[n].data(), (n + 1) * B_, br_of(n), nullptr, s_,
                        conv_var_[n][1]);
     prof_.stop(id, tag("conv_fwd_branch", n), conv_bytes(n, kL, (n + 1) * B_, false));
     id = prof_.start();
@@ -831,10 +1013,10 @@ class EngineT final : public Engine {
     const int K = gram_k(n);
     int64_t m = rows * kL * (cross_ ? K * K : gram_e(n));
     if (cross_)
-      launch_gram_x<T>(n, N, Bs_, f_->br[n].data(), f_->z[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(),
+      launch_gram_x<T>(n, N, Bs_, br_of(n), z_of(n), moments, nb, gpart_[n].data(), gpart2_[n].data(),
                        mv, s_);
     else
-      launch_gram<T>(n, N, B_, f_->br[n].data(), f_->z[n].data(), moments, nb, gpart_[n].data(), gpart2_[n].data(),
+      launch_gram<T>(n, N, B_, br_of(n), z_of(n), moments, nb, gpart_[n].data(), gpart2_[n].data(),
                      zpart_[n].data(), mv, s_);
     launch_reduce_parts(gpart_[n].data(), nb, m * 2, gsum_[n].data(), s_);
     if (moments) launch_reduce_parts(gpart2_[n].data(), nb, m, gsum2_[n].data(), s_);
@@ -842,13 +1024,18 @@ class EngineT final : public Engine {
     id = prof_.start();
     // The S1/S2 sums came with the Gram pass (one partial per Gram block);
     // the cross S1 sums take their own pass.
-    if (cross_) launch_zcross<T>(N, Bs_, f_->z[n].data(), nblk_z_[n], zpart_[n].data(), mv, s_);
+    if (cross_) launch_zcross<T>(N, Bs_, z_of(n), nblk_z_[n], zpart_[n].data(), mv, s_);
     launch_reduce_parts(zpart_[n].data(), cross_ ? nblk_z_[n] : nb, rows * kL * 4, zsum_[n].data(), s_);
     if (n == 0 && !cross_) {
       launch_meanvar<T>(N, B_, f_->maps[0].data(), nblk_mv_, mvpart_.data(), mv, s_);
       launch_reduce_parts(mvpart_.data(), nblk_mv_, rows * 4, mvsum_.data(), s_);
     }
-    prof_.stop(id, tag("zstats", n), double(rows) * kL * N * 2 * sizeof(T));
+    // The S1/S2 sums come with the Gram pass: this is the reduction of their
+    // partials, the cross S1 pass over z and level 0's mean / variance pass.
+    double zbytes = double(cross_ ? nblk_z_[n] : nb) * rows * kL * 4 * 8.0 + double(rows) * kL * 4 * 8.0;
+    if (cross_) zbytes += double(B_) * kL * N * 2 * sizeof(T);
+    if (n == 0 && !cross_) zbytes += double(B_) * N * sizeof(T) + double(nblk_mv_ + 1) * rows * 4 * 8.0;
+    prof_.stop(id, tag("zstats", n), zbytes);
   }
 
   void fwd_finish(bool moments) {
@@ -868,6 +1055,7 @@ class EngineT final : public Engine {
 
   // After the forward's stream work has completed: statistics out of the pinned staging.
   void finish_forward(bool moments, cplx* values, double* variance) override {
+    check_queue();
     const size_t nv = nvals();
     std::memcpy(values, hvalues_.data(), nv * sizeof(double2));
     if (moments && variance) std::memcpy(variance, hvar_.data(), nv * sizeof(double));
@@ -941,6 +1129,7 @@ class EngineT final : public Engine {
       for (const auto& st : steps) st.body();
       STL_HIP(hipStreamSynchronize(s_));
       prof_.collect();
+      check_queue();
       return;
     }
     GraphEntry* e = cached(cache, ptr, flag);
@@ -969,6 +1158,7 @@ class EngineT final : public Engine {
     STL_HIP(e3);
     STL_HIP(e2);
     STL_HIP(e1);
+    check_queue();
   }
 
   // The forward's steps.  After level n's z convolution (and the moduli it
@@ -984,16 +1174,12 @@ class EngineT final : public Engine {
     auto dec_all = [this] {
       for (int n = 1; n < levels_; ++n) dec_maps(n);
     };
-    auto coarser = [this, moments](int from) {
-      return [this, moments, from] {
-        for (int n = from; n < levels_; ++n) {
-          fwd_z(n);
-          fwd_sums(n, moments);
-        }
-      };
-    };
+    auto coarser = [this, moments](int from) { return [this, moments, from] { fwd_levels(from, moments); }; };
     auto sums = [this, moments](int n) { return [this, moments, n] { fwd_sums(n, moments); }; };
     auto finish = [this, moments] { fwd_finish(moments); };
+    // The recompute store holds one level's z and branches: the levels in turn.
+    if (f_->recompute)
+      return {{[this, x_dev, moments] { enqueue_forward(x_dev, moments); }, 0, {}}};
     if (levels_ == 1) return {{[first, sums, finish] { first(); sums(0)(); finish(); }, 0, {}}};
     if (levels_ == 2)
       return {{first, 0, {}},
@@ -1021,20 +1207,13 @@ class EngineT final : public Engine {
   // and 1's run on side streams beside the coarser levels; their chains
   // (which need the coarser chain) follow on the engine stream.
   std::vector<Step> reverse_steps(double* grad_dev) {
-    auto coarser = [this](int down_to) {
-      return [this, down_to] {
-        for (int n = levels_ - 1; n >= down_to; --n) {
-          rev_branch(n);
-          rev_chain(n);
-        }
-      };
-    };
+    auto coarser = [this](int down_to) { return [this, down_to] { rev_levels(down_to); }; };
     auto last = [this, grad_dev] {
       rev_chain(0);
       rev_finish(grad_dev);
     };
     auto prelude = [this] { rev_prelude(); };
-    if (levels_ == 1) return {{[this, grad_dev] { sweep_reverse(grad_dev); }, 0, {}}};
+    if (levels_ == 1 || f_->recompute) return {{[this, grad_dev] { sweep_reverse(grad_dev); }, 0, {}}};
     if (levels_ == 2)
       return {{prelude, 0, {}}, {[this] { rev_branch(0); }, 1, {0}}, {coarser(1), 0, {}}, {last, 0, {1}}};
     return {{prelude, 0, {}},        {[this] { rev_branch(0); }, 1, {0}}, {[this] { rev_branch(1); }, 2, {0}},
@@ -1077,7 +1256,7 @@ class EngineT final : public Engine {
         auto timed = [&](int v) {
           if (!adj) {
             T* x = NI == 1 ? f_->maps[n].data() : f_->M[n].data();
-            C2* y = NI == 1 ? f_->z[n].data() : f_->br[n].data();
+            C2* y = NI == 1 ? z_of(n) : br_of(n);
             T* mod = NI == 1 ? f_->M[n].data() + size_t(n) * B_ * N_[n] * kL : nullptr;
             return time_launch([&] { launch_conv_fwd<T>(op, NI, x, rows, y, mod, s_, v); });
           }
@@ -1121,6 +1300,217 @@ class EngineT final : public Engine {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     return ms / 2.0;
+  }
+
+  // One work-queue launch on the engine stream; its error flag is copied to
+  // pinned memory, for check_queue once the sweep has completed.
+  void run_queue(const QueueRun& q) {
+    int id = prof_.start();
+    launch_queue<T>(q.tasks.data(), q.ntasks, q.items, reinterpret_cast<QueueState*>(qstate_.data()), q.lds,
+                    q_workgroups_, s_);
+    prof_.stop(id, q.name, q.bytes);
+    STL_HIP(hipMemcpyAsync(hqerr_.data(), qstate_.data() + offsetof(QueueState, error), sizeof(unsigned int),
+                           hipMemcpyDeviceToHost, s_));
+  }
+
+ public:
+  int queue_level() const override { return queue_on() ? qc_ : levels_; }
+  void check_queue() const override {
+    STL_CHECK(hqerr_.data()[0] == 0u, "work queue: a dependency wait ran out (results void)");
+  }
+
+ private:
+  // The task lists of the work queue's levels (qc_ .. J - 1): forward (with
+  // and without moments) and reverse, in dependency order; the pointers are
+  // this engine's buffers, fixed once allocated.
+  void build_queue() {
+    if (!queue_on()) return;
+    auto base = [&](int op, int n, int ny, int64_t nx) {
+      QueueTask<T> t;
+      t.op = op;
+      t.n = n;
+      t.N = N_[n];
+      t.B = B_;
+      t.ny = ny;
+      t.nx = nx;
+      return t;
+    };
+    auto blocks = [](int64_t n, int per) { return (n + per - 1) / per; };
+    const int64_t rows = int64_t(Bs_) * Km_;
+    for (int mom = 0; mom < 2; ++mom) {
+      std::vector<QueueTask<T>> ts;
+      double bytes = 0.0;
+      std::vector<int> zt(levels_, -1), dt(levels_, -1);
+      for (int n = qc_; n < levels_; ++n) {
+        const int64_t N = N_[n];
+        const ConvOp<T> conv = conv_[n]->store.view();
+        QueueTask<T> z = base(kQConvFwd1, n, B_, blocks(N * 8, 256));
+        z.conv = conv;
+        z.in0 = f_->maps[n].data();
+        z.out0 = f_->z[n].data();
+        z.out1 = f_->M[n].data() + size_t(n) * B_ * N * kL;
+        zt[n] = int(ts.size());
+        ts.push_back(z);
+        bytes += conv_bytes(n, 1, B_, false) + double(B_) * N * kL * sizeof(T);
+        const DecOp<T> dec = dec_[n - 1]->store.view();
+        QueueTask<T> d = base(kQDecFwd4, n, n * B_, blocks(dec.Rc * 4, 256));
+        d.dec = dec;
+        d.in0 = f_->M[n - 1].data();
+        d.out0 = f_->M[n].data();
+        if (n - 1 >= qc_) {
+          d.dep[0] = zt[n - 1];
+          d.dep[1] = dt[n - 1];
+        }
+        dt[n] = int(ts.size());
+        ts.push_back(d);
+        bytes += dec_bytes(n - 1, kL, n * B_);
+        QueueTask<T> b = base(kQConvFwd4, n, (n + 1) * B_, blocks(N * 8, 256));
+        b.conv = conv;
+        b.in0 = f_->M[n].data();
+        b.out0 = f_->br[n].data();
+        b.dep[0] = zt[n];
+        b.dep[1] = dt[n];
+        const int bt = int(ts.size());
+        ts.push_back(b);
+        bytes += conv_bytes(n, kL, (n + 1) * B_, false);
+        const int K = gram_k(n), E = gram_e(n), nb = nblk_gram_[n];
+        QueueTask<T> g = base(kQGram, n, int(rows) * kL, nb);
+        g.in0 = f_->br[n].data();
+        g.in1 = f_->z[n].data();
+        g.out0 = gpart_[n].data();
+        g.out1 = gpart2_[n].data();
+        g.out2 = zpart_[n].data();
+        g.tpx = gram_tile(n);
+        g.bs = K <= 13 ? 2 : 4;
+        g.moments = mom;
+        g.dep[0] = bt;
+        const int gt = int(ts.size());
+        ts.push_back(g);
+        bytes += double(rows) * kL * N * K * 2 * sizeof(T);
+        auto reduce = [&](const double* part, int64_t m, double* out) {
+          QueueTask<T> r = base(kQReduce, n, 1, blocks(m, 64));
+          r.in0 = part;
+          r.nblk = nb;
+          r.m = m;
+          r.out0 = out;
+          r.dep[0] = gt;
+          ts.push_back(r);
+          bytes += (double(nb) + 1.0) * m * 8.0;
+        };
+        reduce(gpart_[n].data(), rows * kL * E * 2, gsum_[n].data());
+        if (mom) reduce(gpart2_[n].data(), rows * kL * E, gsum2_[n].data());
+        reduce(zpart_[n].data(), rows * kL * 4, zsum_[n].data());
+      }
+      upload_queue(qfwd_[mom], ts, bytes, mom ? "queue_fwd_moments" : "queue_fwd");
+    }
+    // Reverse: gV and branch adjoints of every level (independent), then the
+    // chain down the levels with each level's gz, z adjoint and map cotangent.
+    std::vector<QueueTask<T>> ts;
+    double bytes = 0.0;
+    std::vector<int> badj(levels_, -1), ch(levels_, -1), zadj(levels_, -1), mg(levels_, -1);
+    for (int n = levels_ - 1; n >= qc_; --n) {
+      const int64_t N = N_[n];
+      QueueTask<T> v = base(kQGv, n, B_ * kL, blocks(N, 32));
+      v.in0 = f_->br[n].data();
+      v.in1 = f_->z[n].data();
+      v.in2 = dH_[n].data();
+      v.out0 = f_->qgbr[n].data();
+      v.out1 = f_->qgzp[n].data();
+      const int vt = int(ts.size());
+      ts.push_back(v);
+      bytes += double(B_) * N * ((n + 1) * kL * kL + kL) * 2 * sizeof(T) * 2;
+      const ConvOp<T> conv = conv_[n]->store.view();
+      QueueTask<T> a = base(kQConvAdj4, n, (n + 1) * B_, blocks(N * 8, 256));
+      a.conv = conv;
+      a.in0 = f_->qgbr[n].data();
+      a.out0 = f_->qchain[n].data();
+      a.dep[0] = vt;
+      badj[n] = int(ts.size());
+      ts.push_back(a);
+      bytes += conv_bytes(n, kL, (n + 1) * B_, true);
+      if (conv.nfix > 0) {
+        QueueTask<T> f = base(kQFix4, n, (n + 1) * B_, blocks(int64_t(conv.nfix) * kL, 256));
+        f.conv = conv;
+        f.in0 = f_->qgbr[n].data();
+        f.out0 = f_->qchain[n].data();
+        f.dep[0] = badj[n];
+        badj[n] = int(ts.size());
+        ts.push_back(f);
+      }
+    }
+    for (int n = levels_ - 1; n >= qc_; --n) {
+      const int64_t N = N_[n];
+      const ConvOp<T> conv = conv_[n]->store.view();
+      ch[n] = badj[n];
+      if (n + 1 < levels_) {
+        const DecOp<T> dec = dec_[n]->store.view();
+        QueueTask<T> c = base(kQDecAdj4, n, (n + 1) * B_, blocks(dec.Rf * 4, 256));
+        c.dec = dec;
+        c.in0 = f_->qchain[n + 1].data();
+        c.in1 = f_->qchain[n].data();
+        c.out0 = f_->qchain[n].data();
+        c.dep[0] = badj[n];
+        c.dep[1] = ch[n + 1];
+        ch[n] = int(ts.size());
+        ts.push_back(c);
+        bytes += dec_bytes(n, kL, (n + 1) * B_, true);
+      }
+      QueueTask<T> z = base(kQGz, n, B_ * kL, blocks(N, 256));
+      z.in0 = f_->z[n].data();
+      z.in1 = f_->qgzp[n].data();
+      z.in2 = f_->qchain[n].data();
+      z.s1 = ds1_[n].data();
+      z.s2 = ds2_[n].data();
+      z.out0 = f_->qgz[n].data();
+      z.dep[0] = ch[n];
+      const int zt = int(ts.size());
+      ts.push_back(z);
+      bytes += double(B_) * kL * N * (7 * sizeof(T));
+      QueueTask<T> a = base(kQConvAdj1, n, B_, blocks(N * 8, 256));
+      a.conv = conv;
+      a.in0 = f_->qgz[n].data();
+      a.out0 = f_->qmg[n].data();
+      a.dep[0] = zt;
+      zadj[n] = int(ts.size());
+      ts.push_back(a);
+      bytes += conv_bytes(n, 1, B_, true);
+      if (conv.nfix > 0) {
+        QueueTask<T> f = base(kQFix1, n, B_, blocks(conv.nfix, 256));
+        f.conv = conv;
+        f.in0 = f_->qgz[n].data();
+        f.out0 = f_->qmg[n].data();
+        f.dep[0] = zadj[n];
+        zadj[n] = int(ts.size());
+        ts.push_back(f);
+      }
+      mg[n] = zadj[n];
+      if (n + 1 < levels_) {
+        const DecOp<T> dec = dec_[n]->store.view();
+        QueueTask<T> m = base(kQDecAdj1, n, B_, blocks(dec.Rf * 4, 256));
+        m.dec = dec;
+        m.in0 = f_->qmg[n + 1].data();
+        m.in1 = f_->qmg[n].data();
+        m.out0 = f_->qmg[n].data();
+        m.dep[0] = zadj[n];
+        m.dep[1] = mg[n + 1];
+        mg[n] = int(ts.size());
+        ts.push_back(m);
+        bytes += dec_bytes(n, 1, B_, true);
+      }
+    }
+    upload_queue(qrev_, ts, bytes, "queue_rev");
+    size_t state = 0;
+    for (const QueueRun* q : {&qfwd_[0], &qfwd_[1], &qrev_}) state = std::max(state, queue_state_bytes(q->ntasks));
+    if (qstate_.size() < state) qstate_.resize(state);
+  }
+
+  void upload_queue(QueueRun& q, std::vector<QueueTask<T>>& ts, double bytes, const char* name) {
+    q.items = queue_layout<T>(ts.data(), int(ts.size()));
+    q.lds = queue_lds_bytes<T>(ts.data(), int(ts.size()));
+    q.ntasks = int(ts.size());
+    q.bytes = bytes;
+    q.name = std::string(name) + "@L" + std::to_string(qc_);
+    upload(q.tasks, ts);
   }
 
   // Device tables of the statistics assembly and of the cotangent scatter.
@@ -1222,6 +1612,12 @@ class EngineT final : public Engine {
     d.resize(std::max<size_t>(h.size(), 1));
     if (!h.empty()) STL_HIP(hipMemcpy(d.data(), h.data(), h.size() * sizeof(U), hipMemcpyHostToDevice));
   }
+
+  int qc_ = 0;  // first level of the work queue (levels_: none)
+  int q_workgroups_ = 1;
+  QueueRun qfwd_[2], qrev_;  // forward without / with moments, reverse
+  DevArray<unsigned char> qstate_;
+  PinnedArray<unsigned int> hqerr_;  // the error flag of the last queue launch
 
   EngineOptions opt_;
   hipStream_t s_ = nullptr;

@@ -45,6 +45,20 @@ struct EngineOptions {
   // and kernel by timing at creation (cached per device, precision, level and
   // shape); 0, 1, 2 force one where the kernel has it (see sweeps.hpp).
   int conv_variant = -1;
+  // What the forward keeps for the reverse (the reference's FieldStore,
+  // crates/stl-gpu/src/engine.rs:172-186): keep stores every level's z and
+  // branches; recompute stores the maps and moduli only and recomputes each
+  // level's z and branches in the reverse (two more convolutions per level,
+  // one level's z, branch and cotangent buffers instead of all of them);
+  // auto keeps when the kept fields take at most half of the free device
+  // memory after the operators are built.
+  enum Fields { kKeep = 0, kRecompute = 1, kAuto = 2 };
+  int fields = kAuto;
+  // Levels of at most this nside (from level 2 on) run as one persistent
+  // work-queue launch per sweep (forward and reverse) instead of a launch per
+  // kernel and level; 0 switches the queue off.  Used with the keep store,
+  // one-channel plans and no masks; otherwise every level has its own launches.
+  int queue_nside = 32;
 };
 
 // One timed candidate of the convolution-variant choice: level, kernel
@@ -104,6 +118,14 @@ class Engine {
   virtual hipGraph_t capture_vjp(double* grad_dev) = 0;
   virtual hipStream_t stream() const = 0;
   virtual size_t device_bytes() const = 0;
+  // The field store in use (EngineOptions::kKeep or kRecompute) and the bytes of the level
+  // fields and reverse buffers.
+  virtual int field_store() const = 0;
+  virtual size_t field_bytes() const = 0;
+  // First level of the work-queue launches (levels() when there are none).
+  virtual int queue_level() const = 0;
+  // Raises if the last sweep's work queue flagged a dependency wait that ran out.
+  virtual void check_queue() const = 0;
   virtual double setup_seconds() const = 0;
   virtual std::vector<KernelTimer> timers() const = 0;
   // Chosen convolution variant per level: [level][forward z, forward branch,
@@ -117,6 +139,12 @@ class Engine {
   // [rows][N] -> [rows][4][N] complex, 1 its adjoint, 2 decimation to level + 1,
   // 3 its adjoint); for operator-level tests.
   virtual void apply_operator(int kind, int level, int rows, const double* in_dev, double* out_dev) = 0;
+  // Device bytes of staging apply_operator needs for (kind, level, rows), and the same
+  // operator application with that staging carved from a caller's device workspace
+  // (256-byte aligned, at least that many bytes): no allocation.
+  virtual size_t operator_workspace_bytes(int kind, int level, int rows) const = 0;
+  virtual void apply_operator_in(int kind, int level, int rows, const double* in_dev, double* out_dev,
+                                 void* workspace, size_t workspace_bytes) = 0;
   // Host copy of the forward state (maps, z, moduli, branches) and its restore.
   virtual std::vector<char> save_state() const = 0;
   virtual void load_state(const std::vector<char>& s) = 0;

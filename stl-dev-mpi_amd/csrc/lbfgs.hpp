@@ -63,6 +63,10 @@ class Lbfgs {
 
   Lbfgs(Backend& be, const LbfgsOptions& opt) : be_(be), opt_(opt) {}
 
+  // Called with (0, initial loss), then (iteration, loss at the accepted step) after every
+  // iteration's line search.
+  std::function<void(int, double)> on_iteration;
+
   // One LBFGS.step(closure) from x (updated in place); returns the number of evaluations
   // (lbfgs.rs:356-465).
   size_t minimize(V& x, const Objective& f) {
@@ -70,6 +74,7 @@ class Lbfgs {
     V flat_grad = be_.copy(x);
     double loss = f(x, flat_grad);
     size_t current_evals = 1;
+    if (on_iteration) on_iteration(0, loss);
     if (be_.abs_max(flat_grad) <= opt_.tolerance_grad) return current_evals;
     V d = be_.copy(x), prev_flat_grad = be_.copy(x);
     double t = 0.0;
@@ -120,6 +125,7 @@ class Lbfgs {
       be_.fma_into(x, d, t, x_init);
       bool opt_cond = be_.abs_max(flat_grad) <= opt_.tolerance_grad;
       current_evals += ls_evals;
+      if (on_iteration) on_iteration(n_iter, loss);
       if (n_iter == opt_.max_iter || current_evals >= size_t(max_eval) || opt_cond) break;
       if (be_.scaled_abs_max(d, t) <= opt_.tolerance_change) break;
       if (std::fabs(loss - prev_loss) < opt_.tolerance_change) break;

@@ -188,16 +188,18 @@ __global__ __launch_bounds__(256) void k_conv_adj(ConvOp<T> op, const cplx_t<T>*
 // lanes, for levels too small to fill the GPU: each thread's chain of
 // dependent loads is LP times shorter.  The LP partial sums are added by a
 // fixed butterfly (deterministic).
+// Bodies of the 256-thread kernels take their block (bx, row) as arguments, so
+// that the work-queue kernel of the coarse levels can run them as items.
 template <typename T, int NI, int LP>
-__global__ __launch_bounds__(256) void k_conv_fwd_split(ConvOp<T> op, const T* __restrict__ x,
-                                                        cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
-  const int64_t o = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) / LP;
+__device__ __forceinline__ void conv_fwd_split_body(const ConvOp<T>& op, const T* __restrict__ x,
+                                                    cplx_t<T>* __restrict__ y, T* __restrict__ mod, int64_t bx,
+                                                    int row) {
+  const int64_t o = (bx * 256 + threadIdx.x) / LP;
   const int lane = int(threadIdx.x % LP);
   const bool active = o < 12 * op.npf;
   const bool mir = active && (o >> 2) >= op.R;
   const int64_t r = active ? (mir ? (o >> 2) - op.R : o >> 2) : 0;
   const int k = int(o & 3);
-  const int row = blockIdx.y;
   const int64_t N = 12 * op.npf;
   const T* xr = x + size_t(row) * N * NI;
   T acc[NI][kL][2];
@@ -247,15 +249,20 @@ __global__ __launch_bounds__(256) void k_conv_fwd_split(ConvOp<T> op, const T* _
 }
 
 template <typename T, int NI, int LP>
-__global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
-                                                        T* __restrict__ gx) {
-  const int64_t o = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) / LP;
+__global__ __launch_bounds__(256) void k_conv_fwd_split(ConvOp<T> op, const T* __restrict__ x,
+                                                        cplx_t<T>* __restrict__ y, T* __restrict__ mod) {
+  conv_fwd_split_body<T, NI, LP>(op, x, y, mod, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int NI, int LP>
+__device__ __forceinline__ void conv_adj_split_body(const ConvOp<T>& op, const cplx_t<T>* __restrict__ gy,
+                                                    T* __restrict__ gx, int64_t bx, int row) {
+  const int64_t o = (bx * 256 + threadIdx.x) / LP;
   const int lane = int(threadIdx.x % LP);
   const bool active = o < 12 * op.npf;
   const bool mir = active && (o >> 2) >= op.R;
   const int64_t q = active ? (mir ? (o >> 2) - op.R : o >> 2) : 0;
   const int k = int(o & 3);
-  const int row = blockIdx.y;
   const int64_t N = 12 * op.npf;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
   T acc[NI];
@@ -293,6 +300,12 @@ __global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx
   if (mir) qk = mirror_pix(qk, op.order);
 #pragma unroll
   for (int j = 0; j < NI; ++j) gx[(size_t(row) * N + qk) * NI + j] = acc[j];
+}
+
+template <typename T, int NI, int LP>
+__global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
+                                                        T* __restrict__ gx) {
+  conv_adj_split_body<T, NI, LP>(op, gy, gx, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -876,12 +889,11 @@ void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
 // Correction of the mirrored columns, after the gather: thread per (south cell
 // of the list, j), gx += sum over its entries of Re(conj(delta_g) gy[g][row]).
 template <typename T, int NI>
-__global__ __launch_bounds__(256) void k_conv_adj_fix(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
-                                                      T* __restrict__ gx) {
-  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+__device__ __forceinline__ void conv_adj_fix_body(const ConvOp<T>& op, const cplx_t<T>* __restrict__ gy,
+                                                  T* __restrict__ gx, int64_t bx, int row) {
+  const int64_t t = bx * 256 + threadIdx.x;
   if (t >= int64_t(op.nfix) * NI) return;
   const int i = int(t / NI), j = int(t % NI);
-  const int row = blockIdx.y;
   const int64_t N = 12 * op.npf;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
   const cplx_t<T>* d = reinterpret_cast<const cplx_t<T>*>(op.fix_coef);
@@ -896,6 +908,12 @@ __global__ __launch_bounds__(256) void k_conv_adj_fix(ConvOp<T> op, const cplx_t
     }
   }
   gx[(size_t(row) * N + op.fix_cell[i]) * NI + j] += acc;
+}
+
+template <typename T, int NI>
+__global__ __launch_bounds__(256) void k_conv_adj_fix(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
+                                                      T* __restrict__ gx) {
+  conv_adj_fix_body<T, NI>(op, gy, gx, blockIdx.x, blockIdx.y);
 }
 
 // LDS-tiled decimation: a workgroup is a TC x TC tile of coarse pixels of one
@@ -981,13 +999,13 @@ __global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T*
 }
 
 template <typename T, int NI>
-__global__ __launch_bounds__(256) void k_dec_fwd(DecOp<T> op, const T* __restrict__ x, T* __restrict__ y) {
+__device__ __forceinline__ void dec_fwd_body(const DecOp<T>& op, const T* __restrict__ x, T* __restrict__ y,
+                                             int64_t bx, int row) {
   // Wave w of the workgroup = face shift w over 64 consecutive rep rows: a
   // wave's gathers stay in one face, and the 4 waves share the rows' entries.
-  const int64_t r = blockIdx.x * int64_t(64) + (threadIdx.x & 63);
+  const int64_t r = bx * 64 + (threadIdx.x & 63);
   if (r >= op.Rc) return;
   const int k = int(threadIdx.x >> 6);
-  int row = blockIdx.y;
   int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
   const T* xr = x + size_t(row) * Nf * NI;
   T acc[NI];
@@ -1009,12 +1027,16 @@ __global__ __launch_bounds__(256) void k_dec_fwd(DecOp<T> op, const T* __restric
 }
 
 template <typename T, int NI>
-__global__ __launch_bounds__(256) void k_dec_adj(DecOp<T> op, const T* __restrict__ gy, const T* add,
-                                                 T* gx) {
-  const int64_t q = blockIdx.x * int64_t(64) + (threadIdx.x & 63);
+__global__ __launch_bounds__(256) void k_dec_fwd(DecOp<T> op, const T* __restrict__ x, T* __restrict__ y) {
+  dec_fwd_body<T, NI>(op, x, y, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int NI>
+__device__ __forceinline__ void dec_adj_body(const DecOp<T>& op, const T* __restrict__ gy, const T* add, T* gx,
+                                             int64_t bx, int row) {
+  const int64_t q = bx * 64 + (threadIdx.x & 63);
   if (q >= op.Rf) return;
   const int k = int(threadIdx.x >> 6);
-  int row = blockIdx.y;
   int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
   const T* gr = gy + size_t(row) * Nc * NI;
   T acc[NI];
@@ -1034,6 +1056,12 @@ __global__ __launch_bounds__(256) void k_dec_adj(DecOp<T> op, const T* __restric
   size_t o = (size_t(row) * Nf + qk) * NI;
 #pragma unroll
   for (int j = 0; j < NI; ++j) gx[o + j] = add ? acc[j] + add[o + j] : acc[j];
+}
+
+template <typename T, int NI>
+__global__ __launch_bounds__(256) void k_dec_adj(DecOp<T> op, const T* __restrict__ gy, const T* add,
+                                                 T* gx) {
+  dec_adj_body<T, NI>(op, gy, add, gx, blockIdx.x, blockIdx.y);
 }
 
 template <typename T>
@@ -1163,19 +1191,21 @@ __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const
 // every LDS read feeds 4 products.  The values widen exactly to FP64; the
 // products and their sums are FP64.  The per-group sums are added in group
 // order: deterministic.
+// gx_dim / gy_dim: the grid of the launch (tiles are strided by gx_dim, the
+// partials indexed by it); smem: the dynamic LDS.
 template <typename T, bool MOM, int BS, bool MASKED>
-__global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
-                                              const cplx_t<T>* __restrict__ z, int TPX, double* __restrict__ part,
-                                              double* __restrict__ part2, double* __restrict__ zpart,
-                                              MaskView mask) {
-  extern __shared__ double2 smem[];
+__device__ __forceinline__ void gram_body(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
+                                          const cplx_t<T>* __restrict__ z, int TPX, double* __restrict__ part,
+                                          double* __restrict__ part2, double* __restrict__ zpart,
+                                          const MaskView& mask, int bx, int gx_dim, int bl, int gy_dim,
+                                          double2* smem) {
   const int K = (n + 1) * kL + 1;
   const int KP = (K + BS - 1) / BS * BS, NR = KP / BS, NBLK = NR * (NR + 1) / 2;
   // The S1/S2 sums of z, accumulated by the threads that stage z.
   double zs[4] = {0.0, 0.0, 0.0, 0.0};
   const int E = K * (K + 1) / 2;
   double2* Vs = smem;
-  const int bl = blockIdx.y;  // (b K + k) kL + l3
+  // bl = (b K + k) kL + l3
   const int l3 = bl % kL, mk = (bl / kL) % mask.K, b = bl / (kL * mask.K);
   const int tid = threadIdx.x;
   const int G = 256 / NBLK;
@@ -1190,12 +1220,12 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
 #pragma unroll
     for (int c = 0; c < BS; ++c) are[r][c] = aim[r][c] = asq[r][c] = 0.0;
   const int64_t ntiles = (N + TPX - 1) / TPX;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  for (int64_t tile = bx; tile < ntiles; tile += gx_dim) {
     const int64_t p0 = tile * TPX;
     const int tn = int(min<int64_t>(TPX, N - p0));
     for (int d = 0; d <= n; ++d) {
       const cplx_t<T>* src = br + ((size_t(d) * B + b) * kL + l3) * N * kL + size_t(p0) * kL;
-      for (int idx = tid; idx < tn * kL; idx += blockDim.x) {
+      for (int idx = tid; idx < tn * kL; idx += 256) {
         int p = idx / kL, l1 = idx % kL;
         // Pixels outside the mask stage zeros: their products add nothing.
         Vs[p * KP + d * kL + l1] =
@@ -1203,7 +1233,7 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
       }
     }
     const cplx_t<T>* zt = z + (size_t(b) * kL + l3) * N + p0;
-    for (int p = tid; p < tn; p += blockDim.x) {
+    for (int p = tid; p < tn; p += 256) {
       const double2 v = !MASKED || in_mask(mask, p0 + p, mk) ? to_d2<T>(zt[p]) : make_double2(0.0, 0.0);
       Vs[p * KP + K - 1] = v;
       const double h = hypot(v.x, v.y);
@@ -1213,7 +1243,7 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
       zs[3] += (h * h) * (h * h);
     }
     const int pad = KP - K;
-    for (int idx = tid; idx < tn * pad; idx += blockDim.x) Vs[(idx / pad) * KP + K + idx % pad] = make_double2(0.0, 0.0);
+    for (int idx = tid; idx < tn * pad; idx += 256) Vs[(idx / pad) * KP + K + idx % pad] = make_double2(0.0, 0.0);
     __syncthreads();
     if (act) {
       for (int p = grp; p < tn; p += G) {
@@ -1263,8 +1293,8 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
       }
       __syncthreads();
     }
-  const size_t base = (size_t(blockIdx.x) * gridDim.y + bl) * E;
-  for (int e = tid; e < E; e += blockDim.x) {
+  const size_t base = (size_t(bx) * gy_dim + bl) * E;
+  for (int e = tid; e < E; e += 256) {
     int i = 0, re = e;
     while (re >= K - i) { re -= K - i; ++i; }
     const int j = i + re;
@@ -1277,7 +1307,17 @@ __global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cpl
   __syncthreads();
   block_sum4(zs, reinterpret_cast<double*>(smem));
   if (tid == 0)
-    for (int c = 0; c < 4; ++c) zpart[(size_t(blockIdx.x) * gridDim.y + bl) * 4 + c] = zs[c];
+    for (int c = 0; c < 4; ++c) zpart[(size_t(bx) * gy_dim + bl) * 4 + c] = zs[c];
+}
+
+template <typename T, bool MOM, int BS, bool MASKED>
+__global__ __launch_bounds__(256) void k_gram(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
+                                              const cplx_t<T>* __restrict__ z, int TPX, double* __restrict__ part,
+                                              double* __restrict__ part2, double* __restrict__ zpart,
+                                              MaskView mask) {
+  extern __shared__ double2 smem[];
+  gram_body<T, MOM, BS, MASKED>(n, N, B, br, z, TPX, part, part2, zpart, mask, blockIdx.x, gridDim.x, blockIdx.y,
+                                gridDim.y, smem);
 }
 
 
@@ -1474,6 +1514,30 @@ __global__ __launch_bounds__(256) void k_meanvar(int64_t N, const T* __restrict_
 // part [nblk][m] -> out [m]: 64 outputs per workgroup, wave w adding the
 // blocks b = w, w + 16, ... in order, then the 16 wave sums in order.
 constexpr int kRedWaves = 16;
+// The same sums by a 256-thread workgroup (a work-queue item): each wave
+// takes the block sequences of 4 of the 16 "waves" above in turn, so every
+// output adds the same terms in the same order (the same bits).  lds: 16 x 64 doubles.
+__device__ __forceinline__ void reduce_parts_body(const double* __restrict__ part, int nblk, int64_t m,
+                                                  double* __restrict__ out, int64_t bx, double* lds) {
+  const int lane = threadIdx.x & 63, w4 = threadIdx.x >> 6;
+  const int64_t i = bx * 64 + lane;
+  for (int v = w4; v < kRedWaves; v += 4) {
+    double s = 0.0;
+    if (i < m) {
+#pragma unroll 8
+      for (int b = v; b < nblk; b += kRedWaves) s += part[size_t(b) * m + i];
+    }
+    lds[v * 64 + lane] = s;
+  }
+  __syncthreads();
+  if (w4 == 0 && i < m) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < kRedWaves; ++k) t += lds[k * 64 + lane];
+    out[i] = t;
+  }
+}
+
 __global__ __launch_bounds__(64 * kRedWaves) void k_reduce_parts(const double* __restrict__ part, int nblk,
                                                                  int64_t m, double* __restrict__ out) {
   __shared__ double sh[kRedWaves][64];
@@ -1699,16 +1763,17 @@ __global__ __launch_bounds__(256) void k_gz_x(int n, int64_t N, int Bf, const cp
 // lanes split the K outputs (output i in group i mod 8), so each lane forms
 // K / 8 outputs instead of K.  One mask class (unmasked).
 constexpr int kGvSmallPix = 32;
+// LDS: (K^2 + kGvSmallPix K) double2.
 template <typename T, int NL>
-__global__ __launch_bounds__(256) void k_gv_small(int64_t N, int B, const cplx_t<T>* __restrict__ br,
-                                                  const cplx_t<T>* __restrict__ z, const double2* __restrict__ H,
-                                                  cplx_t<T>* __restrict__ gbr, cplx_t<T>* __restrict__ gzp) {
+__device__ __forceinline__ void gv_small_body(int64_t N, int B, const cplx_t<T>* __restrict__ br,
+                                              const cplx_t<T>* __restrict__ z, const double2* __restrict__ H,
+                                              cplx_t<T>* __restrict__ gbr, cplx_t<T>* __restrict__ gzp, int64_t bx,
+                                              int bl, double2* lds) {
   constexpr int K = NL * kL + 1;
-  __shared__ double2 Hs[K * K];
-  __shared__ double2 Vs[kGvSmallPix * K];
-  const int bl = blockIdx.y;
+  double2* Hs = lds;
+  double2* Vs = lds + K * K;
   const int b = bl / kL, l3 = bl % kL;
-  const int64_t p0 = int64_t(blockIdx.x) * kGvSmallPix;
+  const int64_t p0 = bx * kGvSmallPix;
   for (int e = threadIdx.x; e < K * K; e += blockDim.x) Hs[e] = H[size_t(bl) * K * K + e];
   for (int e = threadIdx.x; e < kGvSmallPix * K; e += blockDim.x) {
     const int px = e / K, j = e % K;
@@ -1746,14 +1811,22 @@ __global__ __launch_bounds__(256) void k_gv_small(int64_t N, int B, const cplx_t
   }
 }
 
+template <typename T, int NL>
+__global__ __launch_bounds__(256) void k_gv_small(int64_t N, int B, const cplx_t<T>* __restrict__ br,
+                                                  const cplx_t<T>* __restrict__ z, const double2* __restrict__ H,
+                                                  cplx_t<T>* __restrict__ gbr, cplx_t<T>* __restrict__ gzp) {
+  constexpr int K = NL * kL + 1;
+  __shared__ double2 lds[K * K + kGvSmallPix * K];
+  gv_small_body<T, NL>(N, B, br, z, H, gbr, gzp, blockIdx.x, blockIdx.y, lds);
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_gz(int n, int64_t N, int B, const cplx_t<T>* __restrict__ z,
-                                            const cplx_t<T>* __restrict__ gzp, const T* __restrict__ chain,
-                                            const double* __restrict__ s1, const double* __restrict__ s2,
-                                            cplx_t<T>* __restrict__ gz, MaskView mask) {
-  int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+__device__ __forceinline__ void gz_body(int n, int64_t N, int B, const cplx_t<T>* __restrict__ z,
+                                        const cplx_t<T>* __restrict__ gzp, const T* __restrict__ chain,
+                                        const double* __restrict__ s1, const double* __restrict__ s2,
+                                        cplx_t<T>* __restrict__ gz, const MaskView& mask, int64_t bx, int bl) {
+  int64_t p = bx * 256 + threadIdx.x;
   if (p >= N) return;
-  int bl = blockIdx.y;
   int b = bl / kL, l = bl % kL;
   size_t at = size_t(bl) * N + p;
   double zr = double(z[at].x), zi = double(z[at].y);
@@ -1765,6 +1838,14 @@ __global__ __launch_bounds__(256) void k_gz(int n, int64_t N, int B, const cplx_
   double re = double(gzp[at].x) + (a * zr) / den + c * zr + (h * zr) / den;
   double im = double(gzp[at].y) + (a * zi) / den + c * zi + (h * zi) / den;
   gz[at] = make_c<T>(T(re), T(im));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gz(int n, int64_t N, int B, const cplx_t<T>* __restrict__ z,
+                                            const cplx_t<T>* __restrict__ gzp, const T* __restrict__ chain,
+                                            const double* __restrict__ s1, const double* __restrict__ s2,
+                                            cplx_t<T>* __restrict__ gz, MaskView mask) {
+  gz_body<T>(n, N, B, z, gzp, chain, s1, s2, gz, mask, blockIdx.x, blockIdx.y);
 }
 
 template <typename T>
@@ -1784,7 +1865,207 @@ __global__ void k_cast(int64_t n, const TI* __restrict__ in, TO* __restrict__ ou
   if (i < n) out[i] = TO(in[i]);
 }
 
+// ---------------------------------------------------------------------------
+// The work queue of the coarse levels (QueueTask / QueueState, sweeps.hpp).
+// ---------------------------------------------------------------------------
+
+// Spins of a dependency wait before it gives up and flags the launch (a wait
+// is normally a few microseconds; this is seconds).
+constexpr unsigned kQueueSpinLimit = 1u << 24;
+
+__global__ void k_queue_reset(QueueState* st, int ntasks) {
+  for (int i = threadIdx.x; i < ntasks; i += blockDim.x) st->done[i] = 0u;
+  if (threadIdx.x == 0) {
+    st->head = 0ull;
+    st->error = 0u;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void queue_item(const QueueTask<T>& t, int64_t bx, int by, double2* lds) {
+  using C2 = cplx_t<T>;
+  const MaskView none;
+  switch (t.op) {
+    case kQDecFwd1:
+      dec_fwd_body<T, 1>(t.dec, static_cast<const T*>(t.in0), static_cast<T*>(t.out0), bx, by);
+      break;
+    case kQDecFwd4:
+      dec_fwd_body<T, kL>(t.dec, static_cast<const T*>(t.in0), static_cast<T*>(t.out0), bx, by);
+      break;
+    case kQConvFwd1:
+      conv_fwd_split_body<T, 1, 8>(t.conv, static_cast<const T*>(t.in0), static_cast<C2*>(t.out0),
+                                   static_cast<T*>(t.out1), bx, by);
+      break;
+    case kQConvFwd4:
+      conv_fwd_split_body<T, kL, 8>(t.conv, static_cast<const T*>(t.in0), static_cast<C2*>(t.out0), nullptr, bx,
+                                    by);
+      break;
+    case kQGram: {
+      const C2* br = static_cast<const C2*>(t.in0);
+      const C2* z = static_cast<const C2*>(t.in1);
+      double* part = static_cast<double*>(t.out0);
+      double* part2 = static_cast<double*>(t.out1);
+      double* zpart = static_cast<double*>(t.out2);
+      const int bxi = int(bx);
+      if (t.bs == 2) {
+        if (t.moments) gram_body<T, true, 2, false>(t.n, t.N, t.B, br, z, t.tpx, part, part2, zpart, none, bxi, int(t.nx), by, t.ny, lds);
+        else gram_body<T, false, 2, false>(t.n, t.N, t.B, br, z, t.tpx, part, part2, zpart, none, bxi, int(t.nx), by, t.ny, lds);
+      } else {
+        if (t.moments) gram_body<T, true, 4, false>(t.n, t.N, t.B, br, z, t.tpx, part, part2, zpart, none, bxi, int(t.nx), by, t.ny, lds);
+        else gram_body<T, false, 4, false>(t.n, t.N, t.B, br, z, t.tpx, part, part2, zpart, none, bxi, int(t.nx), by, t.ny, lds);
+      }
+      break;
+    }
+    case kQReduce:
+      reduce_parts_body(static_cast<const double*>(t.in0), t.nblk, t.m, static_cast<double*>(t.out0), bx,
+                        reinterpret_cast<double*>(lds));
+      break;
+    case kQGv: {
+      const C2* br = static_cast<const C2*>(t.in0);
+      const C2* z = static_cast<const C2*>(t.in1);
+      const double2* H = static_cast<const double2*>(t.in2);
+      C2* gbr = static_cast<C2*>(t.out0);
+      C2* gzp = static_cast<C2*>(t.out1);
+      switch (t.n + 1) {
+#define STL_QGV(NL) \
+  case NL: gv_small_body<T, NL>(t.N, t.B, br, z, H, gbr, gzp, bx, by, lds); break;
+        STL_QGV(1) STL_QGV(2) STL_QGV(3) STL_QGV(4) STL_QGV(5) STL_QGV(6) STL_QGV(7) STL_QGV(8) STL_QGV(9)
+        STL_QGV(10) STL_QGV(11) STL_QGV(12)
+#undef STL_QGV
+        default: break;
+      }
+      break;
+    }
+    case kQConvAdj1:
+      conv_adj_split_body<T, 1, 8>(t.conv, static_cast<const C2*>(t.in0), static_cast<T*>(t.out0), bx, by);
+      break;
+    case kQConvAdj4:
+      conv_adj_split_body<T, kL, 8>(t.conv, static_cast<const C2*>(t.in0), static_cast<T*>(t.out0), bx, by);
+      break;
+    case kQFix1:
+      conv_adj_fix_body<T, 1>(t.conv, static_cast<const C2*>(t.in0), static_cast<T*>(t.out0), bx, by);
+      break;
+    case kQFix4:
+      conv_adj_fix_body<T, kL>(t.conv, static_cast<const C2*>(t.in0), static_cast<T*>(t.out0), bx, by);
+      break;
+    case kQDecAdj1:
+      dec_adj_body<T, 1>(t.dec, static_cast<const T*>(t.in0), static_cast<const T*>(t.in1), static_cast<T*>(t.out0),
+                         bx, by);
+      break;
+    case kQDecAdj4:
+      dec_adj_body<T, kL>(t.dec, static_cast<const T*>(t.in0), static_cast<const T*>(t.in1),
+                          static_cast<T*>(t.out0), bx, by);
+      break;
+    case kQGz:
+      gz_body<T>(t.n, t.N, t.B, static_cast<const C2*>(t.in0), static_cast<const C2*>(t.in1),
+                 static_cast<const T*>(t.in2), t.s1, t.s2, static_cast<C2*>(t.out0), none, bx, by);
+      break;
+    default:
+      break;
+  }
+}
+
+// Persistent workgroups take items in queue order; an item runs once the tasks
+// its task depends on have completed.  Every workgroup leaves once the items
+// are exhausted (the item counter only grows), so the grid always drains.
+template <typename T>
+__global__ __launch_bounds__(256) void k_queue(const QueueTask<T>* __restrict__ tasks, int ntasks, int64_t items,
+                                               QueueState* st) {
+  extern __shared__ double2 qlds[];
+  __shared__ int64_t s_item;
+  __shared__ int s_task;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      s_item = int64_t(__hip_atomic_fetch_add(&st->head, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      int lo = 0, hi = ntasks - 1;  // the last task whose first item <= s_item
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (tasks[mid].first <= s_item) lo = mid;
+        else hi = mid - 1;
+      }
+      s_task = lo;
+    }
+    __syncthreads();
+    const int64_t item = s_item;
+    const int ti = s_task;
+    if (item >= items) break;
+    const QueueTask<T>& t = tasks[ti];
+    if (threadIdx.x == 0) {
+      for (int q = 0; q < kQueueMaxDeps; ++q) {
+        const int d = t.dep[q];
+        if (d < 0) continue;
+        const unsigned need = unsigned(tasks[d].nx * tasks[d].ny);
+        unsigned spins = 0;
+        while (__hip_atomic_load(&st->done[d], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          __builtin_amdgcn_s_sleep(2);
+          if (++spins >= kQueueSpinLimit) {
+            __hip_atomic_store(&st->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int64_t local = item - t.first;
+    queue_item<T>(t, local % t.nx, int(local / t.nx), qlds);
+    // The item's writes are visible device-wide before it counts as done.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(&st->done[ti], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 }  // namespace
+
+template <typename T>
+int64_t queue_layout(QueueTask<T>* tasks, int ntasks) {
+  int64_t first = 0;
+  for (int i = 0; i < ntasks; ++i) {
+    STL_CHECK(tasks[i].nx >= 1 && tasks[i].ny >= 1, "queue: empty task");
+    for (int q = 0; q < kQueueMaxDeps; ++q)
+      STL_CHECK(tasks[i].dep[q] < i, "queue: a task depends on a later one");
+    tasks[i].first = first;
+    first += tasks[i].nx * tasks[i].ny;
+  }
+  STL_CHECK(first < (int64_t(1) << 31), "queue: too many items");
+  return first;
+}
+
+template <typename T>
+size_t queue_lds_bytes(const QueueTask<T>* tasks, int ntasks) {
+  size_t lds = 0;
+  for (int i = 0; i < ntasks; ++i) {
+    const QueueTask<T>& t = tasks[i];
+    size_t need = 0;
+    if (t.op == kQGram) {
+      const int K = (t.n + 1) * kL + 1, BS = t.bs;
+      const int KP = (K + BS - 1) / BS * BS, NR = KP / BS, NBLK = NR * (NR + 1) / 2;
+      need = std::max(size_t(t.tpx) * KP * sizeof(double2),
+                      (256 + 128) * sizeof(double2) + size_t(NBLK) * BS * BS * (sizeof(double2) + sizeof(double)));
+      need = std::max(need, size_t(4 * 256) * sizeof(double));
+    } else if (t.op == kQReduce) {
+      need = size_t(kRedWaves) * 64 * sizeof(double);
+    } else if (t.op == kQGv) {
+      const int K = (t.n + 1) * kL + 1;
+      need = size_t(K * K + kGvSmallPix * K) * sizeof(double2);
+    }
+    lds = std::max(lds, need);
+  }
+  return lds;
+}
+
+template <typename T>
+void launch_queue(const QueueTask<T>* tasks, int ntasks, int64_t items, QueueState* state, size_t lds,
+                  int workgroups, hipStream_t s) {
+  STL_CHECK(ntasks >= 1 && items >= 1 && workgroups >= 1, "queue: nothing to run");
+  STL_CHECK(lds <= 64 * 1024, "queue: the tasks need more than 64 KB of LDS");
+  k_queue_reset<<<1, 256, 0, s>>>(state, ntasks);
+  STL_HIP(hipGetLastError());
+  const int grid = int(std::min<int64_t>(items, workgroups));
+  k_queue<T><<<grid, 256, lds, s>>>(tasks, ntasks, items, state);
+  STL_HIP(hipGetLastError());
+}
 
 template <typename T, int TS>
 void conv_fwd_tiled(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<T>* y, T* mod, hipStream_t s) {
@@ -2247,6 +2528,14 @@ void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s) {
 STL_INSTANTIATE(double)
 STL_INSTANTIATE(float)
 #undef STL_INSTANTIATE
+
+#define STL_QUEUE_INSTANTIATE(T)                                                                  \
+  template int64_t queue_layout<T>(QueueTask<T>*, int);                                          \
+  template size_t queue_lds_bytes<T>(const QueueTask<T>*, int);                                  \
+  template void launch_queue<T>(const QueueTask<T>*, int, int64_t, QueueState*, size_t, int, hipStream_t);
+STL_QUEUE_INSTANTIATE(double)
+STL_QUEUE_INSTANTIATE(float)
+#undef STL_QUEUE_INSTANTIATE
 
 template void launch_cast<double, float>(int64_t, const float*, double*, hipStream_t);
 template void launch_cast<float, double>(int64_t, const double*, float*, hipStream_t);

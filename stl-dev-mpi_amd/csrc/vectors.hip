@@ -36,6 +36,24 @@ __global__ void k_fma(int64_t n, const double* __restrict__ d, double t, const d
   if (i < n) o[i] = fma(d[i], t, x[i]);
 }
 
+// Stream copy of n 16-byte words: each workgroup moves 4 x 256 consecutive
+// words (16 KB), every lane 4 words 4 KB apart, loads before stores.
+__global__ __launch_bounds__(kThreads) void k_stream_copy(int64_t n, const float4* __restrict__ src,
+                                                          float4* __restrict__ dst) {
+  const int64_t base = int64_t(blockIdx.x) * 4 * kThreads + threadIdx.x;
+  float4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = base + int64_t(k) * kThreads;
+    if (i < n) v[k] = src[i];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = base + int64_t(k) * kThreads;
+    if (i < n) dst[i] = v[k];
+  }
+}
+
 // kind 0: sum a*b, 1: sum |a|, 2: max |a|, 3: max |a * t|
 __global__ __launch_bounds__(kThreads) void k_chunk_reduce(int64_t n, int kind, const double* __restrict__ a,
                                                            const double* __restrict__ b, double t,
@@ -152,5 +170,33 @@ double VectorOps::dot(const Vec& a, const Vec& b) { return reduce(0, a.p, b.p, 0
 double VectorOps::abs_sum(const Vec& a) { return reduce(1, a.p, nullptr, 0.0); }
 double VectorOps::abs_max(const Vec& a) { return reduce(2, a.p, nullptr, 0.0); }
 double VectorOps::scaled_abs_max(const Vec& d, double t) { return reduce(3, d.p, nullptr, t); }
+
+// Mean milliseconds of one stream copy of `bytes` (a multiple of 16, both
+// addresses 16-byte aligned) over `reps` copies after one warm-up, HIP events
+// on the stream the copies run on.
+double stream_copy_ms(const void* src, void* dst, size_t bytes, int reps, hipStream_t s) {
+  STL_CHECK(bytes % 16 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0 &&
+                reinterpret_cast<uintptr_t>(dst) % 16 == 0 && reps >= 1,
+            "stream copy: 16-byte multiples and alignment, reps >= 1");
+  const int64_t n = int64_t(bytes / 16);
+  const unsigned grid = unsigned((n + 4 * kThreads - 1) / (4 * kThreads));
+  auto copy = [&] {
+    k_stream_copy<<<grid, kThreads, 0, s>>>(n, static_cast<const float4*>(src), static_cast<float4*>(dst));
+    STL_HIP(hipGetLastError());
+  };
+  hipEvent_t a, b;
+  STL_HIP(hipEventCreate(&a));
+  STL_HIP(hipEventCreate(&b));
+  copy();
+  STL_HIP(hipEventRecord(a, s));
+  for (int r = 0; r < reps; ++r) copy();
+  STL_HIP(hipEventRecord(b, s));
+  STL_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  STL_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return double(ms) / reps;
+}
 
 }  // namespace stl

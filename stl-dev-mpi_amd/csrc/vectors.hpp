@@ -72,4 +72,8 @@ class VectorOps {
   std::vector<double> host_;
 };
 
+// Mean ms of one copy of `bytes` by a hand-written stream kernel (the
+// attainable HBM rate, against which the engine's kernels are read).
+double stream_copy_ms(const void* src, void* dst, size_t bytes, int reps, hipStream_t s);
+
 }  // namespace stl

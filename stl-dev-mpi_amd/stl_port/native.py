@@ -15,7 +15,8 @@ Maps and gradients on the device (torch tensors or raw addresses) go through the
 methods without host copies.  The library has no CPU path and this module has no fallback: a
 missing library raises at construction.  Keywords of this library beyond the reference's:
 `precision` ("fp64" | "mixed"), `fields` ("keep" | "recompute" | "auto"), `timing`,
-`conv_variant`, `families`.
+`conv_variant`, `families`, `queue_nside` (levels of at most this nside run as one work-queue
+launch per sweep; 0: none).
 """
 from __future__ import annotations
 
@@ -65,12 +66,12 @@ ABI = {
     "stlp_train_bias": "ipwUdz", "stlp_train_loss": "ipwbd", "stlp_train_step": "ipwgd",
     "stlp_train_live": "ipwUdzUdzbgd", "stlp_train_lbfgs": "ppWzs", "stlp_train_set_joint": "ipddwwsg",
     "stlp_train_front": "ipdzbd",
-    "stlx_share": "pps", "stlx_apply_operator": "ipiiiuu", "stlx_synthesize": "lpuzddsdz",
+    "stlx_share": "pps", "stlx_apply_operator": "ipiiiuu", "stlx_apply_operator_noalloc": "ipiiiuu", "stlx_synthesize": "lpuzddsdz",
     "stlx_ensemble_new": "pIIIzs", "stlx_ensemble_free": "-p", "stlx_ensemble_ncoef": "ipZ",
     "stlx_ensemble_forward_device": "ipUzbdd", "stlx_ensemble_vjp_device": "ipdzUz",
     "stlx_map_len": "isZ", "stlx_read_map": "isdz", "stlx_read_map_range": "iszzd", "stlx_read_npy": "isdzZZ",
     "stlx_write_npy": "isdZz", "stlx_write_map": "isdzsb", "stlx_sliced_create": "iszsb",
-    "stlx_sliced_write": "isbzdz", "stlx_pixel_normal": "iulzd",
+    "stlx_sliced_write": "isbzdz", "stlx_pixel_normal": "iulzd", "stlx_stream_copy": "iuuzid",
 }
 
 
@@ -276,7 +277,7 @@ class NativeEngine:
     def __init__(self, nside, j, batch=1, backend="gpu", kernel=5, orientations=4, block_pixels=4096,
                  cache_bytes=1 << 30, threads=1, cache="transposition", device=0, library=None, cross=False,
                  masks=None, spin=0, spin_kernel=5, spin_pyramid="scalar", spin_a=2, precision="fp64",
-                 fields="keep", timing=False, conv_variant=None, families=None):
+                 fields="auto", timing=False, conv_variant=None, families=None, queue_nside=None):
         self._lock = threading.RLock()
         self._h = None
         self._lib = bridge(library)
@@ -299,6 +300,8 @@ class NativeEngine:
             config["conv_variant"] = int(conv_variant)
         if families is not None:
             config["families"] = ",".join(families)
+        if queue_nside is not None:
+            config["queue_nside"] = int(queue_nside)
         self._adopt(self._lib.call("stlp_new", json.dumps(config).encode()), config)
         if masks is not None:
             self.set_masks(masks)
@@ -510,7 +513,7 @@ class NativeEngine:
     # device synthesis ----------------------------------------------------------------------
     @_engine_call
     def synthesize(self, x, target, weights=None, lr=1.0, max_iter=20, max_eval=None, tolerance_grad=1e-12,
-                   tolerance_change=1e-15, history_size=10, ensemble=False):
+                   tolerance_change=1e-15, history_size=10, ensemble=False, record="evaluations"):
         """Device L-BFGS of sum w |S(x) - target|^2 from the float64 device tensor x [batch, npix]
         (updated in place); returns the loss of every evaluation.
 
@@ -518,7 +521,9 @@ class NativeEngine:
         row [K, ncoeff] for the mean statistics of the members (the reference's stl-synth
         --ensemble, crates/stl-mpi/src/synthesis.rs:362-470, without its scatter term).
         weights: one per mask and coefficient, [K, ncoeff] (the shape of native_pixels()),
-        applied to every map; None weighs all 1."""
+        applied to every map; None weighs all 1.  record: "evaluations" (the loss of every
+        evaluation) or "iterations" (the start's loss, then the loss at each iteration's
+        accepted step)."""
         rows = max(self.nmasks, 1) * self.ncoeff
         addr, count = _device_arg(x, self._map_count())
         t = np.ascontiguousarray(np.asarray(target, dtype=np.complex128).reshape(-1))
@@ -530,14 +535,16 @@ class NativeEngine:
             if w.size != rows:
                 raise ValueError(f"expected {rows} weights (masks x coefficients)")
         options = dict(lr=lr, max_iter=max_iter, tolerance_grad=tolerance_grad, tolerance_change=tolerance_change,
-                       history_size=history_size, ensemble=bool(ensemble))
+                       history_size=history_size, ensemble=bool(ensemble), history=record)
         if max_eval is not None:
             options["max_eval"] = max_eval
-        cap = (max_eval or max_iter * 5 // 4) + 64
+        cap = max(max_eval or max_iter * 5 // 4, max_iter) + 64
         history = np.zeros(cap)
         self._state = None
         n = self._lib.call("stlx_synthesize", self._h.ptr, addr, count, _dptr(t.view(np.float64)),
                            None if w is None else _dptr(w), json.dumps(options).encode(), _dptr(history), cap)
+        if record == "iterations":
+            return history[~np.isnan(history)].copy()
         return history[:min(n, cap)].copy()
 
     @_engine_call
@@ -643,6 +650,22 @@ class NativeEnsemble:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def stream_copy_gbps(nbytes=2 << 30, reps=5, device=0, library=None):
+    """HBM rate (bytes read + written per second, GB/s) of the library's hand-written stream
+    copy kernel over a buffer of `nbytes` (stlx_stream_copy)."""
+    import torch
+    words = int(nbytes) // 16
+    src = torch.empty(words * 4, dtype=torch.float32, device=f"cuda:{device}").uniform_()
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    ms = C.c_double(0.0)
+    bridge(library).call("stlx_stream_copy", int(src.data_ptr()), int(dst.data_ptr()), words * 16, int(reps),
+                         C.byref(ms))
+    if not torch.equal(src, dst):
+        raise RuntimeError("stream copy: destination differs from the source")
+    return 2.0 * words * 16 / (ms.value * 1e-3) / 1e9
 
 
 def ergodic_weights(native_pixels, variance):

@@ -1,11 +1,13 @@
 """Parity of the HIP engine (through the C ABI) with the FP64 oracle and the
 reference's stats fixtures.
 
-Tolerances: the FP64 GPU path must match the oracle to 1e-10 relative
-(statistics: max |dS| / max |S| per family; gradient: max |dg| / max |g|);
-the reference's own fixture tolerances (stats.rs:183-201) for the fixtures.
-The mixed path (FP32 bulk, FP64 sums) is held to 1e-5 per family and 1e-4
-on the gradient here; its measured errors are reported by the benchmark.
+Tolerances: the FP64 GPU path is held to the reference's own device-vs-CPU
+bars (crates/stl-gpu/tests/stats.rs:248-275), relative to the largest entry:
+statistics 1e-13 per family (max |dS| / max |S|), variances 1e-12, gradient
+1e-12 (max |dg| / max |g|); the reference's fixture tolerances
+(stats.rs:183-201) for its fixtures.  The mixed path (FP32 bulk, FP64 sums) is
+held to 1e-5 per family and 1e-4 on the gradient at these small sizes; its
+error at the benchmark's size is checked against oracle vectors elsewhere.
 """
 import numpy as np
 import pytest
@@ -19,6 +21,7 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 FAMILY_NAMES = ["mean", "var", "S1", "S2", "S3", "S4"]
+STAT_BAR, VAR_BAR, GRAD_BAR = 1e-13, 1e-12, 1e-12  # crates/stl-gpu/tests/stats.rs:248-275
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -126,9 +129,12 @@ def test_forward_vjp_vs_oracle(nside, J):
     print(f"[gpu] forward+vjp nside={nside} J={J}", flush=True)
     values, _ = eng.forward(x, generation=5)
     errs = family_errors(values, sx, eng.plan)
-    assert max(errs.values()) < 1e-10, errs
+    print(f"[gpu] fp64 vs oracle per family {errs}", flush=True)
+    assert max(errs.values()) < STAT_BAR, errs
     grad = eng.vjp(cot, generation=5)
-    assert rel(grad, g_ref) < 1e-10
+    gerr = rel(grad, g_ref)
+    print(f"[gpu] fp64 vs oracle gradient {gerr:.3e}", flush=True)
+    assert gerr < GRAD_BAR
 
 
 def test_mixed_precision_vs_oracle():
@@ -179,8 +185,30 @@ def test_moments_match_oracle():
     sx, second = ref.forward(moments=True)
     var_ref = np.maximum(second - np.abs(sx) ** 2, 0.0)
     values, var = eng.forward(x, moments=True)
-    assert rel(values, sx) < 1e-12
-    assert np.abs(var - var_ref).max() <= 1e-10 * np.abs(var_ref).max()
+    assert max(family_errors(values, sx, eng.plan).values()) < STAT_BAR
+    assert np.abs(var - var_ref).max() <= VAR_BAR * np.abs(var_ref).max()
+
+
+@pytest.mark.parametrize("case", [c for c in cases("stats", "stats")
+                                  if c["L"] == 4 and c["kernel_sz"] == 5 and c["channels"] == 1
+                                  and not c["fewer"] and c["j_to_dg"] == list(range(c["J"]))],
+                         ids=lambda c: c.name)
+def test_forward_levels_fixtures(case):
+    """stlp_forward_levels (lib.rs:583-623): the fixture's map of every level in (map{d}, the
+    reference's own decimations), its statistics out, at the fixture tolerance; no VJP may
+    follow.  Also against the oracle fed the same levels' decimation of level 0."""
+    eng = engine(case["nside"], case["J"], case["batch"])
+    levels = [case.arr(f"map{d}").reshape(case["batch"], -1) for d in range(case["J"])]
+    print(f"[gpu] forward_levels {case.name}", flush=True)
+    values = eng.forward_levels(levels)
+    ref = case.complex("values").reshape(values.shape)
+    flat = lambda z: np.concatenate([z.real.ravel(), z.imag.ravel()])
+    assert np.max(np.abs(flat(values) - flat(ref)) / (1e-11 + 1e-10 * np.abs(flat(ref)))) <= 1.0
+    full, _ = eng.forward(levels[0])
+    assert max(family_errors(values, full, eng.plan).values()) < 1e-12
+    with pytest.raises(RuntimeError, match="stale"):
+        eng.forward_levels(levels)
+        eng.vjp(values)
 
 
 def test_identity_is_checked():
@@ -192,17 +220,20 @@ def test_identity_is_checked():
         eng.vjp(values, generation=12, member=2, objective="a")
     with pytest.raises(RuntimeError):
         eng.vjp(values, generation=11, member=2, objective="b")
-    lib = eng._lib
+    lib = eng._lib.cdll
     key = (11, 3, 0)
     out = np.empty(eng.npix)
     c = np.ascontiguousarray(values.astype(np.complex128))
-    assert lib.stlp_vjp(eng._handle, c.ctypes.data_as(lib.stlp_vjp.argtypes[1]), c.size, *key,
+    assert lib.stlp_vjp(eng._h.ptr, c.ctypes.data_as(lib.stlp_vjp.argtypes[1]), c.size, *key,
                         out.ctypes.data_as(lib.stlp_vjp.argtypes[1])) == -1
     assert "stale" in lib.stlp_last_error().decode()
     eng.vjp(values, generation=11, member=2, objective="a")
 
 
 def test_checkpoint_restore():
+    """A VJP after restoring a checkpoint equals the oracle's gradient of the checkpointed
+    forward (and a fresh forward's, bit for bit); the reference's bridge test of checkpoint and
+    restore against fresh handles (crates/stl-bridge/tests/bridge.rs:202-240)."""
     nside, J = 16, 3
     eng = engine(nside, J)
     rng = np.random.default_rng(6)
@@ -214,6 +245,11 @@ def test_checkpoint_restore():
     gy = eng.vjp(vy, generation=1, objective="y")
     eng.restore(snap, generation=1, objective="x")
     gx = eng.vjp(vx, generation=1, objective="x")
+    ref = O.Engine(nside, J)
+    ref.load(x)
+    sx = ref.forward()
+    assert max(family_errors(vx, sx, eng.plan).values()) < STAT_BAR
+    assert rel(gx, ref.gradient(vx)) < GRAD_BAR
     vx2, _ = eng.forward(x, generation=2, objective="x")
     assert np.array_equal(vx, vx2)
     assert np.array_equal(eng.vjp(vx2, generation=2, objective="x"), gx)
@@ -348,3 +384,36 @@ def test_two_row_adjoint_with_odd_rows_agrees():
             assert max(family_errors(vb, out[3][0][b:b + 1], plan).values()) <= 1e-6
             gb = one.vjp(np.conj(vb) * 0.5 + 1.0)
             assert rel(gb, out[3][1][b:b + 1]) <= 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_work_queue_levels_agree_with_level_launches(precision):
+    """The coarse levels as one persistent work-queue launch per sweep (levels of nside <= 32
+    from level 2 on, the default) against a launch per kernel and level (queue_nside=0), on
+    a batch of 2 maps at nside 128 (J 6: levels 2..5 in the queue); and the queued FP64 engine
+    against the oracle at the reference's device bars."""
+    from stl_port import NativeEngine
+    nside, J = 128, 6
+    rng = np.random.default_rng(51)
+    x = rng.standard_normal((2, 12 * nside * nside))
+    out = {}
+    for q in (32, 0):
+        print(f"[gpu] work queue nside<={q} {precision}", flush=True)
+        with NativeEngine(nside, J, batch=2, precision=precision, queue_nside=q) as e:
+            assert e.info()["queue_level"] == (2 if q else J)
+            vals, var = e.forward(x, moments=True)
+            cot = rng.standard_normal(vals.shape) + 1j * rng.standard_normal(vals.shape) if q else out[32][3]
+            out[q] = (vals, var, e.vjp(cot), cot)
+            plan = e.plan
+    if precision == "fp64":
+        assert max(family_errors(out[32][0], out[0][0], plan).values()) < STAT_BAR
+        assert rel(out[32][1], out[0][1]) < VAR_BAR
+        assert rel(out[32][2], out[0][2]) < GRAD_BAR
+        ref = O.Engine(nside, J, B=2)
+        ref.load(x)
+        sx = ref.forward()
+        assert max(family_errors(out[32][0], sx, plan).values()) < STAT_BAR
+        assert rel(out[32][2], ref.gradient(out[32][3])) < GRAD_BAR
+    else:
+        assert max(family_errors(out[32][0], out[0][0], plan).values()) < 1e-6
+        assert rel(out[32][2], out[0][2]) < 1e-5

@@ -81,3 +81,32 @@ def test_shared_engines_match_standalone_ones():
     assert np.array_equal(v1b, v1)
     for e in (one, subset, pair):
         e.close()
+
+
+def test_shared_family_subset_matches_the_oracle():
+    """A shared engine with a plan of some families (S3 and S4 only, batch 1 of the owner's 2)
+    against the oracle's complete plan restricted to those rows, at the reference's device bars
+    (statistics 1e-13, gradient 1e-12 relative to the largest entry; crates/stl-gpu/tests/
+    stats.rs:248-275): values, and the gradient of a cotangent on those rows only."""
+    import oracle as O
+    from stl_port import NativeEngine
+    rng = np.random.default_rng(12)
+    x2 = rng.standard_normal((2, NPIX))
+    print(f"[gpu] share families S3,S4 nside={NSIDE} J={J} vs oracle", flush=True)
+    with NativeEngine(NSIDE, J, batch=2, precision="fp64") as owner:
+        sub = owner.share(batch=1, families=["S3", "S4"])
+        sel = np.array([i for i, c in enumerate(owner.plan) if c["family"] in ("S3", "S4")])
+        owner.forward(x2)  # the owner's forward first: the shared engine then takes its turn
+        vs, _ = sub.forward(x2[1:])
+        cot = rng.standard_normal(vs.shape) + 1j * rng.standard_normal(vs.shape)
+        gs = sub.vjp(cot)
+        sub.close()
+    ref = O.Engine(NSIDE, J)
+    ref.load(x2[1])
+    full = ref.forward()
+    assert vs.shape == (1, sel.size)
+    assert rel(vs[0], full[0, sel]) < 1e-13
+    cfull = np.zeros_like(full)
+    cfull[0, sel] = cot[0]
+    gref = ref.gradient(cfull)
+    assert rel(gs, gref) < 1e-12
