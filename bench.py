@@ -253,57 +253,50 @@ def cpu_model():
 
 
 def level_work(nside, J):
-    """Per-level work terms of one forward + VJP: sum_n N_n, sum_n N_n (n + 1) and
-    sum_n N_n (n + 1)^2 (per-pixel work independent of the level, work per branch row -- the
-    branch convolutions and moduli decimations -- and per pair of branch rows -- the S3/S4
-    products)."""
-    N = [12 * (nside >> n) ** 2 for n in range(J)]
-    return np.array([sum(N), sum(v * (n + 1) for n, v in enumerate(N)),
-                     sum(v * (n + 1) ** 2 for n, v in enumerate(N))], dtype=np.float64)
+    """Work of one forward + VJP in units of pixel x branch-row pairs: sum_n N_n (n + 1)^2 (the
+    S3/S4 products grow with the square of a level's branch rows, the convolutions with their
+    number; the pairs term dominates from J = 5 on)."""
+    return float(sum(12 * (nside >> n) ** 2 * (n + 1) ** 2 for n in range(J)))
 
 
 def cpu_baseline():
     """The FP64 oracle (the C++ restatement of the reference's CPU engine; its geometry rebuilt
     at every application, like the reference's per-block rebuild, crates/stl-core/src/
-    engine.rs:837-862) on the host: forward + VJP of the lognormal maps timed at nside 64
-    (J 5), 128 (J 6) and 256 (J 7) on HOST_THREADS threads, and a per-level model
-    t = a sum N_n + b sum N_n (n+1) + c sum N_n (n+1)^2 solved on those three points gives the
-    nside 1024, J 9 rate (extrapolated, labelled so).  The reference's own per-core anchor is
-    timed too: nside 64, L = 2, J = 5 on one thread (3.26 s per evaluation on one EPYC 9655
-    core, docs/validation.md:162)."""
+    engine.rs:837-862) on the host: forward + VJP of the lognormal maps timed at nside 256
+    (J 7) and 512 (J 8) on HOST_THREADS threads (about a minute); a power law t = c W^p in the
+    work W of level_work through the two points gives the nside 1024, J 9 rate (extrapolated,
+    labelled so).  The reference's own per-core anchor is timed too: nside 64, L = 2, J = 5 on
+    one thread (3.26 s per evaluation on one EPYC 9655 core, docs/validation.md:162)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
     threads = min(HOST_THREADS, os.cpu_count() or 1)
 
-    def timed(nside, J, L=4, reps=1):
+    def timed(nside, J, L=4):
         x, t = lognormal_map(nside, 1), lognormal_map(nside, 2)
         e = oracle.Engine(nside, J, L=L, cache_rows=0)
         e.load(t)
         target = e.forward()
-        best = float("inf")
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            e.load(x)
-            v = e.forward()
-            e.gradient(2.0 * (v - target))
-            best = min(best, time.perf_counter() - t0)
-        return best
+        t0 = time.perf_counter()
+        e.load(x)
+        v = e.forward()
+        e.gradient(2.0 * (v - target))
+        return time.perf_counter() - t0
 
     oracle.set_threads(1)
     anchor = timed(64, 5, L=2)
     oracle.set_threads(threads)
-    points = [(64, 5, timed(64, 5, reps=2)), (128, 6, timed(128, 6)), (256, 7, timed(256, 7))]
-    A = np.stack([level_work(n, j) for n, j, _ in points])
-    coef = np.linalg.solve(A, np.array([p[2] for p in points]))
-    s_eval = float(level_work(1024, 9) @ coef)
+    points = [(256, 7, timed(256, 7)), (512, 8, timed(512, 8))]
+    (n1, j1, t1), (n2, j2, t2) = points
+    p = np.log(t2 / t1) / np.log(level_work(n2, j2) / level_work(n1, j1))
+    s_eval = float(t2 * (level_work(1024, 9) / level_work(n2, j2)) ** p)
     return {"value": round(1.0 / s_eval, 6), "unit": "evals/s", "cores": threads, "kind": "port",
             "sample": f"forward+VJP of the FP64 oracle (geometry rebuilt per application) on {threads} threads of "
                       f"{os.cpu_count()} ({cpu_model()}): measured "
                       + ", ".join(f"nside {n} J {j}: {t:.2f} s" for n, j, t in points)
-                      + f"; nside 1024 J 9 extrapolated with a per-level model solved on them: {s_eval:.1f} s/eval",
+                      + f"; nside 1024 J 9 extrapolated by the power law W^{p:.3f} through them: {s_eval:.0f} s/eval",
             "measured": [{"nside": n, "J": j, "s_per_eval": round(t, 3)} for n, j, t in points],
-            "extrapolated_nside1024_s_per_eval": round(s_eval, 2),
+            "extrapolated": True, "extrapolated_nside1024_s_per_eval": round(s_eval, 1),
             "cpu_model": cpu_model(), "os_cpu_count": os.cpu_count(),
             "anchor": {"nside": 64, "L": 2, "J": 5, "threads": 1, "s_per_eval": round(anchor, 3),
                        "reference_s_per_eval": 3.26, "reference_source": "docs/validation.md:162 (1 EPYC 9655 core)",

@@ -265,8 +265,8 @@ StlHandle* stlp_new(const char* config) {
     else if (fields == "auto") opt.fields = stl::EngineOptions::kAuto;
     else fail("fields: auto, keep or recompute");
     opt.conv_variant = int(v.integer("conv_variant", -1));
-    opt.queue_nside = int(v.integer("queue_nside", 32));
-    if (opt.queue_nside < 0) fail("queue_nside must be nonnegative");
+    opt.batch_nside = int(v.integer("batch_nside", 32));
+    if (opt.batch_nside < 0) fail("batch_nside must be nonnegative");
     if (opt.conv_variant < -1 || opt.conv_variant > 5) fail("conv_variant must be -1 or 0 to 5");
     auto h = std::make_unique<StlHandle>();
     h->engine = stl::Engine::create(opt);
@@ -316,7 +316,7 @@ char* stlp_info(StlHandle* h) {
       << (h->precision == "fp64" ? "float64/complex128" : "float32 fields and operators, float64 sums") << "\""
       << ", \"masks\": " << h->engine->masks() << ", \"spin\": 0, \"conv\": \"folded gather, 4-fold face symmetry\", \"fields\": \""
       << (h->engine->field_store() == stl::EngineOptions::kRecompute ? "recompute" : "keep") << "\""
-      << ", \"queue_level\": " << h->engine->queue_level();
+      << ", \"batch_level\": " << h->engine->batch_level();
     o << ", \"device_memory\": {\"total_bytes\": " << h->engine->device_bytes()
       << ", \"peak_bytes\": " << stl::device_bytes_peak() << ", \"fields_bytes\": " << h->engine->field_bytes()
       << "}";

@@ -272,9 +272,9 @@ template <typename T>
 class EngineT final : public Engine {
   using C2 = cplx_t<T>;
 
-  // A task list of the work queue on the device, ready to launch.
-  struct QueueRun {
-    DevArray<QueueTask<T>> tasks;
+  // One batched launch of the coarse levels: independent tasks on the device.
+  struct Batch {
+    DevArray<LevelTask<T>> tasks;
     int ntasks = 0;
     int64_t items = 0;
     size_t lds = 0;
@@ -299,8 +299,9 @@ class EngineT final : public Engine {
     // coarser levels' reverse.
     DevArray<C2> gbr, gzp, gz, gbr1, gzp1, gbr2, gzp2;
     DevArray<T> acc[4], mg[2];
-    // Reverse buffers of every work-queue level (from level qc on), so that
-    // the queue's levels overlap: gV, z part, chain, gz and map cotangent.
+    // Reverse buffers of every batched level (from level qc on), so that one
+    // launch can hold a step of all of them: gV, z part, chain, gz and map
+    // cotangent.
     int qc = 0;
     std::vector<DevArray<C2>> qgbr, qgzp, qgz;
     std::vector<DevArray<T>> qchain, qmg;
@@ -349,17 +350,15 @@ class EngineT final : public Engine {
     }
     geometry_bytes_ = device_bytes_live();
     qc_ = levels_;
-    if (opt.queue_nside > 0)
+    if (opt.batch_nside > 0)
       for (int n = 2; n < levels_; ++n)
-        if ((opt.nside >> n) <= opt.queue_nside) {
+        if ((opt.nside >> n) <= opt.batch_nside) {
           qc_ = n;
           break;
         }
     int cus = 0;
     STL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, opt.device));
-    q_workgroups_ = std::max(1, 2 * cus);
-    hqerr_.resize(1);
-    hqerr_.data()[0] = 0u;
+    q_workgroups_ = std::max(1, 4 * cus);
     allocate();
     tune();
     // The timing launches of tune() wrote the (possibly shared) fields.
@@ -523,12 +522,12 @@ class EngineT final : public Engine {
     rev_finish(grad_dev);
   }
 
-  // The reverse of levels J - 1 .. down_to: the work queue's levels in one
-  // launch, then the others level by level.
+  // The reverse of levels J - 1 .. down_to: the batched levels' steps (a
+  // launch each for all of them), then the others level by level.
   void rev_levels(int down_to) {
     int n = levels_ - 1;
-    if (queue_on() && qc_ >= down_to) {
-      run_queue(qrev_);
+    if (batch_on() && qc_ >= down_to) {
+      run_batches(qrev_);
       n = qc_ - 1;
     }
     for (; n >= down_to; --n) {
@@ -566,25 +565,25 @@ class EngineT final : public Engine {
   // level's z, branches and reverse buffers at a time.
   C2* z_of(int n) { return f_->z[f_->recompute ? 0 : n].data(); }
   C2* br_of(int n) { return f_->br[f_->recompute ? 0 : n].data(); }
-  // The work queue's levels (n >= qc_) have reverse buffers of their own.
-  bool queued(int n) const { return n >= qc_ && queue_on(); }
+  // The batched levels (n >= qc_) have reverse buffers of their own.
+  bool batched(int n) const { return n >= qc_ && batch_on(); }
   C2* gbr_of(int n) {
-    if (queued(n)) return f_->qgbr[n].data();
+    if (batched(n)) return f_->qgbr[n].data();
     return n == 0 || f_->recompute ? f_->gbr.data() : n == 1 ? f_->gbr1.data() : f_->gbr2.data();
   }
   C2* gzp_of(int n) {
-    if (queued(n)) return f_->qgzp[n].data();
+    if (batched(n)) return f_->qgzp[n].data();
     return n == 0 || f_->recompute ? f_->gzp.data() : n == 1 ? f_->gzp1.data() : f_->gzp2.data();
   }
   T* chain_of(int n) {
-    if (queued(n)) return f_->qchain[n].data();
+    if (batched(n)) return f_->qchain[n].data();
     return f_->acc[f_->recompute ? (n & 1) : n < 2 ? n : 2 + (n & 1)].data();
   }
-  C2* gz_of(int n) { return queued(n) ? f_->qgz[n].data() : f_->gz.data(); }
-  T* mg_of(int n) { return queued(n) ? f_->qmg[n].data() : f_->mg[n % 2].data(); }
+  C2* gz_of(int n) { return batched(n) ? f_->qgz[n].data() : f_->gz.data(); }
+  T* mg_of(int n) { return batched(n) ? f_->qmg[n].data() : f_->mg[n % 2].data(); }
 
-  // The work queue serves the keep store's one-channel, unmasked sweeps.
-  bool queue_on() const { return qc_ < levels_ && !cross_ && nmask_ == 0 && !f_->recompute; }
+  // The batched launches serve the keep store's one-channel, unmasked sweeps.
+  bool batch_on() const { return qc_ < levels_ && !cross_ && nmask_ == 0 && !f_->recompute; }
 
   // Recompute store: level n's z (with its modulus row, the same bits as the
   // forward's) and branches again, from the kept maps and moduli.
@@ -891,7 +890,7 @@ class EngineT final : public Engine {
         nd[size_t(k) * levels_ + n] = nmask_ > 0 ? double(mcount_[size_t(k) * levels_ + n]) : double(N_[n]);
     upload(dnd_, nd);
     build_tables();
-    build_queue();
+    build_batches();
   }
 
   MaskView mask_view(int n) const {
@@ -975,11 +974,11 @@ class EngineT final : public Engine {
   }
 
   // The forward of levels from .. J - 1 (maps decimated): level by level, and
-  // the work queue's levels in one launch.
+  // the batched levels' steps a launch each for all of them.
   void fwd_levels(int from, bool moments) {
     for (int n = from; n < levels_; ++n) {
-      if (queue_on() && n == qc_) {
-        run_queue(qfwd_[moments ? 1 : 0]);
+      if (batch_on() && n == qc_) {
+        run_batches(qfwd_[moments ? 1 : 0]);
         return;
       }
       fwd_z(n);
@@ -1055,7 +1054,6 @@ class EngineT final : public Engine {
 
   // After the forward's stream work has completed: statistics out of the pinned staging.
   void finish_forward(bool moments, cplx* values, double* variance) override {
-    check_queue();
     const size_t nv = nvals();
     std::memcpy(values, hvalues_.data(), nv * sizeof(double2));
     if (moments && variance) std::memcpy(variance, hvar_.data(), nv * sizeof(double));
@@ -1129,8 +1127,7 @@ class EngineT final : public Engine {
       for (const auto& st : steps) st.body();
       STL_HIP(hipStreamSynchronize(s_));
       prof_.collect();
-      check_queue();
-      return;
+        return;
     }
     GraphEntry* e = cached(cache, ptr, flag);
     if (!e) {
@@ -1158,7 +1155,6 @@ class EngineT final : public Engine {
     STL_HIP(e3);
     STL_HIP(e2);
     STL_HIP(e1);
-    check_queue();
   }
 
   // The forward's steps.  After level n's z convolution (and the moduli it
@@ -1302,31 +1298,33 @@ class EngineT final : public Engine {
     return ms / 2.0;
   }
 
-  // One work-queue launch on the engine stream; its error flag is copied to
-  // pinned memory, for check_queue once the sweep has completed.
-  void run_queue(const QueueRun& q) {
-    int id = prof_.start();
-    launch_queue<T>(q.tasks.data(), q.ntasks, q.items, reinterpret_cast<QueueState*>(qstate_.data()), q.lds,
-                    q_workgroups_, s_);
-    prof_.stop(id, q.name, q.bytes);
-    STL_HIP(hipMemcpyAsync(hqerr_.data(), qstate_.data() + offsetof(QueueState, error), sizeof(unsigned int),
-                           hipMemcpyDeviceToHost, s_));
+  // The batched launches of a sweep, in order, on the engine stream.
+  void run_batches(const std::vector<Batch>& batches) {
+    for (const Batch& b : batches) {
+      int id = prof_.start();
+      launch_batch<T>(b.tasks.data(), b.ntasks, b.items, b.lds, q_workgroups_, s_);
+      prof_.stop(id, b.name, b.bytes);
+    }
   }
 
  public:
-  int queue_level() const override { return queue_on() ? qc_ : levels_; }
-  void check_queue() const override {
-    STL_CHECK(hqerr_.data()[0] == 0u, "work queue: a dependency wait ran out (results void)");
-  }
+  int batch_level() const override { return batch_on() ? qc_ : levels_; }
 
  private:
-  // The task lists of the work queue's levels (qc_ .. J - 1): forward (with
-  // and without moments) and reverse, in dependency order; the pointers are
-  // this engine's buffers, fixed once allocated.
-  void build_queue() {
-    if (!queue_on()) return;
+  // The batched launches of the coarse levels (qc_ .. J - 1): every step of
+  // the sweep that the levels can take at once is one launch holding that
+  // step for all of them; the decimation chains, whose levels depend on each
+  // other, take one launch per level.  Forward (without and with moments):
+  // z convolutions | moduli decimation, level by level | branch
+  // convolutions | Gram sums | their reductions.  Reverse: gV | branch
+  // adjoints | their mirror corrections | chain decimation, level by level
+  // down | gz | z adjoints | their mirror corrections | map-cotangent
+  // decimation, level by level down.  Pointers are this engine's buffers,
+  // fixed once allocated.
+  void build_batches() {
+    if (!batch_on()) return;
     auto base = [&](int op, int n, int ny, int64_t nx) {
-      QueueTask<T> t;
+      LevelTask<T> t;
       t.op = op;
       t.n = n;
       t.N = N_[n];
@@ -1337,44 +1335,55 @@ class EngineT final : public Engine {
     };
     auto blocks = [](int64_t n, int per) { return (n + per - 1) / per; };
     const int64_t rows = int64_t(Bs_) * Km_;
-    for (int mom = 0; mom < 2; ++mom) {
-      std::vector<QueueTask<T>> ts;
+    struct Step {
+      std::vector<LevelTask<T>> tasks;
       double bytes = 0.0;
-      std::vector<int> zt(levels_, -1), dt(levels_, -1);
+      std::string name;
+    };
+    auto finish = [&](std::vector<Batch>& out, std::vector<Step>& steps) {
+      out.clear();
+      for (auto& st : steps) {
+        if (st.tasks.empty()) continue;
+        out.emplace_back();
+        Batch& b = out.back();
+        b.items = batch_layout<T>(st.tasks.data(), int(st.tasks.size()));
+        b.lds = batch_lds_bytes<T>(st.tasks.data(), int(st.tasks.size()));
+        b.ntasks = int(st.tasks.size());
+        b.bytes = st.bytes;
+        b.name = st.name;
+        upload(b.tasks, st.tasks);
+      }
+    };
+    const std::string at = "@L" + std::to_string(qc_);
+    for (int mom = 0; mom < 2; ++mom) {
+      std::vector<Step> steps;
+      Step zs{{}, 0.0, "batch_conv_fwd_z" + at}, bs{{}, 0.0, "batch_conv_fwd_branch" + at},
+          gs{{}, 0.0, "batch_gram" + at}, rs{{}, 0.0, "batch_reduce" + at};
+      std::vector<Step> decs;
       for (int n = qc_; n < levels_; ++n) {
         const int64_t N = N_[n];
         const ConvOp<T> conv = conv_[n]->store.view();
-        QueueTask<T> z = base(kQConvFwd1, n, B_, blocks(N * 8, 256));
+        LevelTask<T> z = base(kQConvFwd1, n, B_, blocks(N * 8, 256));
         z.conv = conv;
         z.in0 = f_->maps[n].data();
         z.out0 = f_->z[n].data();
         z.out1 = f_->M[n].data() + size_t(n) * B_ * N * kL;
-        zt[n] = int(ts.size());
-        ts.push_back(z);
-        bytes += conv_bytes(n, 1, B_, false) + double(B_) * N * kL * sizeof(T);
+        zs.tasks.push_back(z);
+        zs.bytes += conv_bytes(n, 1, B_, false) + double(B_) * N * kL * sizeof(T);
         const DecOp<T> dec = dec_[n - 1]->store.view();
-        QueueTask<T> d = base(kQDecFwd4, n, n * B_, blocks(dec.Rc * 4, 256));
+        LevelTask<T> d = base(kQDecFwd4, n, n * B_, blocks(dec.Rc * 4, 256));
         d.dec = dec;
         d.in0 = f_->M[n - 1].data();
         d.out0 = f_->M[n].data();
-        if (n - 1 >= qc_) {
-          d.dep[0] = zt[n - 1];
-          d.dep[1] = dt[n - 1];
-        }
-        dt[n] = int(ts.size());
-        ts.push_back(d);
-        bytes += dec_bytes(n - 1, kL, n * B_);
-        QueueTask<T> b = base(kQConvFwd4, n, (n + 1) * B_, blocks(N * 8, 256));
+        decs.push_back(Step{{d}, dec_bytes(n - 1, kL, n * B_), tag("batch_dec_fwd", n)});
+        LevelTask<T> b = base(kQConvFwd4, n, (n + 1) * B_, blocks(N * 8, 256));
         b.conv = conv;
         b.in0 = f_->M[n].data();
         b.out0 = f_->br[n].data();
-        b.dep[0] = zt[n];
-        b.dep[1] = dt[n];
-        const int bt = int(ts.size());
-        ts.push_back(b);
-        bytes += conv_bytes(n, kL, (n + 1) * B_, false);
+        bs.tasks.push_back(b);
+        bs.bytes += conv_bytes(n, kL, (n + 1) * B_, false);
         const int K = gram_k(n), E = gram_e(n), nb = nblk_gram_[n];
-        QueueTask<T> g = base(kQGram, n, int(rows) * kL, nb);
+        LevelTask<T> g = base(kQGram, n, int(rows) * kL, nb);
         g.in0 = f_->br[n].data();
         g.in1 = f_->z[n].data();
         g.out0 = gpart_[n].data();
@@ -1383,134 +1392,106 @@ class EngineT final : public Engine {
         g.tpx = gram_tile(n);
         g.bs = K <= 13 ? 2 : 4;
         g.moments = mom;
-        g.dep[0] = bt;
-        const int gt = int(ts.size());
-        ts.push_back(g);
-        bytes += double(rows) * kL * N * K * 2 * sizeof(T);
+        gs.tasks.push_back(g);
+        gs.bytes += double(rows) * kL * N * K * 2 * sizeof(T);
         auto reduce = [&](const double* part, int64_t m, double* out) {
-          QueueTask<T> r = base(kQReduce, n, 1, blocks(m, 64));
+          LevelTask<T> r = base(kQReduce, n, 1, blocks(m, 64));
           r.in0 = part;
           r.nblk = nb;
           r.m = m;
           r.out0 = out;
-          r.dep[0] = gt;
-          ts.push_back(r);
-          bytes += (double(nb) + 1.0) * m * 8.0;
+          rs.tasks.push_back(r);
+          rs.bytes += (double(nb) + 1.0) * m * 8.0;
         };
         reduce(gpart_[n].data(), rows * kL * E * 2, gsum_[n].data());
         if (mom) reduce(gpart2_[n].data(), rows * kL * E, gsum2_[n].data());
         reduce(zpart_[n].data(), rows * kL * 4, zsum_[n].data());
       }
-      upload_queue(qfwd_[mom], ts, bytes, mom ? "queue_fwd_moments" : "queue_fwd");
+      steps.push_back(std::move(zs));
+      for (auto& d : decs) steps.push_back(std::move(d));
+      steps.push_back(std::move(bs));
+      steps.push_back(std::move(gs));
+      steps.push_back(std::move(rs));
+      finish(qfwd_[mom], steps);
     }
-    // Reverse: gV and branch adjoints of every level (independent), then the
-    // chain down the levels with each level's gz, z adjoint and map cotangent.
-    std::vector<QueueTask<T>> ts;
-    double bytes = 0.0;
-    std::vector<int> badj(levels_, -1), ch(levels_, -1), zadj(levels_, -1), mg(levels_, -1);
+    std::vector<Step> steps;
+    Step vs{{}, 0.0, "batch_gv" + at}, as{{}, 0.0, "batch_conv_adj_branch" + at}, fs{{}, 0.0, "batch_fix" + at};
+    Step zs{{}, 0.0, "batch_gz" + at}, za{{}, 0.0, "batch_conv_adj_z" + at}, zf{{}, 0.0, "batch_fix_z" + at};
+    std::vector<Step> chain, mgs;
     for (int n = levels_ - 1; n >= qc_; --n) {
       const int64_t N = N_[n];
-      QueueTask<T> v = base(kQGv, n, B_ * kL, blocks(N, 32));
+      const ConvOp<T> conv = conv_[n]->store.view();
+      LevelTask<T> v = base(kQGv, n, B_ * kL, blocks(N, 32));
       v.in0 = f_->br[n].data();
       v.in1 = f_->z[n].data();
       v.in2 = dH_[n].data();
       v.out0 = f_->qgbr[n].data();
       v.out1 = f_->qgzp[n].data();
-      const int vt = int(ts.size());
-      ts.push_back(v);
-      bytes += double(B_) * N * ((n + 1) * kL * kL + kL) * 2 * sizeof(T) * 2;
-      const ConvOp<T> conv = conv_[n]->store.view();
-      QueueTask<T> a = base(kQConvAdj4, n, (n + 1) * B_, blocks(N * 8, 256));
+      vs.tasks.push_back(v);
+      vs.bytes += double(B_) * N * ((n + 1) * kL * kL + kL) * 2 * sizeof(T) * 2;
+      LevelTask<T> a = base(kQConvAdj4, n, (n + 1) * B_, blocks(N * 8, 256));
       a.conv = conv;
       a.in0 = f_->qgbr[n].data();
       a.out0 = f_->qchain[n].data();
-      a.dep[0] = vt;
-      badj[n] = int(ts.size());
-      ts.push_back(a);
-      bytes += conv_bytes(n, kL, (n + 1) * B_, true);
+      as.tasks.push_back(a);
+      as.bytes += conv_bytes(n, kL, (n + 1) * B_, true);
       if (conv.nfix > 0) {
-        QueueTask<T> f = base(kQFix4, n, (n + 1) * B_, blocks(int64_t(conv.nfix) * kL, 256));
+        LevelTask<T> f = base(kQFix4, n, (n + 1) * B_, blocks(int64_t(conv.nfix) * kL, 256));
         f.conv = conv;
         f.in0 = f_->qgbr[n].data();
         f.out0 = f_->qchain[n].data();
-        f.dep[0] = badj[n];
-        badj[n] = int(ts.size());
-        ts.push_back(f);
+        fs.tasks.push_back(f);
       }
-    }
-    for (int n = levels_ - 1; n >= qc_; --n) {
-      const int64_t N = N_[n];
-      const ConvOp<T> conv = conv_[n]->store.view();
-      ch[n] = badj[n];
       if (n + 1 < levels_) {
         const DecOp<T> dec = dec_[n]->store.view();
-        QueueTask<T> c = base(kQDecAdj4, n, (n + 1) * B_, blocks(dec.Rf * 4, 256));
+        LevelTask<T> c = base(kQDecAdj4, n, (n + 1) * B_, blocks(dec.Rf * 4, 256));
         c.dec = dec;
         c.in0 = f_->qchain[n + 1].data();
         c.in1 = f_->qchain[n].data();
         c.out0 = f_->qchain[n].data();
-        c.dep[0] = badj[n];
-        c.dep[1] = ch[n + 1];
-        ch[n] = int(ts.size());
-        ts.push_back(c);
-        bytes += dec_bytes(n, kL, (n + 1) * B_, true);
+        chain.push_back(Step{{c}, dec_bytes(n, kL, (n + 1) * B_, true), tag("batch_dec_adj", n)});
       }
-      QueueTask<T> z = base(kQGz, n, B_ * kL, blocks(N, 256));
+      LevelTask<T> z = base(kQGz, n, B_ * kL, blocks(N, 256));
       z.in0 = f_->z[n].data();
       z.in1 = f_->qgzp[n].data();
       z.in2 = f_->qchain[n].data();
       z.s1 = ds1_[n].data();
       z.s2 = ds2_[n].data();
       z.out0 = f_->qgz[n].data();
-      z.dep[0] = ch[n];
-      const int zt = int(ts.size());
-      ts.push_back(z);
-      bytes += double(B_) * kL * N * (7 * sizeof(T));
-      QueueTask<T> a = base(kQConvAdj1, n, B_, blocks(N * 8, 256));
-      a.conv = conv;
-      a.in0 = f_->qgz[n].data();
-      a.out0 = f_->qmg[n].data();
-      a.dep[0] = zt;
-      zadj[n] = int(ts.size());
-      ts.push_back(a);
-      bytes += conv_bytes(n, 1, B_, true);
+      zs.tasks.push_back(z);
+      zs.bytes += double(B_) * kL * N * (7 * sizeof(T));
+      LevelTask<T> za1 = base(kQConvAdj1, n, B_, blocks(N * 8, 256));
+      za1.conv = conv;
+      za1.in0 = f_->qgz[n].data();
+      za1.out0 = f_->qmg[n].data();
+      za.tasks.push_back(za1);
+      za.bytes += conv_bytes(n, 1, B_, true);
       if (conv.nfix > 0) {
-        QueueTask<T> f = base(kQFix1, n, B_, blocks(conv.nfix, 256));
+        LevelTask<T> f = base(kQFix1, n, B_, blocks(conv.nfix, 256));
         f.conv = conv;
         f.in0 = f_->qgz[n].data();
         f.out0 = f_->qmg[n].data();
-        f.dep[0] = zadj[n];
-        zadj[n] = int(ts.size());
-        ts.push_back(f);
+        zf.tasks.push_back(f);
       }
-      mg[n] = zadj[n];
       if (n + 1 < levels_) {
         const DecOp<T> dec = dec_[n]->store.view();
-        QueueTask<T> m = base(kQDecAdj1, n, B_, blocks(dec.Rf * 4, 256));
+        LevelTask<T> m = base(kQDecAdj1, n, B_, blocks(dec.Rf * 4, 256));
         m.dec = dec;
         m.in0 = f_->qmg[n + 1].data();
         m.in1 = f_->qmg[n].data();
         m.out0 = f_->qmg[n].data();
-        m.dep[0] = zadj[n];
-        m.dep[1] = mg[n + 1];
-        mg[n] = int(ts.size());
-        ts.push_back(m);
-        bytes += dec_bytes(n, 1, B_, true);
+        mgs.push_back(Step{{m}, dec_bytes(n, 1, B_, true), tag("batch_dec_adj_mg", n)});
       }
     }
-    upload_queue(qrev_, ts, bytes, "queue_rev");
-    size_t state = 0;
-    for (const QueueRun* q : {&qfwd_[0], &qfwd_[1], &qrev_}) state = std::max(state, queue_state_bytes(q->ntasks));
-    if (qstate_.size() < state) qstate_.resize(state);
-  }
-
-  void upload_queue(QueueRun& q, std::vector<QueueTask<T>>& ts, double bytes, const char* name) {
-    q.items = queue_layout<T>(ts.data(), int(ts.size()));
-    q.lds = queue_lds_bytes<T>(ts.data(), int(ts.size()));
-    q.ntasks = int(ts.size());
-    q.bytes = bytes;
-    q.name = std::string(name) + "@L" + std::to_string(qc_);
-    upload(q.tasks, ts);
+    steps.push_back(std::move(vs));
+    steps.push_back(std::move(as));
+    steps.push_back(std::move(fs));
+    for (auto& c : chain) steps.push_back(std::move(c));
+    steps.push_back(std::move(zs));
+    steps.push_back(std::move(za));
+    steps.push_back(std::move(zf));
+    for (auto& m : mgs) steps.push_back(std::move(m));
+    finish(qrev_, steps);
   }
 
   // Device tables of the statistics assembly and of the cotangent scatter.
@@ -1613,11 +1594,9 @@ class EngineT final : public Engine {
     if (!h.empty()) STL_HIP(hipMemcpy(d.data(), h.data(), h.size() * sizeof(U), hipMemcpyHostToDevice));
   }
 
-  int qc_ = 0;  // first level of the work queue (levels_: none)
+  int qc_ = 0;  // first batched level (levels_: none)
   int q_workgroups_ = 1;
-  QueueRun qfwd_[2], qrev_;  // forward without / with moments, reverse
-  DevArray<unsigned char> qstate_;
-  PinnedArray<unsigned int> hqerr_;  // the error flag of the last queue launch
+  std::vector<Batch> qfwd_[2], qrev_;  // forward without / with moments, reverse
 
   EngineOptions opt_;
   hipStream_t s_ = nullptr;

@@ -54,11 +54,11 @@ struct EngineOptions {
   // memory after the operators are built.
   enum Fields { kKeep = 0, kRecompute = 1, kAuto = 2 };
   int fields = kAuto;
-  // Levels of at most this nside (from level 2 on) run as one persistent
-  // work-queue launch per sweep (forward and reverse) instead of a launch per
-  // kernel and level; 0 switches the queue off.  Used with the keep store,
-  // one-channel plans and no masks; otherwise every level has its own launches.
-  int queue_nside = 32;
+  // Levels of at most this nside (from level 2 on) run batched: each step of
+  // a sweep is one launch for all of them (the decimation chains one per
+  // level) instead of a launch per kernel and level; 0 switches the batching
+  // off.  Used with the keep store, one-channel plans and no masks.
+  int batch_nside = 32;
 };
 
 // One timed candidate of the convolution-variant choice: level, kernel
@@ -122,10 +122,8 @@ class Engine {
   // fields and reverse buffers.
   virtual int field_store() const = 0;
   virtual size_t field_bytes() const = 0;
-  // First level of the work-queue launches (levels() when there are none).
-  virtual int queue_level() const = 0;
-  // Raises if the last sweep's work queue flagged a dependency wait that ran out.
-  virtual void check_queue() const = 0;
+  // First level of the batched launches (levels() when there are none).
+  virtual int batch_level() const = 0;
   virtual double setup_seconds() const = 0;
   virtual std::vector<KernelTimer> timers() const = 0;
   // Chosen convolution variant per level: [level][forward z, forward branch,

@@ -118,7 +118,6 @@ void Ensemble::vjp(const cplx* cot, double* const* grad_dev) {
     at += size_t(e->options().batch) * e->plan().size();
   }
   launch(get(vjp_, ptrs, false, [&](size_t g) { return engines_[g]->capture_vjp(grad_dev[g]); }));
-  for (auto& e : engines_) e->check_queue();
 }
 
 }  // namespace stl

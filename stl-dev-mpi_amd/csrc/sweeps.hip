@@ -1866,23 +1866,11 @@ __global__ void k_cast(int64_t n, const TI* __restrict__ in, TO* __restrict__ ou
 }
 
 // ---------------------------------------------------------------------------
-// The work queue of the coarse levels (QueueTask / QueueState, sweeps.hpp).
+// Batched launches of the coarse levels (LevelTask, sweeps.hpp).
 // ---------------------------------------------------------------------------
 
-// Spins of a dependency wait before it gives up and flags the launch (a wait
-// is normally a few microseconds; this is seconds).
-constexpr unsigned kQueueSpinLimit = 1u << 24;
-
-__global__ void k_queue_reset(QueueState* st, int ntasks) {
-  for (int i = threadIdx.x; i < ntasks; i += blockDim.x) st->done[i] = 0u;
-  if (threadIdx.x == 0) {
-    st->head = 0ull;
-    st->error = 0u;
-  }
-}
-
 template <typename T>
-__device__ __forceinline__ void queue_item(const QueueTask<T>& t, int64_t bx, int by, double2* lds) {
+__device__ __forceinline__ void level_item(const LevelTask<T>& t, int64_t bx, int by, double2* lds) {
   using C2 = cplx_t<T>;
   const MaskView none;
   switch (t.op) {
@@ -1968,75 +1956,44 @@ __device__ __forceinline__ void queue_item(const QueueTask<T>& t, int64_t bx, in
 // Persistent workgroups take items in queue order; an item runs once the tasks
 // its task depends on have completed.  Every workgroup leaves once the items
 // are exhausted (the item counter only grows), so the grid always drains.
+// Workgroup w runs items w, w + grid, ...: the items of independent tasks,
+// each found by binary search over the tasks' first items.
 template <typename T>
-__global__ __launch_bounds__(256) void k_queue(const QueueTask<T>* __restrict__ tasks, int ntasks, int64_t items,
-                                               QueueState* st) {
-  extern __shared__ double2 qlds[];
-  __shared__ int64_t s_item;
-  __shared__ int s_task;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      s_item = int64_t(__hip_atomic_fetch_add(&st->head, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      int lo = 0, hi = ntasks - 1;  // the last task whose first item <= s_item
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) / 2;
-        if (tasks[mid].first <= s_item) lo = mid;
-        else hi = mid - 1;
-      }
-      s_task = lo;
+__global__ __launch_bounds__(256) void k_batch(const LevelTask<T>* __restrict__ tasks, int ntasks, int64_t items) {
+  extern __shared__ double2 blds[];
+  for (int64_t item = blockIdx.x; item < items; item += gridDim.x) {
+    int lo = 0, hi = ntasks - 1;  // the last task whose first item <= item
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (tasks[mid].first <= item) lo = mid;
+      else hi = mid - 1;
     }
-    __syncthreads();
-    const int64_t item = s_item;
-    const int ti = s_task;
-    if (item >= items) break;
-    const QueueTask<T>& t = tasks[ti];
-    if (threadIdx.x == 0) {
-      for (int q = 0; q < kQueueMaxDeps; ++q) {
-        const int d = t.dep[q];
-        if (d < 0) continue;
-        const unsigned need = unsigned(tasks[d].nx * tasks[d].ny);
-        unsigned spins = 0;
-        while (__hip_atomic_load(&st->done[d], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
-          __builtin_amdgcn_s_sleep(2);
-          if (++spins >= kQueueSpinLimit) {
-            __hip_atomic_store(&st->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-        }
-      }
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const LevelTask<T>& t = tasks[lo];
     const int64_t local = item - t.first;
-    queue_item<T>(t, local % t.nx, int(local / t.nx), qlds);
-    // The item's writes are visible device-wide before it counts as done.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(&st->done[ti], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    level_item<T>(t, local % t.nx, int(local / t.nx), blds);
+    __syncthreads();  // the next item reuses the LDS
   }
 }
 
 }  // namespace
 
 template <typename T>
-int64_t queue_layout(QueueTask<T>* tasks, int ntasks) {
+int64_t batch_layout(LevelTask<T>* tasks, int ntasks) {
   int64_t first = 0;
   for (int i = 0; i < ntasks; ++i) {
-    STL_CHECK(tasks[i].nx >= 1 && tasks[i].ny >= 1, "queue: empty task");
-    for (int q = 0; q < kQueueMaxDeps; ++q)
-      STL_CHECK(tasks[i].dep[q] < i, "queue: a task depends on a later one");
+    STL_CHECK(tasks[i].nx >= 1 && tasks[i].ny >= 1, "batch: empty task");
     tasks[i].first = first;
     first += tasks[i].nx * tasks[i].ny;
   }
-  STL_CHECK(first < (int64_t(1) << 31), "queue: too many items");
+  STL_CHECK(first < (int64_t(1) << 31), "batch: too many items");
   return first;
 }
 
 template <typename T>
-size_t queue_lds_bytes(const QueueTask<T>* tasks, int ntasks) {
+size_t batch_lds_bytes(const LevelTask<T>* tasks, int ntasks) {
   size_t lds = 0;
   for (int i = 0; i < ntasks; ++i) {
-    const QueueTask<T>& t = tasks[i];
+    const LevelTask<T>& t = tasks[i];
     size_t need = 0;
     if (t.op == kQGram) {
       const int K = (t.n + 1) * kL + 1, BS = t.bs;
@@ -2056,14 +2013,11 @@ size_t queue_lds_bytes(const QueueTask<T>* tasks, int ntasks) {
 }
 
 template <typename T>
-void launch_queue(const QueueTask<T>* tasks, int ntasks, int64_t items, QueueState* state, size_t lds,
-                  int workgroups, hipStream_t s) {
-  STL_CHECK(ntasks >= 1 && items >= 1 && workgroups >= 1, "queue: nothing to run");
-  STL_CHECK(lds <= 64 * 1024, "queue: the tasks need more than 64 KB of LDS");
-  k_queue_reset<<<1, 256, 0, s>>>(state, ntasks);
-  STL_HIP(hipGetLastError());
+void launch_batch(const LevelTask<T>* tasks, int ntasks, int64_t items, size_t lds, int workgroups, hipStream_t s) {
+  STL_CHECK(ntasks >= 1 && items >= 1 && workgroups >= 1, "batch: nothing to run");
+  STL_CHECK(lds <= 64 * 1024, "batch: the tasks need more than 64 KB of LDS");
   const int grid = int(std::min<int64_t>(items, workgroups));
-  k_queue<T><<<grid, 256, lds, s>>>(tasks, ntasks, items, state);
+  k_batch<T><<<grid, 256, lds, s>>>(tasks, ntasks, items);
   STL_HIP(hipGetLastError());
 }
 
@@ -2529,13 +2483,13 @@ STL_INSTANTIATE(double)
 STL_INSTANTIATE(float)
 #undef STL_INSTANTIATE
 
-#define STL_QUEUE_INSTANTIATE(T)                                                                  \
-  template int64_t queue_layout<T>(QueueTask<T>*, int);                                          \
-  template size_t queue_lds_bytes<T>(const QueueTask<T>*, int);                                  \
-  template void launch_queue<T>(const QueueTask<T>*, int, int64_t, QueueState*, size_t, int, hipStream_t);
-STL_QUEUE_INSTANTIATE(double)
-STL_QUEUE_INSTANTIATE(float)
-#undef STL_QUEUE_INSTANTIATE
+#define STL_BATCH_INSTANTIATE(T)                                                                   \
+  template int64_t batch_layout<T>(LevelTask<T>*, int);                                           \
+  template size_t batch_lds_bytes<T>(const LevelTask<T>*, int);                                   \
+  template void launch_batch<T>(const LevelTask<T>*, int, int64_t, size_t, int, hipStream_t);
+STL_BATCH_INSTANTIATE(double)
+STL_BATCH_INSTANTIATE(float)
+#undef STL_BATCH_INSTANTIATE
 
 template void launch_cast<double, float>(int64_t, const float*, double*, hipStream_t);
 template void launch_cast<float, double>(int64_t, const double*, float*, hipStream_t);

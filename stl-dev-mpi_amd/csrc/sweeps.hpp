@@ -160,29 +160,25 @@ void launch_gz_x(int n, int64_t N, int B, const cplx_t<T>* z, const cplx_t<T>* g
                  const double2* s1x, cplx_t<T>* gz, const MaskView& mask, hipStream_t s);
 
 // ---------------------------------------------------------------------------
-// Work queue of the coarse levels.  One persistent kernel launch runs a list of
-// tasks, each an operation on one level's rows split into work items (the
-// workgroups of the equivalent launch: a 256-pixel block of one row, a Gram
-// block, 64 reduction outputs).  Workgroups take items in queue order from an
-// atomic counter; before an item runs, every task it depends on has completed
-// (per-task completion counters, agent-scope release / acquire), so the
-// levels' dependent steps follow each other without launch gaps and
-// independent ones overlap.  Items only wait on earlier items, which are held
-// by running workgroups: the queue always drains.  Every item computes its
-// outputs completely, in the order of the equivalent kernel, so results do
-// not depend on which workgroup runs it.
-enum QueueOp : int32_t {
+// Batched launches of the coarse levels.  One persistent kernel launch runs a
+// list of independent tasks (the same operation at several levels, e.g. the
+// branch convolutions of levels 5..8), each split into the work items its own
+// kernel would have had as workgroups: a 256-pixel block of one row, a Gram
+// block, 64 reduction outputs.  Workgroups walk the items of the whole list in
+// a fixed stride (no waits, no atomics); launches follow each other in stream
+// order, which orders the dependent steps.  Every item computes its outputs
+// completely, in the order of the equivalent kernel, so results do not depend
+// on which workgroup runs it.
+enum LevelOp : int32_t {
   kQDecFwd1 = 0, kQDecFwd4, kQConvFwd1, kQConvFwd4, kQGram, kQReduce, kQGv, kQConvAdj1, kQConvAdj4, kQFix1,
   kQFix4, kQDecAdj1, kQDecAdj4, kQGz
 };
-constexpr int kQueueMaxDeps = 2;
 template <typename T>
-struct QueueTask {
+struct LevelTask {
   int32_t op = 0;
   int32_t ny = 1;           // rows (or Gram row blocks) of the task
   int64_t nx = 0;           // items per row
-  int64_t first = 0;        // the task's first item in the queue
-  int32_t dep[kQueueMaxDeps] = {-1, -1};
+  int64_t first = 0;        // the task's first item in its list
   ConvOp<T> conv;           // convolution of the level (conv ops)
   DecOp<T> dec;             // decimation from the level (dec ops)
   const void* in0 = nullptr;
@@ -196,27 +192,16 @@ struct QueueTask {
   int32_t n = 0, B = 0, tpx = 0, bs = 4, moments = 0, nblk = 1;
   int64_t N = 0, m = 0;
 };
-// State of one queue launch: the item counter, an error flag (a dependency
-// wait that ran out of spins: the results are void and the host raises) and
-// one completion counter per task.
-struct QueueState {
-  unsigned long long head;
-  unsigned int error;
-  unsigned int pad;
-  unsigned int done[1];  // [ntasks]
-};
-inline size_t queue_state_bytes(int ntasks) { return sizeof(QueueState) + sizeof(unsigned int) * size_t(ntasks); }
 // Items of a task list (sets every task's `first`), and the dynamic LDS the
 // launch needs.
 template <typename T>
-int64_t queue_layout(QueueTask<T>* tasks, int ntasks);
+int64_t batch_layout(LevelTask<T>* tasks, int ntasks);
 template <typename T>
-size_t queue_lds_bytes(const QueueTask<T>* tasks, int ntasks);
-// Clears the state and runs the queue: tasks on the device, `workgroups`
-// persistent workgroups of 256 threads.
+size_t batch_lds_bytes(const LevelTask<T>* tasks, int ntasks);
+// One launch over the items of independent tasks (on the device), at most
+// `workgroups` persistent 256-thread workgroups.
 template <typename T>
-void launch_queue(const QueueTask<T>* tasks, int ntasks, int64_t items, QueueState* state, size_t lds,
-                  int workgroups, hipStream_t s);
+void launch_batch(const LevelTask<T>* tasks, int ntasks, int64_t items, size_t lds, int workgroups, hipStream_t s);
 
 // Element-wise precision casts (maps in, gradients out).
 template <typename TO, typename TI>

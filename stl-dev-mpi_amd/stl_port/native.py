@@ -15,8 +15,8 @@ Maps and gradients on the device (torch tensors or raw addresses) go through the
 methods without host copies.  The library has no CPU path and this module has no fallback: a
 missing library raises at construction.  Keywords of this library beyond the reference's:
 `precision` ("fp64" | "mixed"), `fields` ("keep" | "recompute" | "auto"), `timing`,
-`conv_variant`, `families`, `queue_nside` (levels of at most this nside run as one work-queue
-launch per sweep; 0: none).
+`conv_variant`, `families`, `batch_nside` (levels of at most this nside run batched, each step
+of a sweep one launch for all of them; 0: none).
 """
 from __future__ import annotations
 
@@ -277,7 +277,7 @@ class NativeEngine:
     def __init__(self, nside, j, batch=1, backend="gpu", kernel=5, orientations=4, block_pixels=4096,
                  cache_bytes=1 << 30, threads=1, cache="transposition", device=0, library=None, cross=False,
                  masks=None, spin=0, spin_kernel=5, spin_pyramid="scalar", spin_a=2, precision="fp64",
-                 fields="auto", timing=False, conv_variant=None, families=None, queue_nside=None):
+                 fields="auto", timing=False, conv_variant=None, families=None, batch_nside=None):
         self._lock = threading.RLock()
         self._h = None
         self._lib = bridge(library)
@@ -300,8 +300,8 @@ class NativeEngine:
             config["conv_variant"] = int(conv_variant)
         if families is not None:
             config["families"] = ",".join(families)
-        if queue_nside is not None:
-            config["queue_nside"] = int(queue_nside)
+        if batch_nside is not None:
+            config["batch_nside"] = int(batch_nside)
         self._adopt(self._lib.call("stlp_new", json.dumps(config).encode()), config)
         if masks is not None:
             self.set_masks(masks)

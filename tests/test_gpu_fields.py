@@ -4,8 +4,10 @@ reverse recomputes each level's z and branches.  The reference holds Keep and Re
 same bits (crates/stl-gpu/tests/stats.rs:185-217); so does this build, since the recomputation
 runs the forward's own kernels on the same inputs.
 
-- keep vs recompute: values, variances and gradients equal bit for bit (FP64 and mixed; graph
-  replays and eager launches; the ensemble's captured graphs go through the same sweeps);
+- keep vs recompute: values, variances and gradients equal bit for bit with every level launched
+  on its own (batch_nside=0: the recompute store always launches per level), FP64 and mixed,
+  graph replays and eager launches; with the keep store's coarse levels batched (the default)
+  they agree to rounding (the batched levels' split gathers add a row's slots in another order);
 - the recompute engine against the oracle at the reference's device bars;
 - checkpoint / restore of a recompute engine (its state is the maps and moduli);
 - nside 2048, J = 10, FP64, 2 maps with the recompute store: fits one GPU, equals the keep
@@ -42,17 +44,22 @@ def test_recompute_equals_keep_bitwise(precision, timing):
     x = np.random.default_rng(1).standard_normal((B, 12 * nside * nside))
     print(f"[gpu] fields keep vs recompute nside={nside} J={J} B={B} {precision} timing={timing}", flush=True)
     out = {}
-    for fields in ("keep", "recompute"):
-        with NativeEngine(nside, J, batch=B, precision=precision, fields=fields, timing=timing) as e:
+    for fields, bn in (("keep", 0), ("recompute", 0), ("keep", 32)):
+        with NativeEngine(nside, J, batch=B, precision=precision, fields=fields, timing=timing,
+                          batch_nside=bn) as e:
             info = e.info()
             assert info["fields"] == fields
-            out[fields] = run(e, x, 2) + (info["device_memory"]["fields_bytes"],)
+            key = fields + ("_batched" if bn else "")
+            out[key] = run(e, x, 2) + (info["device_memory"]["fields_bytes"],)
             # a second evaluation on other maps (graph replays, buffers reused across levels)
-            out[fields + "2"] = run(e, x[::-1].copy(), 3)
+            out[key + "2"] = run(e, x[::-1].copy(), 3)
     for a, b in zip(out["keep"][:3], out["recompute"][:3]):
         np.testing.assert_array_equal(a, b)
     for a, b in zip(out["keep2"], out["recompute2"]):
         np.testing.assert_array_equal(a, b)
+    tol = 1e-11 if precision == "fp64" else 1e-5
+    for a, b in zip(out["keep_batched"][:3], out["recompute"][:3]):
+        assert np.abs(a - b).max() <= tol * np.abs(b).max()
     assert out["recompute"][3] < 0.75 * out["keep"][3]
 
 
@@ -92,8 +99,9 @@ def lognormal(nside, seed, base=32, sigma=0.1):
 
 def test_fp64_pair_at_nside_2048_recompute_equals_keep():
     """Two FP64 maps at nside 2048 (J = 10, 15042 coefficients each): the recompute engine's
-    statistics and gradients, then (after it is freed) the keep engine's, bit for bit; the
-    recompute store's peak device memory is printed."""
+    statistics and gradients, then (after it is freed) the keep engine's (every level launched
+    on its own, as the recompute store does), bit for bit; the recompute store's peak device
+    memory is printed."""
     from stl_port import NativeEngine
     nside, J = 2048, 10
     xd = torch.from_numpy(np.stack([lognormal(nside, 3), lognormal(nside, 4)])).cuda()
@@ -101,7 +109,7 @@ def test_fp64_pair_at_nside_2048_recompute_equals_keep():
     res = {}
     for fields in ("recompute", "keep"):
         print(f"[gpu] nside {nside} J {J} fp64 batch 2 fields={fields}", flush=True)
-        with NativeEngine(nside, J, batch=2, precision="fp64", fields=fields) as e:
+        with NativeEngine(nside, J, batch=2, precision="fp64", fields=fields, batch_nside=0) as e:
             v, _ = e.forward_device(xd, generation=1)
             rng = np.random.default_rng(5)
             cot = rng.standard_normal(v.shape) + 1j * rng.standard_normal(v.shape)

@@ -386,6 +386,8 @@ def test_two_row_adjoint_with_odd_rows_agrees():
             assert rel(gb, out[3][1][b:b + 1]) <= 1e-5
 
 
+@pytest.mark.skip(reason="the work-queue path is switched off: its first two GPU runs did not finish "
+                         "(the stats-fixture forward at nside 16 hung inside the queue launch)")
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_work_queue_levels_agree_with_level_launches(precision):
     """The coarse levels as one persistent work-queue launch per sweep (levels of nside <= 32
@@ -417,3 +419,42 @@ def test_work_queue_levels_agree_with_level_launches(precision):
     else:
         assert max(family_errors(out[32][0], out[0][0], plan).values()) < 1e-6
         assert rel(out[32][2], out[0][2]) < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_levels_agree_with_level_launches(precision):
+    """The coarse levels batched (levels of nside <= 32 from level 2 on, the default: each step
+    of a sweep one launch for all of them) against a launch per kernel and level
+    (batch_nside=0), on a batch of 2 maps at nside 128 (J 6: levels 2..5 batched), graph
+    replays and eager launches; and the batched FP64 engine against the oracle at the
+    reference's device bars."""
+    from stl_port import NativeEngine
+    nside, J = 128, 6
+    rng = np.random.default_rng(51)
+    x = rng.standard_normal((2, 12 * nside * nside))
+    cot = None
+    out = {}
+    for bn, timing in ((32, False), (32, True), (0, False)):
+        print(f"[gpu] batched levels nside<={bn} timing={timing} {precision}", flush=True)
+        with NativeEngine(nside, J, batch=2, precision=precision, batch_nside=bn, timing=timing) as e:
+            assert e.info()["batch_level"] == (2 if bn else J)
+            vals, var = e.forward(x, moments=True)
+            if cot is None:
+                cot = rng.standard_normal(vals.shape) + 1j * rng.standard_normal(vals.shape)
+            out[(bn, timing)] = (vals, var, e.vjp(cot))
+            plan = e.plan
+    for a, b in zip(out[(32, False)], out[(32, True)]):
+        np.testing.assert_array_equal(a, b)
+    got, ref0 = out[(32, False)], out[(0, False)]
+    if precision == "fp64":
+        assert max(family_errors(got[0], ref0[0], plan).values()) < STAT_BAR
+        assert rel(got[1], ref0[1]) < VAR_BAR
+        assert rel(got[2], ref0[2]) < GRAD_BAR
+        ref = O.Engine(nside, J, B=2)
+        ref.load(x)
+        sx = ref.forward()
+        assert max(family_errors(got[0], sx, plan).values()) < STAT_BAR
+        assert rel(got[2], ref.gradient(cot)) < 2 * GRAD_BAR
+    else:
+        assert max(family_errors(got[0], ref0[0], plan).values()) < 1e-6
+        assert rel(got[2], ref0[2]) < 1e-5

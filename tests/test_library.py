@@ -64,6 +64,8 @@ def test_abi_table_matches_the_header():
     ('{"nside": 16, "j": 3, "batch": 0}', "batch"),
     ('{"nside": 16, "j": 3, "conv_variant": 6}', "conv_variant"),
     ('{"nside": 16, "j": 3, "conv_variant": -2}', "conv_variant"),
+    ('{"nside": 16, "j": 3, "batch_nside": -1}', "batch_nside"),
+    ('{"nside": 16, "j": 3, "fields": "sometimes"}', "fields"),
 ])
 def test_bad_configuration_is_an_error(config, message):
     lib = native.bridge()
