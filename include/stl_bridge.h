@@ -205,6 +205,15 @@ int stlx_ensemble_vjp_device(StlEnsemble* e, const double* cot, size_t n, const 
  * 1 conv adjoint (gy -> gx), 2 decimation forward level -> level + 1,
  * 3 decimation adjoint.  Arrays are float64 on the device. */
 int stlx_apply_operator(StlHandle* h, int kind, int level, int rows, uint64_t in_dev, uint64_t out_dev);
+/* stlx_apply_operator without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * The staging of the operator's input and output lives in a static device
+ * scratch of 64 MiB per GPU (held from the library's load until the process
+ * ends); an application whose staging exceeds it fails with an error naming
+ * the sizes.  Every call on one GPU shares that scratch: calls that overlap
+ * on one GPU race.  Results equal stlx_apply_operator's bit for bit. */
+int stlx_apply_operator_noalloc(StlHandle* h, int kind, int level, int rows, uint64_t in_dev, uint64_t out_dev);
 
 /* Measurement hook (no counterpart in the reference): mean milliseconds of one
  * copy of `bytes` (a multiple of 16; 16-byte aligned device addresses) from

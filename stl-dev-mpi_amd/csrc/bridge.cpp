@@ -140,6 +140,39 @@ std::string json_opt(int v) { return v < 0 ? "null" : std::to_string(v); }
 
 }  // namespace
 
+// Static device scratch of the allocation-free entry point
+// (stlx_apply_operator_noalloc): the staging of one operator application, at
+// most kScratchBytes.  Each device holds its own copy from the library's load
+// until the process ends; host code uses only the per-device address from
+// hipGetSymbolAddress (scratch_address), never the symbol itself.  Calls that
+// overlap on one device share it and race.
+namespace {
+constexpr size_t kScratchBytes = size_t(64) << 20;
+}
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+// Address of `scratch` on a device, looked up once per device with that device
+// current (at handle creation, the library's set-up of a device).
+void* scratch_address(int device) {
+  static std::mutex lock;
+  static std::map<int, void*> addresses;
+  std::lock_guard<std::mutex> guard_lock(lock);
+  auto it = addresses.find(device);
+  if (it != addresses.end()) return it->second;
+  int current = 0;
+  STL_HIP(hipGetDevice(&current));
+  STL_HIP(hipSetDevice(device));
+  void* ptr = nullptr;
+  const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  STL_HIP(hipSetDevice(current));
+  if (err != hipSuccess || ptr == nullptr)
+    throw std::runtime_error(std::string("scratch address: ") + hipGetErrorString(err));
+  addresses.emplace(device, ptr);
+  return ptr;
+}
+}  // namespace
+
 struct StlHandle {
   std::unique_ptr<stl::Engine> engine;
   uint64_t id = 0;
@@ -237,6 +270,7 @@ StlHandle* stlp_new(const char* config) {
     if (opt.conv_variant < -1 || opt.conv_variant > 5) fail("conv_variant must be -1 or 0 to 5");
     auto h = std::make_unique<StlHandle>();
     h->engine = stl::Engine::create(opt);
+    (void)scratch_address(opt.device);
     h->id = g_next_id.fetch_add(1);
     h->nside = int(nside);
     h->batch = int(batch);
@@ -650,6 +684,20 @@ int stlx_ensemble_vjp_device(StlEnsemble* e, const double* cot, size_t n, const 
       g[k] = reinterpret_cast<double*>(grad_dev[k]);
     }
     e->ensemble->vjp(reinterpret_cast<const stl::cplx*>(cot), g.data());
+    return 0;
+  });
+}
+
+int stlx_apply_operator_noalloc(StlHandle* h, int kind, int level, int rows, uint64_t in_dev, uint64_t out_dev) {
+  return guard(-1, [&] {
+    if (!h || !in_dev || !out_dev) fail("null argument");
+    const size_t need = h->engine->operator_workspace_bytes(kind, level, rows);
+    if (need > kScratchBytes)
+      fail("stlx_apply_operator_noalloc: the operator's staging needs " + std::to_string(need) +
+           " bytes, more than the " + std::to_string(kScratchBytes) + "-byte static scratch");
+    void* ws = scratch_address(h->engine->options().device);
+    h->engine->apply_operator_in(kind, level, rows, reinterpret_cast<const double*>(in_dev),
+                                 reinterpret_cast<double*>(out_dev), ws, kScratchBytes);
     return 0;
   });
 }

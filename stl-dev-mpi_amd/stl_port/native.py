@@ -66,7 +66,7 @@ ABI = {
     "stlp_train_bias": "ipwUdz", "stlp_train_loss": "ipwbd", "stlp_train_step": "ipwgd",
     "stlp_train_live": "ipwUdzUdzbgd", "stlp_train_lbfgs": "ppWzs", "stlp_train_set_joint": "ipddwwsg",
     "stlp_train_front": "ipdzbd",
-    "stlx_share": "pps", "stlx_apply_operator": "ipiiiuu", "stlx_synthesize": "lpuzddsdz",
+    "stlx_share": "pps", "stlx_apply_operator": "ipiiiuu", "stlx_apply_operator_noalloc": "ipiiiuu", "stlx_synthesize": "lpuzddsdz",
     "stlx_ensemble_new": "pIIIzs", "stlx_ensemble_free": "-p", "stlx_ensemble_ncoef": "ipZ",
     "stlx_ensemble_forward_device": "ipUzbdd", "stlx_ensemble_vjp_device": "ipdzUz",
     "stlx_map_len": "isZ", "stlx_read_map": "isdz", "stlx_read_map_range": "iszzd", "stlx_read_npy": "isdzZZ",
