@@ -43,7 +43,10 @@ void stlp_string_free(char* s);
  * orientations (4), block_pixels (accepted, unused:
  * the whole sky is resident), backend ("gpu"; "cpu" is refused), device (0),
  * and the keys of this library: precision ("fp64" | "mixed": FP32 bulk
- * fields and operators with FP64 sky-wide sums), timing (false).  NULL on
+ * fields and operators with FP64 sky-wide sums), timing (false), fields
+ * ("auto" | "keep" | "recompute": the field store), batch_nside (16: the
+ * levels n >= 2 whose pixels over the batch number at most 12 batch_nside^2
+ * run batched, one launch per sweep step for all of them; 0: none).  NULL on
  * failure. */
 StlHandle* stlp_new(const char* config);
 /* Frees a handle (lib.rs:353-359); NULL is ignored. */
@@ -183,7 +186,8 @@ int64_t stlx_synthesize(StlHandle* h, uint64_t x_dev, size_t n, const double* ta
  * ensemble driver loops over NativeEngine handles): groups g of batch[g] maps
  * at nside[g] with j[g] scales, evaluated together, each forward and each VJP
  * one HIP graph per group on concurrent streams (operators shared between the
- * groups' levels of equal nside).  options: precision ("mixed" | "fp64"), device. */
+ * groups' levels of equal nside).  options: precision ("mixed" | "fp64"), device,
+ * batch_nside (as stlp_new's, for every group). */
 typedef struct StlEnsemble StlEnsemble;
 StlEnsemble* stlx_ensemble_new(const int32_t* nside, const int32_t* j, const int32_t* batch, size_t groups,
                                const char* options);

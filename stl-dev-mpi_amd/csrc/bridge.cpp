@@ -265,7 +265,7 @@ StlHandle* stlp_new(const char* config) {
     else if (fields == "auto") opt.fields = stl::EngineOptions::kAuto;
     else fail("fields: auto, keep or recompute");
     opt.conv_variant = int(v.integer("conv_variant", -1));
-    opt.batch_nside = int(v.integer("batch_nside", 32));
+    opt.batch_nside = int(v.integer("batch_nside", 16));
     if (opt.batch_nside < 0) fail("batch_nside must be nonnegative");
     if (opt.conv_variant < -1 || opt.conv_variant > 5) fail("conv_variant must be -1 or 0 to 5");
     auto h = std::make_unique<StlHandle>();
@@ -630,6 +630,8 @@ StlEnsemble* stlx_ensemble_new(const int32_t* nside, const int32_t* j, const int
     Json v = Json::parse(options ? options : "{}");
     std::string precision = v.string("precision", "mixed");
     if (precision != "fp64" && precision != "mixed") fail("precision must be fp64 or mixed");
+    int64_t batch_nside = v.integer("batch_nside", 16);
+    if (batch_nside < 0 || batch_nside > (int64_t(1) << 20)) fail("batch_nside must be nonnegative");
     std::vector<stl::EnsembleGroup> gs(groups);
     for (size_t g = 0; g < groups; ++g) {
       if (nside[g] < 1 || (nside[g] & (nside[g] - 1))) fail("nside must be a power of two");
@@ -640,7 +642,8 @@ StlEnsemble* stlx_ensemble_new(const int32_t* nside, const int32_t* j, const int
       gs[g] = stl::EnsembleGroup{nside[g], j[g], batch[g]};
     }
     auto e = std::make_unique<StlEnsemble>();
-    e->ensemble = std::make_unique<stl::Ensemble>(gs, precision == "fp64", int(v.integer("device", 0)));
+    e->ensemble = std::make_unique<stl::Ensemble>(gs, precision == "fp64", int(v.integer("device", 0)),
+                                                  int(batch_nside));
     return e.release();
   });
 }

@@ -23,6 +23,7 @@
 #include <cstddef>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -349,16 +350,20 @@ class EngineT final : public Engine {
       if (n + 1 < levels_) dec_.push_back(shared_dec<T>(opt.device, order, s_));
     }
     geometry_bytes_ = device_bytes_live();
+    // Batched from the first level (n >= 2) whose pixels over all the field
+    // rows number at most 12 batch_nside^2: below that a level's kernels are
+    // launch-bound, above it the tuned per-level kernels do better.
     qc_ = levels_;
     if (opt.batch_nside > 0)
       for (int n = 2; n < levels_; ++n)
-        if ((opt.nside >> n) <= opt.batch_nside) {
+        if (int64_t(B_) * N_[n] <= 12 * int64_t(opt.batch_nside) * opt.batch_nside) {
           qc_ = n;
           break;
         }
-    int cus = 0;
-    STL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, opt.device));
-    q_workgroups_ = std::max(1, 4 * cus);
+    // One workgroup per item: the dispatcher balances the items over the CUs,
+    // also beside other streams' kernels (a fixed stride over a persistent grid
+    // waits for its last workgroup to find a free CU).
+    q_workgroups_ = std::numeric_limits<int>::max();
     allocate();
     tune();
     // The timing launches of tune() wrote the (possibly shared) fields.

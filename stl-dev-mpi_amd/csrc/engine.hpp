@@ -54,11 +54,12 @@ struct EngineOptions {
   // memory after the operators are built.
   enum Fields { kKeep = 0, kRecompute = 1, kAuto = 2 };
   int fields = kAuto;
-  // Levels of at most this nside (from level 2 on) run batched: each step of
-  // a sweep is one launch for all of them (the decimation chains one per
-  // level) instead of a launch per kernel and level; 0 switches the batching
-  // off.  Used with the keep store, one-channel plans and no masks.
-  int batch_nside = 32;
+  // Levels (from level 2 on) whose pixels over all the maps of the batch
+  // number at most 12 batch_nside^2 run batched: each step of a sweep is one
+  // launch for all of them (the decimation chains one per level) instead of a
+  // launch per kernel and level; 0 switches the batching off.  Used with the
+  // keep store, one-channel plans and no masks.
+  int batch_nside = 16;
 };
 
 // One timed candidate of the convolution-variant choice: level, kernel

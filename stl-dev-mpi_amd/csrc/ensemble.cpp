@@ -11,7 +11,7 @@ namespace {
 constexpr size_t kMaxGraphs = 4;
 }
 
-Ensemble::Ensemble(const std::vector<EnsembleGroup>& groups, bool fp64, int device) {
+Ensemble::Ensemble(const std::vector<EnsembleGroup>& groups, bool fp64, int device, int batch_nside) {
   auto t0 = std::chrono::steady_clock::now();
   STL_CHECK(!groups.empty(), "an ensemble needs at least one group");
   STL_HIP(hipSetDevice(device));
@@ -29,6 +29,7 @@ Ensemble::Ensemble(const std::vector<EnsembleGroup>& groups, bool fp64, int devi
     o.fp64 = fp64;
     o.device = device;
     o.timing = false;
+    o.batch_nside = batch_nside;
     engines_[g] = Engine::create(o);
   }
   // Default-priority streams, one per group: the groups' graphs run

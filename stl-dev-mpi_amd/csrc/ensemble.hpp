@@ -18,7 +18,8 @@ struct EnsembleGroup {
 
 class Ensemble {
  public:
-  Ensemble(const std::vector<EnsembleGroup>& groups, bool fp64, int device);
+  // batch_nside: as EngineOptions::batch_nside, for every group.
+  Ensemble(const std::vector<EnsembleGroup>& groups, bool fp64, int device, int batch_nside = 16);
   ~Ensemble();
   Ensemble(const Ensemble&) = delete;
   Ensemble& operator=(const Ensemble&) = delete;

@@ -583,7 +583,7 @@ class NativeEnsemble:
     (and the variances with moments=True); vjp_device takes cotangents of those shapes and
     writes each group's gradient into `out`."""
 
-    def __init__(self, groups, precision="mixed", device=0, library=None):
+    def __init__(self, groups, precision="mixed", device=0, library=None, batch_nside=None):
         self._lock = threading.RLock()
         self._h = None
         self._lib = bridge(library)
@@ -596,7 +596,10 @@ class NativeEnsemble:
         self.groups = spec
         cols = [np.ascontiguousarray([s[k] for s in spec], dtype=np.int32) for k in range(3)]
         ints = [c.ctypes.data_as(_CTYPES["I"]) for c in cols]
-        options = json.dumps(dict(precision=precision, device=int(device))).encode()
+        options = dict(precision=precision, device=int(device))
+        if batch_nside is not None:
+            options["batch_nside"] = int(batch_nside)
+        options = json.dumps(options).encode()
         self._h = _Owned(self._lib, self._lib.call("stlx_ensemble_new", ints[0], ints[2], ints[1], len(spec),
                                                    options), "stlx_ensemble_free")
         counts = np.zeros(len(spec), dtype=np.uintp)

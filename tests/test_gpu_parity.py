@@ -423,18 +423,18 @@ def test_work_queue_levels_agree_with_level_launches(precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batched_levels_agree_with_level_launches(precision):
-    """The coarse levels batched (levels of nside <= 32 from level 2 on, the default: each step
-    of a sweep one launch for all of them) against a launch per kernel and level
-    (batch_nside=0), on a batch of 2 maps at nside 128 (J 6: levels 2..5 batched), graph
-    replays and eager launches; and the batched FP64 engine against the oracle at the
-    reference's device bars."""
+    """The coarse levels batched (batch_nside=64: from level 2 on, the levels of at most
+    12 * 64^2 pixels over the batch; each step of a sweep one launch for all of them) against a
+    launch per kernel and level (batch_nside=0), on a batch of 2 maps at nside 128 (J 6: levels
+    2..5 batched), graph replays and eager launches; and the batched FP64 engine against the
+    oracle at the reference's device bars."""
     from stl_port import NativeEngine
     nside, J = 128, 6
     rng = np.random.default_rng(51)
     x = rng.standard_normal((2, 12 * nside * nside))
     cot = None
     out = {}
-    for bn, timing in ((32, False), (32, True), (0, False)):
+    for bn, timing in ((64, False), (64, True), (0, False)):
         print(f"[gpu] batched levels nside<={bn} timing={timing} {precision}", flush=True)
         with NativeEngine(nside, J, batch=2, precision=precision, batch_nside=bn, timing=timing) as e:
             assert e.info()["batch_level"] == (2 if bn else J)
@@ -443,9 +443,9 @@ def test_batched_levels_agree_with_level_launches(precision):
                 cot = rng.standard_normal(vals.shape) + 1j * rng.standard_normal(vals.shape)
             out[(bn, timing)] = (vals, var, e.vjp(cot))
             plan = e.plan
-    for a, b in zip(out[(32, False)], out[(32, True)]):
+    for a, b in zip(out[(64, False)], out[(64, True)]):
         np.testing.assert_array_equal(a, b)
-    got, ref0 = out[(32, False)], out[(0, False)]
+    got, ref0 = out[(64, False)], out[(0, False)]
     if precision == "fp64":
         assert max(family_errors(got[0], ref0[0], plan).values()) < STAT_BAR
         assert rel(got[1], ref0[1]) < VAR_BAR
