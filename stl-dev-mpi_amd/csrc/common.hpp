@@ -137,10 +137,22 @@ struct ConvOp {
   const int32_t* trow = nullptr;  // [Tn][R] NESTED row id (any face), by window offset (dy, dx)
   const int16_t* toff = nullptr;  // [Tn][R] offset of that row in the cell's extended face frame
   const T* tcoef = nullptr;       // [kL][Tn][R][2] that row's coefficient (its own gauges), gauge planes
-  // toff in linear form dy * lds_pitch(TS) + dx for the LDS windows of the
-  // 32 x 32 and 16 x 16 tiles; null where the level has no such tiles.
-  const int16_t* tlin32 = nullptr;  // [Tn][R]
-  const int16_t* tlin16 = nullptr;  // [Tn][R]
+  // Tiled levels: the transposed slots are union slots of blocks of 64 cells
+  // (an aligned 8 x 8 NESTED block, one wave of the tiled adjoints): slot s
+  // of every cell of block w reads the same window offset (dy, dx), the cells
+  // without a row there with a zero coefficient; tcnt holds the block's count
+  // tbcnt[w].  The offsets in the linear form dy * lds_pitch(TS) + dx of the
+  // 32 x 32, 16 x 16 and 8 x 8 tiles' windows.  Null on untiled levels.
+  const int32_t* tbcnt = nullptr;  // [R / 64]
+  // The same for the forward rows (fbcnt; cnt holds the block's count), in
+  // the linear form dy * SW + dx of the windows (SW = TS + 2 kWin).
+  const int32_t* fbcnt = nullptr;  // [R / 64]
+  const int32_t* fbo32 = nullptr;  // [R / 64][S]
+  const int32_t* fbo16 = nullptr;  // [R / 64][S]
+  const int32_t* fbo8 = nullptr;   // [R / 64][S]
+  const int32_t* tbo32 = nullptr;  // [R / 64][Tn]
+  const int32_t* tbo16 = nullptr;  // [R / 64][Tn]
+  const int32_t* tbo8 = nullptr;   // [R / 64][Tn]
   // Adjoint correction of the mirrored columns: for each south cell read by an
   // equator-ring row, the true coefficients minus the mirrored ones, summed
   // after the gather (CSR by cell, rows ascending).

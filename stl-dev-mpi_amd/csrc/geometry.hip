@@ -656,20 +656,155 @@ void build_mirror_fix(int order, int64_t R3, ConvStore<T>& st, hipStream_t s) {
   STL_HIP(hipStreamSynchronize(s));
 }
 
-__global__ void k_lin_off(int64_t n, const int16_t* __restrict__ off, int pitch, int16_t* __restrict__ lin) {
-  int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (i < n) lin[i] = int16_t(off_dy(off[i]) * pitch + off_dx(off[i]));
+double host_sum_u8(const DevArray<uint8_t>& cnt) {
+  std::vector<uint8_t> h(cnt.size());
+  STL_HIP(hipMemcpy(h.data(), cnt.data(), cnt.bytes(), hipMemcpyDeviceToHost));
+  double s = 0.0;
+  for (auto v : h) s += v;
+  return s;
 }
 
-void linear_offsets(const DevArray<int16_t>& off, int pitch, DevArray<int16_t>& lin, hipStream_t s) {
-  lin.resize(off.size());
-  k_lin_off<<<blocks_for(int64_t(off.size()), 256), 256, 0, s>>>(int64_t(off.size()), off.data(), pitch, lin.data());
+// Union slots of a tiled level's stored rows, per block of 64 consecutive rep
+// rows (an aligned 8 x 8 NESTED block: the rows of one wave in the tiled
+// kernels, also in the mirrored tiles' lane order).  Slot s of every row of
+// block w is made to hold the block's s-th window offset: the offsets of the
+// block's rows in (dy, dx) order, each as many times as one row has it, so
+// that a wave's lanes read LDS words at one uniform offset from their own
+// positions, with the offset and the slot count scalar loads.  A row without
+// an entry at a slot's offset gets a zero coefficient there (and id 0).  The
+// entries of one offset keep their order.  Pass 1 (uids == nullptr) writes
+// the block's slot count bcnt[w]; pass 2 fills the new arrays of Su slots.
+// kTrans: the transposed layout (coefficients in gauge planes [kL][Sn][R][2],
+// entries sorted by offset); else the forward one ([Sn][R][kL][2]).
+constexpr int kWinSide = 2 * kWin + 1, kWinPos = kWinSide * kWinSide;
+
+__device__ inline int win_pos(int16_t o) { return (off_dy(o) + kWin) * kWinSide + off_dx(o) + kWin; }
+
+template <typename T, bool kTrans>
+__host__ __device__ inline size_t coef_index(int g, int s, int64_t q, int Sn, int64_t R) {
+  return kTrans ? ((size_t(g) * Sn + s) * R + q) * 2 : ((size_t(s) * R + q) * kL + g) * 2;
+}
+
+template <typename T, bool kTrans>
+__global__ __launch_bounds__(64) void k_union_slots(int64_t R, int Sn, const uint8_t* __restrict__ cnt,
+                                                    const int32_t* __restrict__ ids,
+                                                    const int16_t* __restrict__ off, const T* __restrict__ coef,
+                                                    int Su, int32_t* __restrict__ bcnt, int32_t* __restrict__ boff,
+                                                    uint8_t* __restrict__ ucnt, int32_t* __restrict__ uids,
+                                                    int16_t* __restrict__ uoff, T* __restrict__ ucoef) {
+  __shared__ int mult[kWinPos], first[kWinPos + 1];
+  __shared__ int16_t uo[256];
+  const int64_t w = blockIdx.x;
+  const int64_t q = w * 64 + threadIdx.x;
+  for (int i = threadIdx.x; i < kWinPos; i += 64) mult[i] = 0;
+  __syncthreads();
+  const int n = cnt[q];
+  // Occurrence k of entry s: the row's earlier entries at the same offset.
+  auto occurrence = [&](int s, int pos) {
+    int k = 0;
+    for (int t = 0; t < s; ++t) k += win_pos(off[size_t(t) * R + q]) == pos;
+    return k;
+  };
+  for (int s = 0; s < n; ++s) {
+    const int pos = win_pos(off[size_t(s) * R + q]);
+    atomicMax(&mult[pos], occurrence(s, pos) + 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int u = 0;
+    for (int i = 0; i < kWinPos; ++i) {
+      first[i] = u;
+      u += mult[i];
+    }
+    first[kWinPos] = u;
+    bcnt[w] = u;
+  }
+  __syncthreads();
+  if (uids == nullptr) return;
+  const int u = first[kWinPos];
+  // Offsets of the block's slots; zero slots past its count.
+  for (int s = threadIdx.x; s < Su; s += 64) {
+    int16_t o = 0;
+    if (s < u) {
+      int i = 0;
+      while (first[i + 1] <= s) ++i;
+      o = pack_off(i % kWinSide - kWin, i / kWinSide - kWin);
+    }
+    uo[s] = o;
+    boff[w * Su + s] = o;
+  }
+  __syncthreads();
+  for (int s = 0; s < Su; ++s) {
+    const size_t at = size_t(s) * R + q;
+    uids[at] = 0;
+    uoff[at] = uo[s];
+    for (int g = 0; g < kL; ++g) {
+      const size_t c = coef_index<T, kTrans>(g, s, q, Su, R);
+      ucoef[c] = T(0);
+      ucoef[c + 1] = T(0);
+    }
+  }
+  for (int s = 0; s < n; ++s) {
+    const size_t at = size_t(s) * R + q;
+    const int pos = win_pos(off[at]);
+    const int slot = first[pos] + occurrence(s, pos);
+    uids[size_t(slot) * R + q] = ids[at];
+    for (int g = 0; g < kL; ++g) {
+      const size_t c = coef_index<T, kTrans>(g, s, q, Sn, R), d = coef_index<T, kTrans>(g, slot, q, Su, R);
+      ucoef[d] = coef[c];
+      ucoef[d + 1] = coef[c + 1];
+    }
+  }
+  ucnt[q] = uint8_t(u);
+}
+
+__global__ void k_lin_off32(int64_t n, const int32_t* __restrict__ off, int pitch, int32_t* __restrict__ lin) {
+  int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < n) lin[i] = off_dy(int16_t(off[i])) * pitch + off_dx(int16_t(off[i]));
+}
+
+// Re-slots (cnt, ids, off, coef) of Sn slots over R rows into union slots
+// (capacity a multiple of 4: the tiled loops run whole chunks of 4 slots);
+// returns the new capacity.  bcnt: the blocks' counts; lin[i]: the blocks'
+// offsets in linear form for window pitches pitch[i].
+template <typename T, bool kTrans>
+int union_slots(int64_t R, int Sn, DevArray<uint8_t>& cnt, DevArray<int32_t>& ids, DevArray<int16_t>& off,
+                DevArray<T>& coef, DevArray<int32_t>& bcnt, const int (&pitch)[3], DevArray<int32_t>* const (&lin)[3],
+                hipStream_t s) {
+  STL_CHECK(R % 64 == 0, "union slots: blocks of 64 rows");
+  const int64_t nb = R / 64;
+  bcnt.resize(size_t(nb));
+  k_union_slots<T, kTrans><<<unsigned(nb), 64, 0, s>>>(R, Sn, cnt.data(), ids.data(), off.data(), coef.data(), 0,
+                                                       bcnt.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
   STL_HIP(hipGetLastError());
+  const int umax = device_max_i32(bcnt.data(), nb, s);
+  STL_CHECK(umax >= Sn && umax <= 255, "union slots: bad slot count");
+  const int Su = (umax + 3) / 4 * 4;
+  DevArray<uint8_t> ucnt{static_cast<size_t>(R)};
+  DevArray<int32_t> uids(size_t(Su) * R);
+  DevArray<int16_t> uoff(size_t(Su) * R);
+  DevArray<T> ucoef(size_t(kL) * Su * R * 2);
+  DevArray<int32_t> boff(size_t(nb) * Su);
+  k_union_slots<T, kTrans><<<unsigned(nb), 64, 0, s>>>(R, Sn, cnt.data(), ids.data(), off.data(), coef.data(), Su,
+                                                       bcnt.data(), boff.data(), ucnt.data(), uids.data(),
+                                                       uoff.data(), ucoef.data());
+  STL_HIP(hipGetLastError());
+  for (int i = 0; i < 3; ++i) {
+    lin[i]->resize(boff.size());
+    k_lin_off32<<<blocks_for(int64_t(boff.size()), 256), 256, 0, s>>>(int64_t(boff.size()), boff.data(), pitch[i],
+                                                                     lin[i]->data());
+    STL_HIP(hipGetLastError());
+  }
   STL_HIP(hipStreamSynchronize(s));
+  cnt = std::move(ucnt);
+  ids = std::move(uids);
+  off = std::move(uoff);
+  coef = std::move(ucoef);
+  return Su;
 }
 
 // Rows and columns of the faces 0 and 4 only (the faces 8-11 are mirrors),
-// and the linear transposed offsets of the tiled adjoints.
+// and the union slots of the tiled adjoints.
 template <typename T>
 void mirror_store(int order, ConvStore<T>& st, hipStream_t s) {
   const int64_t npf = int64_t(1) << (2 * order), R3 = 3 * npf, R2 = 2 * npf;
@@ -682,8 +817,15 @@ void mirror_store(int order, ConvStore<T>& st, hipStream_t s) {
   keep_rows(st.trow, size_t(st.Tn), R3, R2, 1, s);
   keep_rows(st.toff, size_t(st.Tn), R3, R2, 1, s);
   keep_rows(st.tcoef, size_t(kL) * st.Tn, R3, R2, 2, s);
-  if (st.tiled && order >= 5) linear_offsets(st.toff, lds_pitch(32), st.tlin32, s);
-  if (st.tiled && order >= 4) linear_offsets(st.toff, lds_pitch(16), st.tlin16, s);
+  st.slots = host_sum_u8(st.cnt);
+  st.tslots = host_sum_u8(st.tcnt);
+  if (st.tiled) {
+    // Forward windows: pitch SW = TS + 2 kWin; the adjoints': lds_pitch(TS).
+    st.S = union_slots<T, false>(R2, st.S, st.cnt, st.cell, st.off, st.coef, st.fbcnt,
+                                 {32 + 2 * kWin, 16 + 2 * kWin, 8 + 2 * kWin}, {&st.fbo32, &st.fbo16, &st.fbo8}, s);
+    st.Tn = union_slots<T, true>(R2, st.Tn, st.tcnt, st.trow, st.toff, st.tcoef, st.tbcnt,
+                                 {lds_pitch(32), lds_pitch(16), lds_pitch(8)}, {&st.tbo32, &st.tbo16, &st.tbo8}, s);
+  }
 }
 
 }  // namespace
@@ -944,8 +1086,8 @@ std::shared_ptr<const SharedConv<T>> shared_conv(int device, int order, hipStrea
   static const std::vector<double> ker = wavelet_kernel_f32(5);
   if constexpr (std::is_same<T, double>::value) build_conv(order, ker, &op->store, nullptr, stream);
   else build_conv(order, ker, nullptr, &op->store, stream);
-  op->slots = sum_u8(op->store.cnt);
-  op->tslots = sum_u8(op->store.tcnt);
+  op->slots = op->store.slots;
+  op->tslots = op->store.tslots;
   c[key] = op;
   return op;
 }

@@ -22,7 +22,10 @@ struct ConvStore {
   DevArray<int32_t> cell, trow;
   DevArray<int16_t> off, toff;
   DevArray<T> coef, tcoef;
-  DevArray<int16_t> tlin32, tlin16;  // ConvOp::tlin32 / tlin16
+  DevArray<int32_t> fbcnt, fbo32, fbo16, fbo8;  // ConvOp::fbcnt / fbo* (tiled levels)
+  DevArray<int32_t> tbcnt, tbo32, tbo16, tbo8;  // ConvOp::tbcnt / tbo* (tiled levels)
+  // Entries stored (forward and transposed), without the union slots' zeros.
+  double slots = 0.0, tslots = 0.0;
   // Correction of the mirrored columns (ConvOp::fix_*).
   int nfix = 0;
   DevArray<int32_t> fix_cell, fix_ptr, fix_row;
@@ -37,7 +40,8 @@ struct ConvStore {
     v.tiled = tiled;
     v.cnt = cnt.data(); v.cell = cell.data(); v.off = off.data(); v.coef = coef.data();
     v.tcnt = tcnt.data(); v.trow = trow.data(); v.toff = toff.data(); v.tcoef = tcoef.data();
-    v.tlin32 = tlin32.data(); v.tlin16 = tlin16.data();
+    v.fbcnt = fbcnt.data(); v.fbo32 = fbo32.data(); v.fbo16 = fbo16.data(); v.fbo8 = fbo8.data();
+    v.tbcnt = tbcnt.data(); v.tbo32 = tbo32.data(); v.tbo16 = tbo16.data(); v.tbo8 = tbo8.data();
     v.nfix = nfix;
     v.fix_cell = fix_cell.data(); v.fix_ptr = fix_ptr.data(); v.fix_row = fix_row.data();
     v.fix_coef = fix_coef.data();
@@ -45,7 +49,8 @@ struct ConvStore {
   }
   size_t bytes() const {
     return cnt.bytes() + tcnt.bytes() + cell.bytes() + trow.bytes() + off.bytes() + toff.bytes() + coef.bytes() +
-           tcoef.bytes() + tlin32.bytes() + tlin16.bytes() + fix_cell.bytes() + fix_ptr.bytes() + fix_row.bytes() +
+           tcoef.bytes() + fbcnt.bytes() + fbo32.bytes() + fbo16.bytes() + fbo8.bytes() + tbcnt.bytes() + tbo32.bytes() +
+           tbo16.bytes() + tbo8.bytes() + fix_cell.bytes() + fix_ptr.bytes() + fix_row.bytes() +
            fix_coef.bytes();
   }
 };

@@ -477,6 +477,32 @@ __device__ inline TileInfo tile_info(int order, int rows, int& row) {
   return t;
 }
 
+// Union slots of the wave's block of 64 rep rows or cells (tile rows fr,
+// tile base, lane's row or cell rl): the block's window offsets in the linear
+// form of the TS tiles' windows, the same in every lane, read with scalar
+// loads, and the block's slot count.
+struct BlockSlots {
+  const int32_t* off;
+  int cnt;
+};
+__device__ inline int64_t wave_block(int64_t npf, int fr, int64_t base, int rl) {
+  return (int64_t(fr) * npf + base) / 64 + (__builtin_amdgcn_readfirstlane(rl) >> 6);
+}
+// Transposed slots (the adjoints).
+template <int TS, typename T>
+__device__ inline BlockSlots block_slots(const ConvOp<T>& op, int fr, int64_t base, int rl) {
+  const int64_t blk = wave_block(op.npf, fr, base, rl);
+  const int32_t* lin = TS == 32 ? op.tbo32 : (TS == 16 ? op.tbo16 : op.tbo8);
+  return BlockSlots{lin + blk * op.Tn, op.tbcnt[blk]};
+}
+// Forward slots.
+template <int TS, typename T>
+__device__ inline BlockSlots fwd_block_slots(const ConvOp<T>& op, int fr, int64_t base, int rl) {
+  const int64_t blk = wave_block(op.npf, fr, base, rl);
+  const int32_t* lin = TS == 32 ? op.fbo32 : (TS == 16 ? op.fbo16 : op.fbo8);
+  return BlockSlots{lin + blk * op.S, op.fbcnt[blk]};
+}
+
 // A 32 x 32 tile is a 16-wave workgroup: two of them per CU need 8 waves per
 // SIMD, i.e. at most 64 VGPRs; at 7 waves only one fits.
 template <typename T, int TS>
@@ -512,12 +538,11 @@ void k_conv_fwd_tiled(ConvOp<T> op, const T* __restrict__ x, int rows,
   for (int j = 0; j < NI; ++j)
 #pragma unroll
     for (int g = 0; g < kL; ++g) acc[j][g][0] = acc[j][g][1] = T(0);
-  const int cnt = op.cnt[r];
+  const BlockSlots bs = fwd_block_slots<TS>(op, ti.fr, ti.base, rl);
 #pragma unroll 4
-  for (int s = 0; s < cnt; ++s) {
+  for (int s = 0; s < bs.cnt; ++s) {
     size_t at = size_t(s) * op.R + r;
-    int16_t o = op.off[at];
-    int src = pos + off_dy(o) * SW + off_dx(o);
+    int src = pos + bs.off[s];
     T c[8], a[8];
     Load<T>::coef8(op.coef + at * 8, c);
     if constexpr (kRepSums) {
@@ -620,15 +645,14 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
     for (int j = 0; j < NI; ++j)
 #pragma unroll
       for (int g = 0; g < kL; ++g) acc[f][j][g][0] = acc[f][j][g][1] = T(0);
-  const int cnt = op.cnt[r];
+  const BlockSlots bs = fwd_block_slots<TS>(op, fr, base, rl);
   // The mirror's conjugation, on the coefficients: 4 products per slot for
   // all the faces (at the outputs it would hold more registers across the loop).
   const T sg = m ? T(-1) : T(1);
 #pragma unroll kSlotUnroll
-  for (int s = 0; s < cnt; ++s) {
+  for (int s = 0; s < bs.cnt; ++s) {
     size_t at = size_t(s) * op.R + r;
-    int16_t o = op.off[at];
-    int src = pos + off_dy(o) * SW + off_dx(o);
+    int src = pos + bs.off[s];
     T c[8];
     Load<T>::coef8(op.coef + at * 8, c);
 #pragma unroll
@@ -671,20 +695,78 @@ __device__ inline E plane_load(const E* plane, uint32_t lane_bytes) {
   return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(plane) + lane_bytes);
 }
 
-// Window offset (LDS pitch lds_pitch(TS)) of transposed slot s of the cell
-// whose lane offset in an int16 plane is qo2: the stored linear form for
-// 32 x 32 and 16 x 16 tiles, else the packed (dx, dy) decoded.
-template <int TS, typename T>
-__device__ inline int tslot_off(const ConvOp<T>& op, int s, uint32_t qo2) {
-  const size_t plane = size_t(s) * op.R;
-  if constexpr (TS == 32) {
-    return plane_load(op.tlin32 + plane, qo2);
-  } else if constexpr (TS == 16) {
-    return plane_load(op.tlin16 + plane, qo2);
-  } else {
-    const int16_t o = plane_load(op.toff + plane, qo2);
-    return off_dy(o) * lds_pitch(TS) + off_dx(o);
+// The NP planes (SA elements apart) of an adjoint's LDS window at element
+// offset src: planes 0 .. NP/2 - 1 from one base and the rest from a second,
+// NP/2 planes up, when the whole stack is beyond the 64 KB reach of an LDS
+// read's immediate offset (one address add per base per slot instead of one
+// per far plane).  The second base is made opaque so that the compiler does
+// not fold it back into the first.
+template <int NP, int SA, typename E>
+struct WindowPlanes {
+  static constexpr bool kSplit = size_t(NP) * SA * sizeof(E) > 65536;
+  int hi_pos;
+  __device__ explicit WindowPlanes(int pos) : hi_pos(pos + (kSplit ? NP / 2 * SA : 0)) {
+    // Not volatile: a statement with side effects would count as a memory
+    // write and turn the scalar loads of the slot offsets into vector loads.
+    if constexpr (kSplit) asm("" : "+v"(hi_pos));
   }
+  template <typename F>
+  __device__ void read(const E* w, int pos, int off, F&& out) const {
+    const E* lo = w + pos + off;
+    if constexpr (kSplit) {
+      const E* hi = w + hi_pos + off;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) out(p, p < NP / 2 ? lo[p * SA] : hi[(p - NP / 2) * SA]);
+    } else {
+#pragma unroll
+      for (int p = 0; p < NP; ++p) out(p, lo[p * SA]);
+    }
+  }
+};
+
+// The slot loop of the tiled adjoints, software-pipelined in chunks of U
+// slots: the next chunk's NC coefficients per slot (load(s0, u, k) for slot
+// s0 + u, vector loads) are issued before the current chunk's products run (body(offset,
+// coefficients)), so a wave keeps a chunk of loads in flight behind its
+// arithmetic.  The window offsets (scalar loads) are read at the start of
+// their own chunk: a scalar load in flight would make every LDS wait of the
+// chunk a full wait.  The union slot capacity cap is a multiple of 4 with
+// zero coefficients past a block's count, so whole chunks run.  The
+// prefetches are unconditional (slots clamped to the capacity, the last
+// one unused) and each sits in the same block as the products that use it:
+// one issued on some paths only would make the compiler wait for every load
+// before the current chunk's products, and one followed by a branch would be
+// moved past it, next to its use.
+template <int U, int NC, typename C, typename Load, typename Body>
+__device__ __forceinline__ void pipelined_slots(const int32_t* __restrict__ off, int cnt, int cap, Load load,
+                                                Body body) {
+  const int nch = (cnt + U - 1) / U;
+  C a0[U][NC], a1[U][NC];
+  auto fetch = [&](C (&a)[U][NC], int s0) {
+    // A chunk lies wholly below the capacity or wholly past the count.
+    const int s0c = min(s0, cap - U);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int k = 0; k < NC; ++k) a[u][k] = load(s0c, u, k);
+  };
+  auto run = [&](const C (&a)[U][NC], int s0) {
+    int32_t o[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) o[u] = off[s0 + u];
+#pragma unroll
+    for (int u = 0; u < U; ++u) body(o[u], a[u]);
+  };
+  if (nch == 0) return;
+  fetch(a0, 0);
+  int c = 0;
+  for (; c + 1 < nch; c += 2) {
+    fetch(a1, (c + 1) * U);
+    run(a0, c * U);
+    fetch(a0, (c + 2) * U);
+    run(a1, (c + 1) * U);
+  }
+  if (c < nch) run(a0, c * U);
 }
 
 // LDS index of window position wi (SW x SW, row-major) in planes of pitch PW.
@@ -715,9 +797,10 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
   const int li = threadIdx.x, rl = rep_lane<TS>(ti.m, li);
   const int pos = (int(compress_bits64(rl >> 1)) + kWin) * PW + int(compress_bits64(rl)) + kWin;
   const int64_t q = int64_t(ti.fr) * op.npf + ti.base + rl;
-  const int cnt = op.tcnt[q];
+  const BlockSlots bs = block_slots<TS>(op, ti.fr, ti.base, rl);
+  const WindowPlanes<RW * GP * NI, SA, cplx_t<T>> wp(pos);
   const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
-  const uint32_t qo2 = uint32_t(q) * 2u, qoc = uint32_t(q) * uint32_t(sizeof(cplx_t<T>));
+  const uint32_t qoc = uint32_t(q) * uint32_t(sizeof(cplx_t<T>));
   // Pixels (or -1; -2 past the window) of the window positions this thread
   // stages, the same in every pass; their LDS indexes are recomputed (cheaper
   // than the registers).
@@ -764,22 +847,27 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
       }
     }
     __syncthreads();
-#pragma unroll 4
-    for (int s = 0; s < cnt; ++s) {
-      const int src = pos + tslot_off<TS>(op, s, qo2);
+    // Chunks of 2 slots when every slot loads 4 gauges' coefficients.
+    pipelined_slots<GP >= 4 ? 2 : 4, GP, cplx_t<T>>(
+        bs.off, bs.cnt, op.Tn,
+        [&](int s0, int u, int gg) {
+          return plane_load(tc + (size_t(gauge_src(ti.k, g0 + gg)) * op.Tn + size_t(s0)) * op.R + size_t(u) * op.R,
+                            qoc);
+        },
+        [&](int32_t o, const cplx_t<T>(&a)[GP]) {
+          // Planes [r][gg][j].
+          cplx_t<T> v[RW * GP * NI];
+          wp.read(gs, pos, o, [&](int p, cplx_t<T> x) { v[p] = x; });
 #pragma unroll
-      for (int gg = 0; gg < GP; ++gg) {
-        const int g = g0 + gg;
-        const cplx_t<T> a = plane_load(tc + (size_t(gauge_src(ti.k, g)) * op.Tn + s) * op.R, qoc);
+          for (int gg = 0; gg < GP; ++gg)
 #pragma unroll
-        for (int r = 0; r < RW; ++r)
+            for (int r = 0; r < RW; ++r)
 #pragma unroll
-          for (int j = 0; j < NI; ++j) {
-            cplx_t<T> v = gs[((r * GP + gg) * NI + j) * SA + src];
-            acc[r][j] = fma(a.y, v.y, fma(a.x, v.x, acc[r][j]));
-          }
-      }
-    }
+              for (int j = 0; j < NI; ++j) {
+                const cplx_t<T> x = v[(r * GP + gg) * NI + j];
+                acc[r][j] = fma(a[gg].y, x.y, fma(a[gg].x, x.x, acc[r][j]));
+              }
+        });
   }
   const int64_t p = tile_out<TS>(ti.face, ti.m, ti.base, op.order) + li;
 #pragma unroll
@@ -795,8 +883,19 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
 // Slot loop unrolled 4 times on 32 x 32 tiles (A/B on MI355X: deeper
 // unrolling, or pinning the 4-face z adjoint to 64 VGPRs for two workgroups
 // per CU, was slower), twice on smaller tiles.
+// Waves per SIMD that the LDS window of an adjoint leaves room for (160 KB
+// per CU): the register budget the scheduler may use.
+template <typename T, int NI, int TS, int FW, int GP>
+constexpr int adj_lds_waves() {
+  constexpr int SW = TS + 2 * kWin;
+  constexpr int bytes = FW * GP * NI * SW * lds_pitch(TS) * int(sizeof(cplx_t<T>));
+  constexpr int wgs = (160 * 1024) / bytes;
+  constexpr int waves = wgs * TS * TS / 256;
+  return waves < 1 ? 1 : (waves > 8 ? 8 : waves);
+}
 template <typename T, int NI, int TS, int FW, int GP, int kSlotUnroll = TS == 32 ? 4 : 2>
-__global__ __launch_bounds__(TS * TS)
+__global__ __launch_bounds__(TS * TS) __attribute__((amdgpu_waves_per_eu(adj_lds_waves<T, NI, TS, FW, GP>(),
+                                                                         adj_lds_waves<T, NI, TS, FW, GP>())))
 void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
                                                            int rows, T* __restrict__ gx) {
   // Window of SW x SW positions, planes of SW rows of pitch PW.
@@ -818,9 +917,10 @@ void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
   const int li = threadIdx.x, rl = rep_lane<TS>(m, li);
   const int pos = (int(compress_bits64(rl >> 1)) + kWin) * PW + int(compress_bits64(rl)) + kWin;
   const int64_t q = int64_t(fr) * op.npf + base + rl;
-  const int cnt = op.tcnt[q];
+  const BlockSlots bs = block_slots<TS>(op, fr, base, rl);
+  const WindowPlanes<FW * GP * NI, SA, cplx_t<T>> wp(pos);
   const cplx_t<T>* tc = reinterpret_cast<const cplx_t<T>*>(op.tcoef);
-  const uint32_t qo2 = uint32_t(q) * 2u, qoc = uint32_t(q) * uint32_t(sizeof(cplx_t<T>));
+  const uint32_t qoc = uint32_t(q) * uint32_t(sizeof(cplx_t<T>));
   // Pixels (or -1; -2 past the window) of the window positions (face f,
   // position) this thread stages, the same in every pass.
   constexpr int kPer = (FW * NW + TS * TS - 1) / (TS * TS);
@@ -860,23 +960,25 @@ void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
       }
     }
     __syncthreads();
-#pragma unroll kSlotUnroll
-    for (int s = 0; s < cnt; ++s) {
-      const int src = pos + tslot_off<TS>(op, s, qo2);
+    pipelined_slots<kSlotUnroll, GP, cplx_t<T>>(
+        bs.off, bs.cnt, op.Tn,
+        [&](int s0, int u, int gg) {
+          return plane_load(tc + (size_t(pass * GP + gg) * op.Tn + size_t(s0)) * op.R + size_t(u) * op.R, qoc);
+        },
+        [&](int32_t o, const cplx_t<T>(&a)[GP]) {
+          // Planes [f][gg][j].
+          cplx_t<T> v[FW * GP * NI];
+          wp.read(gs, pos, o, [&](int p, cplx_t<T> x) { v[p] = x; });
 #pragma unroll
-      for (int gg = 0; gg < GP; ++gg) {
-        const int plane = pass * GP + gg;
-        const cplx_t<T> a = plane_load(tc + (size_t(plane) * op.Tn + s) * op.R, qoc);
+          for (int gg = 0; gg < GP; ++gg)
 #pragma unroll
-        for (int f = 0; f < FW; ++f) {
+            for (int f = 0; f < FW; ++f)
 #pragma unroll
-          for (int j = 0; j < NI; ++j) {
-            cplx_t<T> v = gs[((f * GP + gg) * NI + j) * SA + src];
-            acc[f][j] = fma(a.y, v.y, fma(a.x, v.x, acc[f][j]));
-          }
-        }
-      }
-    }
+              for (int j = 0; j < NI; ++j) {
+                const cplx_t<T> x = v[(f * GP + gg) * NI + j];
+                acc[f][j] = fma(a[gg].y, x.y, fma(a[gg].x, x.x, acc[f][j]));
+              }
+        });
   }
 #pragma unroll
   for (int f = 0; f < FW; ++f) {
