@@ -106,9 +106,15 @@ __host__ __device__ inline int64_t mirror_pix(int64_t pix, int order) {
 constexpr int kWin = 5;
 
 // Row pitch (in elements) of the LDS windows of the tiled adjoints, per tile
-// side: 4 or 12 mod 16, so that the 4 rows of a 4 x 4 block of lanes (16 lanes,
-// one LDS cycle of 64-bit reads) start 24 banks apart and fill all 32 banks.
-constexpr int lds_pitch(int ts) { return ts == 32 ? 44 : (ts == 16 ? 28 : 20); }
+// side and element size.  gfx950's LDS serves 64-bit reads 32 lanes at a
+// time over 64 four-byte banks: the lanes of a wave hold an 8 x 8 block of
+// cells, so lanes 0-31 read 4 window rows of 8 elements, which fill the 64
+// banks once the pitch is 8 or 24 mod 32 (measured: 187 B/clk per CU at
+// pitch 56 against 118 at pitch 44, 8-byte elements at uniform offsets).
+// 16-byte elements (FP64 complex) keep pitches 4 or 12 mod 16.
+constexpr int lds_pitch(int ts, int elem_bytes = 8) {
+  return elem_bytes == 8 ? (ts == 32 ? 56 : (ts == 16 ? 40 : 24)) : (ts == 32 ? 44 : (ts == 16 ? 28 : 20));
+}
 
 // Offset (dx, dy) packed as two int8 in an int16.
 __host__ __device__ inline int16_t pack_off(int dx, int dy) {

@@ -820,11 +820,13 @@ void mirror_store(int order, ConvStore<T>& st, hipStream_t s) {
   st.slots = host_sum_u8(st.cnt);
   st.tslots = host_sum_u8(st.tcnt);
   if (st.tiled) {
-    // Forward windows: pitch SW = TS + 2 kWin; the adjoints': lds_pitch(TS).
+    // Forward windows: pitch SW = TS + 2 kWin; the adjoints': lds_pitch(TS)
+    // of their complex elements.
     st.S = union_slots<T, false>(R2, st.S, st.cnt, st.cell, st.off, st.coef, st.fbcnt,
                                  {32 + 2 * kWin, 16 + 2 * kWin, 8 + 2 * kWin}, {&st.fbo32, &st.fbo16, &st.fbo8}, s);
     st.Tn = union_slots<T, true>(R2, st.Tn, st.tcnt, st.trow, st.toff, st.tcoef, st.tbcnt,
-                                 {lds_pitch(32), lds_pitch(16), lds_pitch(8)}, {&st.tbo32, &st.tbo16, &st.tbo8}, s);
+                                 {lds_pitch(32, 2 * sizeof(T)), lds_pitch(16, 2 * sizeof(T)), lds_pitch(8, 2 * sizeof(T))},
+                                 {&st.tbo32, &st.tbo16, &st.tbo8}, s);
   }
 }
 

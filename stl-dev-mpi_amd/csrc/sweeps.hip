@@ -769,11 +769,23 @@ __device__ __forceinline__ void pipelined_slots(const int32_t* __restrict__ off,
   if (c < nch) run(a0, c * U);
 }
 
-// LDS index of window position wi (SW x SW, row-major) in planes of pitch PW.
-template <int SW, int PW>
-__device__ inline int lds_index(int wi) {
-  return (wi / SW) * PW + wi % SW;
-}
+// Staging order of an adjoint's window (SW x SW positions per face): blocks
+// of 8 x 4 positions, x fastest, one block per 32 consecutive staging slots,
+// so that 32 lanes write 4 rows of 8 LDS words (conflict-free at the pitches
+// of lds_pitch) and read two aligned 4 x 4 NESTED blocks of the face, in a
+// mirrored tile as well.  Slots past the window's edge stage nothing.
+template <int SW>
+struct StageGrid {
+  static constexpr int kBx = (SW + 7) / 8, kBy = (SW + 3) / 4;
+  static constexpr int kSlots = kBx * kBy * 32;  // per face
+  // Window position (wx, wy) of slot k of a face; false past the edge.
+  __device__ static bool at(int k, int& wx, int& wy) {
+    const int b = k >> 5, t = k & 31;
+    wx = (b % kBx) * 8 + (t & 7);
+    wy = (b / kBx) * 4 + (t >> 3);
+    return wx < SW && wy < SW;
+  }
+};
 
 // GP gauges of the cotangent staged per pass.
 // RW rows (input maps) per workgroup share each slot's offset and
@@ -786,7 +798,7 @@ __global__ __launch_bounds__(TS * TS) __attribute__((amdgpu_waves_per_eu(
 void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
                                                            T* __restrict__ gx) {
   // Window of SW x SW positions, planes of SW rows of pitch PW.
-  constexpr int SW = TS + 2 * kWin, PW = lds_pitch(TS), NW = SW * SW, SA = SW * PW;
+  constexpr int SW = TS + 2 * kWin, PW = lds_pitch(TS, sizeof(cplx_t<T>)), SA = SW * PW;
   // Planar [r][gg][j][pixel]: consecutive pixels in consecutive banks.
   __shared__ cplx_t<T> gs[RW * GP * NI * SA];
   int rowg;
@@ -804,14 +816,15 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
   // Pixels (or -1; -2 past the window) of the window positions this thread
   // stages, the same in every pass; their LDS indexes are recomputed (cheaper
   // than the registers).
-  constexpr int kPer = (NW + TS * TS - 1) / (TS * TS);
+  using Grid = StageGrid<SW>;
+  constexpr int kPer = (Grid::kSlots + TS * TS - 1) / (TS * TS);
   int32_t spix[kPer];
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int idx = threadIdx.x + i * TS * TS;
-    const int wi = stage_index<SW>(ti.m, idx < NW ? idx : 0);
-    spix[i] = idx < NW ? int32_t(staged_pixel(ti.face, ti.m, ti.x0 - kWin + wi % SW, ti.y0 - kWin + wi / SW, op.order))
-                       : -2;
+    int wx, wy;
+    const bool in = idx < Grid::kSlots && Grid::at(idx, wx, wy);
+    spix[i] = in ? int32_t(staged_pixel(ti.face, ti.m, ti.x0 - kWin + wx, ti.y0 - kWin + wy, op.order)) : -2;
   }
   T acc[RW][NI];
 #pragma unroll
@@ -829,7 +842,9 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
     for (int i = 0; i < kPer; ++i) {
       if (spix[i] == -2) continue;
       const int64_t p = spix[i];
-      const int pi = lds_index<SW, PW>(stage_index<SW>(ti.m, threadIdx.x + i * TS * TS));
+      int wx, wy;
+      Grid::at(threadIdx.x + i * TS * TS, wx, wy);
+      const int pi = wy * PW + wx;
 #pragma unroll
       for (int gg = 0; gg < GP; ++gg) {
         // The face shift's (and the mirror's) conjugation of a gauge is
@@ -888,7 +903,7 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
 template <typename T, int NI, int TS, int FW, int GP>
 constexpr int adj_lds_waves() {
   constexpr int SW = TS + 2 * kWin;
-  constexpr int bytes = FW * GP * NI * SW * lds_pitch(TS) * int(sizeof(cplx_t<T>));
+  constexpr int bytes = FW * GP * NI * SW * lds_pitch(TS, sizeof(cplx_t<T>)) * int(sizeof(cplx_t<T>));
   constexpr int wgs = (160 * 1024) / bytes;
   constexpr int waves = wgs * TS * TS / 256;
   return waves < 1 ? 1 : (waves > 8 ? 8 : waves);
@@ -899,7 +914,7 @@ __global__ __launch_bounds__(TS * TS) __attribute__((amdgpu_waves_per_eu(adj_lds
 void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
                                                            int rows, T* __restrict__ gx) {
   // Window of SW x SW positions, planes of SW rows of pitch PW.
-  constexpr int SW = TS + 2 * kWin, PW = lds_pitch(TS), NW = SW * SW, SA = SW * PW;
+  constexpr int SW = TS + 2 * kWin, PW = lds_pitch(TS, sizeof(cplx_t<T>)), SA = SW * PW;
   // Planar [face][gg][j][pixel].
   __shared__ cplx_t<T> gs[FW * GP * NI * SA];
   int64_t tr;
@@ -923,15 +938,16 @@ void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
   const uint32_t qoc = uint32_t(q) * uint32_t(sizeof(cplx_t<T>));
   // Pixels (or -1; -2 past the window) of the window positions (face f,
   // position) this thread stages, the same in every pass.
-  constexpr int kPer = (FW * NW + TS * TS - 1) / (TS * TS);
+  using Grid = StageGrid<SW>;
+  constexpr int kPer = (FW * Grid::kSlots + TS * TS - 1) / (TS * TS);
   int32_t spix[kPer];
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int idx = threadIdx.x + i * TS * TS;
-    const bool in = idx < FW * NW;
-    const int f = in ? idx / NW : 0, wi = stage_index<SW>(m, in ? idx % NW : 0);
-    spix[i] = in ? int32_t(staged_pixel(4 * fr + fg * FW + f, m, x0 - kWin + wi % SW, y0 - kWin + wi / SW, op.order))
-                 : -2;
+    const int f = idx / Grid::kSlots;
+    int wx, wy;
+    const bool in = idx < FW * Grid::kSlots && Grid::at(idx % Grid::kSlots, wx, wy);
+    spix[i] = in ? int32_t(staged_pixel(4 * fr + fg * FW + f, m, x0 - kWin + wx, y0 - kWin + wy, op.order)) : -2;
   }
   T acc[FW][NI];
 #pragma unroll
@@ -945,7 +961,10 @@ void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
       if (spix[i] == -2) continue;
       const int64_t p = spix[i];
       const int idx = threadIdx.x + i * TS * TS;
-      const int f = idx / NW, pi = lds_index<SW, PW>(stage_index<SW>(m, idx % NW));
+      const int f = idx / Grid::kSlots;
+      int wx, wy;
+      Grid::at(idx % Grid::kSlots, wx, wy);
+      const int pi = wy * PW + wx;
       const int k = fg * FW + f;
 #pragma unroll
       for (int gg = 0; gg < GP; ++gg) {
@@ -960,7 +979,9 @@ void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
       }
     }
     __syncthreads();
-    pipelined_slots<kSlotUnroll, GP, cplx_t<T>>(
+    // Chunks of kSlotUnroll slots; 2 where the window has fewer than 8 planes
+    // (the 4-face z adjoint: two workgroups per CU leave 64 VGPRs).
+    pipelined_slots<(FW * GP * NI >= 8 ? kSlotUnroll : 2), GP, cplx_t<T>>(
         bs.off, bs.cnt, op.Tn,
         [&](int s0, int u, int gg) {
           return plane_load(tc + (size_t(pass * GP + gg) * op.Tn + size_t(s0)) * op.R + size_t(u) * op.R, qoc);
