@@ -234,6 +234,7 @@ StlHandle* stlp_new(const char* config) {
     Json v = Json::parse(config);
     long nside = v.integer("nside", 128);
     if (nside <= 0 || (nside & (nside - 1))) fail("nside must be a power of two");
+    if (nside > stl::kMaxNside) fail("nside must be at most 8192 (32-bit pixel ids)");
     long order = 0;
     while ((1L << order) < nside) ++order;
     long j = v.integer("j", 7);
@@ -635,6 +636,7 @@ StlEnsemble* stlx_ensemble_new(const int32_t* nside, const int32_t* j, const int
     std::vector<stl::EnsembleGroup> gs(groups);
     for (size_t g = 0; g < groups; ++g) {
       if (nside[g] < 1 || (nside[g] & (nside[g] - 1))) fail("nside must be a power of two");
+      if (nside[g] > stl::kMaxNside) fail("nside must be at most 8192 (32-bit pixel ids)");
       int order = 0;
       while ((1 << order) < nside[g]) ++order;
       if (j[g] < 1 || j[g] > order + 1) fail("j must satisfy 1 <= j <= log2(nside) + 1");

@@ -105,6 +105,10 @@ __host__ __device__ inline int64_t mirror_pix(int64_t pix, int order) {
 // nside >= 8 (kernel 5, L = 4).
 constexpr int kWin = 5;
 
+// Largest nside: pixel ids are stored as int32 (12 nside^2 < 2^31) and the
+// kernels' bit interleaving works on 32-bit local indices.
+constexpr int kMaxNside = 8192;
+
 // Row pitch (in elements) of the LDS windows of the tiled adjoints, per tile
 // side and element size.  gfx950's LDS serves 64-bit reads 32 lanes at a
 // time over 64 four-byte banks: the lanes of a wave hold an 8 x 8 block of

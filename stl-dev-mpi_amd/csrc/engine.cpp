@@ -316,6 +316,7 @@ class EngineT final : public Engine {
     id_ = next_id.fetch_add(1);
     auto t0 = std::chrono::steady_clock::now();
     STL_CHECK(opt.nside >= 1 && (opt.nside & (opt.nside - 1)) == 0, "nside must be a power of two");
+    STL_CHECK(opt.nside <= kMaxNside, "nside must be at most 8192 (32-bit pixel ids)");
     int order0 = 0;
     while ((1 << order0) < opt.nside) ++order0;
     STL_CHECK(opt.J >= 1 && opt.J <= order0 + 1, "invalid number of scales");

@@ -318,22 +318,23 @@ __global__ __launch_bounds__(256) void k_conv_adj_split(ConvOp<T> op, const cplx
 // the mirrored cells.
 // ---------------------------------------------------------------------------
 
-__device__ inline int64_t compress_bits64(int64_t v) {
-  v &= 0x5555555555555555LL;
-  v = (v | (v >> 1)) & 0x3333333333333333LL;
-  v = (v | (v >> 2)) & 0x0F0F0F0F0F0F0F0FLL;
-  v = (v | (v >> 4)) & 0x00FF00FF00FF00FFLL;
-  v = (v | (v >> 8)) & 0x0000FFFF0000FFFFLL;
-  return (v | (v >> 16)) & 0x00000000FFFFFFFFLL;
+// The even bits of v packed (local NESTED index -> x; of v >> 1 -> y), and the
+// inverse spread, on 32-bit words: local indices of nside <= 8192 faces (the
+// engine's limit) fit.
+__device__ inline uint32_t compress_bits(uint32_t v) {
+  v &= 0x55555555u;
+  v = (v | (v >> 1)) & 0x33333333u;
+  v = (v | (v >> 2)) & 0x0F0F0F0Fu;
+  v = (v | (v >> 4)) & 0x00FF00FFu;
+  return (v | (v >> 8)) & 0x0000FFFFu;
 }
 
-__device__ inline int64_t spread_bits64(int64_t v) {
-  v &= 0x00000000FFFFFFFFLL;
-  v = (v | (v << 16)) & 0x0000FFFF0000FFFFLL;
-  v = (v | (v << 8)) & 0x00FF00FF00FF00FFLL;
-  v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0FLL;
-  v = (v | (v << 2)) & 0x3333333333333333LL;
-  return (v | (v << 1)) & 0x5555555555555555LL;
+__device__ inline uint32_t spread_bits(uint32_t v) {
+  v &= 0x0000FFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  return (v | (v << 1)) & 0x55555555u;
 }
 
 // HEALPix C++ neighbour faces / swaps by direction 4 + dx + 3 dy (dia.rs:27-41).
@@ -366,7 +367,7 @@ __device__ inline int64_t ext_pixel(int face, int ex, int ey, int order) {
   if (bits & 1) x = nside - x - 1;
   if (bits & 2) y = nside - y - 1;
   if (bits & 4) { int t = x; x = y; y = t; }
-  return (int64_t(f) << (2 * order)) + spread_bits64(x) + (spread_bits64(y) << 1);
+  return (int64_t(f) << (2 * order)) + int64_t(spread_bits(uint32_t(x)) | (spread_bits(uint32_t(y)) << 1));
 }
 
 struct TileInfo {
@@ -472,8 +473,8 @@ __device__ inline TileInfo tile_info(int order, int rows, int& row) {
   }
   t.base = tile * (TS * TS);
   t.face = 4 * t.fr + t.k;
-  t.x0 = int(compress_bits64(t.base));
-  t.y0 = int(compress_bits64(t.base >> 1));
+  t.x0 = int(compress_bits(uint32_t(t.base)));
+  t.y0 = int(compress_bits(uint32_t(t.base >> 1)));
   return t;
 }
 
@@ -527,7 +528,7 @@ void k_conv_fwd_tiled(ConvOp<T> op, const T* __restrict__ x, int rows,
   }
   __syncthreads();
   const int li = threadIdx.x, rl = rep_lane<TS>(ti.m, li);
-  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * SW + int(compress_bits64(rl)) + kWin;
+  const int pos = (int(compress_bits(uint32_t(rl >> 1))) + kWin) * SW + int(compress_bits(uint32_t(rl))) + kWin;
   const int64_t r = int64_t(ti.fr) * op.npf + ti.base + rl;
   // FP32: rep-frame sums, permuted to the face's gauges at the end (the same
   // products and additions); FP64 permutes each slot's coefficients (faster
@@ -624,7 +625,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
   bool m;
   unit_tile(tr, ntile, fr, tile, m);
   const int64_t base = tile * (TS * TS);
-  const int x0 = int(compress_bits64(base)), y0 = int(compress_bits64(base >> 1));
+  const int x0 = int(compress_bits(uint32_t(base))), y0 = int(compress_bits(uint32_t(base >> 1)));
   const int64_t N = 12 * op.npf;
   const T* xr = x + size_t(row) * N * NI;
   for (int idx = threadIdx.x; idx < FW * SA; idx += TS * TS) {
@@ -636,7 +637,7 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
   }
   __syncthreads();
   const int li = threadIdx.x, rl = rep_lane<TS>(m, li);
-  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * SW + int(compress_bits64(rl)) + kWin;
+  const int pos = (int(compress_bits(uint32_t(rl >> 1))) + kWin) * SW + int(compress_bits(uint32_t(rl))) + kWin;
   const int64_t r = int64_t(fr) * op.npf + base + rl;
   T acc[FW][NI][kL][2];
 #pragma unroll
@@ -807,7 +808,7 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
   const int64_t N = 12 * op.npf;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
   const int li = threadIdx.x, rl = rep_lane<TS>(ti.m, li);
-  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * PW + int(compress_bits64(rl)) + kWin;
+  const int pos = (int(compress_bits(uint32_t(rl >> 1))) + kWin) * PW + int(compress_bits(uint32_t(rl))) + kWin;
   const int64_t q = int64_t(ti.fr) * op.npf + ti.base + rl;
   const BlockSlots bs = block_slots<TS>(op, ti.fr, ti.base, rl);
   const WindowPlanes<RW * GP * NI, SA, cplx_t<T>> wp(pos);
@@ -926,11 +927,11 @@ void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
   bool m;
   unit_tile(tr, ntile, fr, tile, m);
   const int64_t base = tile * (TS * TS);
-  const int x0 = int(compress_bits64(base)), y0 = int(compress_bits64(base >> 1));
+  const int x0 = int(compress_bits(uint32_t(base))), y0 = int(compress_bits(uint32_t(base >> 1)));
   const int64_t N = 12 * op.npf;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
   const int li = threadIdx.x, rl = rep_lane<TS>(m, li);
-  const int pos = (int(compress_bits64(rl >> 1)) + kWin) * PW + int(compress_bits64(rl)) + kWin;
+  const int pos = (int(compress_bits(uint32_t(rl >> 1))) + kWin) * PW + int(compress_bits(uint32_t(rl))) + kWin;
   const int64_t q = int64_t(fr) * op.npf + base + rl;
   const BlockSlots bs = block_slots<TS>(op, fr, base, rl);
   const WindowPlanes<FW * GP * NI, SA, cplx_t<T>> wp(pos);
@@ -1061,7 +1062,7 @@ __global__ __launch_bounds__(TC * TC) void k_dec_fwd_tiled(DecOp<T> op, const T*
   }
   __syncthreads();
   const int li = threadIdx.x;
-  const int lx = int(compress_bits64(li)), ly = int(compress_bits64(li >> 1));
+  const int lx = int(compress_bits(uint32_t(li))), ly = int(compress_bits(uint32_t(li >> 1)));
   const int pos = (2 * ly - kDecLo) * SW + 2 * lx - kDecLo;
   const int64_t r = int64_t(ti.fr) * op.npf_c + ti.base + li;
   T acc[NI];
@@ -1100,7 +1101,7 @@ __global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T*
   }
   __syncthreads();
   const int li = threadIdx.x;
-  const int lx = int(compress_bits64(li)), ly = int(compress_bits64(li >> 1));
+  const int lx = int(compress_bits(uint32_t(li))), ly = int(compress_bits(uint32_t(li >> 1)));
   const int pos = ((ly >> 1) + kDecT) * SW + (lx >> 1) + kDecT;
   const int64_t q = int64_t(ti.fr) * op.npf_f + ti.base + li;
   T acc[NI];
@@ -2218,7 +2219,8 @@ void conv_fwd_faces32(const ConvOp<T>& op, int NI, const T* x, int rows, cplx_t<
 
 template <typename T>
 bool conv_fwd_has_variant(int NI, int variant) {
-  return variant != kConvFaces32 || NI == 1 || sizeof(T) == 4;
+  if (variant == kConvFaces32) return NI == 1 || sizeof(T) == 4;
+  return variant >= kConvTiled && variant < kConvFaces32;
 }
 
 template <typename T>
@@ -2290,7 +2292,7 @@ void conv_adj_faces32(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows
 template <typename T>
 bool conv_adj_has_variant(int NI, int variant) {
   if (variant == kConvFaces || variant == kConvFaces32) return NI == 1 || sizeof(T) == 4;
-  return true;
+  return variant >= kConvTiled && variant <= kConvFaces32;
 }
 
 template <typename T>

@@ -54,6 +54,7 @@ def test_abi_table_matches_the_header():
 @pytest.mark.parametrize("config, message", [
     ("{", "valid JSON"),
     ('{"nside": 12}', "power of two"),
+    ('{"nside": 16384, "j": 3}', "at most 8192"),
     ('{"nside": 16, "j": 9}', "invalid number of scales"),
     ('{"nside": 16, "j": 3, "kernel": 3}', "kernel"),
     ('{"nside": 16, "j": 3, "orientations": 2}', "orientations"),
