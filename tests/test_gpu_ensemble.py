@@ -3,7 +3,9 @@ over all groups per forward and per VJP, with operators shared between the
 groups' levels of equal nside.  Every group's statistics and gradients equal,
 bit for bit, those of a standalone engine of the same nside, batch and
 precision (the same kernels on the same operators), over repeated graph
-replays with changing inputs; the FP64 groups also match the oracle."""
+replays with changing inputs, with the coarse levels batched (the default) or
+launched per level (batch_nside=0, passed to every group); the FP64 groups also
+match the oracle."""
 import numpy as np
 import pytest
 
@@ -27,12 +29,16 @@ def maps(seed):
     return [rng.standard_normal((b, 12 * n * n)) for n, b in GROUPS]
 
 
+@pytest.mark.parametrize("batch_nside", [None, 0])
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_ensemble_equals_standalone_engines(precision):
+def test_ensemble_equals_standalone_engines(precision, batch_nside):
     from stl_port import NativeEngine, NativeEnsemble
-    print(f"[gpu] ensemble {GROUPS} precision={precision}", flush=True)
-    with NativeEnsemble(GROUPS, precision=precision) as ens:
-        singles = [NativeEngine(n, n.bit_length() - 2, batch=b, precision=precision) for n, b in GROUPS]
+    print(f"[gpu] ensemble {GROUPS} precision={precision} batch_nside={batch_nside}", flush=True)
+    with NativeEnsemble(GROUPS, precision=precision, batch_nside=batch_nside) as ens:
+        singles = [NativeEngine(n, n.bit_length() - 2, batch=b, precision=precision, batch_nside=batch_nside)
+                   for n, b in GROUPS]
+        if batch_nside == 0:
+            assert [e.info()["batch_level"] for e in singles] == [n.bit_length() - 2 for n, _ in GROUPS]
         rng = np.random.default_rng(5)
         for it in range(3):
             xs = maps(it % 2)
