@@ -411,6 +411,29 @@ __device__ inline int64_t staged_pixel(int face, bool m, int ex, int ey, int ord
   return (m && p >= 0) ? mirror_pix(p, order) : p;
 }
 
+// staged_pixel of a position inside the face (0 <= ex, ey < nside): the
+// NESTED index without the neighbour-face cases, on 32-bit words.
+__device__ inline int64_t inner_pixel(int face, bool m, int ex, int ey, int order) {
+  const uint32_t loc = spread_bits(uint32_t(ex)) | (spread_bits(uint32_t(ey)) << 1);
+  if (!m) return (int64_t(face) << (2 * order)) + loc;
+  const uint32_t v = loc ^ ((1u << (2 * order)) - 1u);
+  return (int64_t(mirror_face(face)) << (2 * order)) + (((v & 0x55555555u) << 1) | ((v >> 1) & 0x55555555u));
+}
+
+// Whether a tile's window (kWin around the TS x TS tile at (x0, y0)) lies
+// inside its face: then every staged pixel is an inner_pixel.
+template <int TS>
+__device__ inline bool window_inside(int x0, int y0, int order) {
+  const int nside = 1 << order;
+  return x0 >= kWin && y0 >= kWin && x0 + TS + kWin <= nside && y0 + TS + kWin <= nside;
+}
+
+// Pixel staged at a window position: the inner form when the whole window is
+// inside the face (a workgroup-uniform choice).
+__device__ inline int64_t window_pixel(bool inside, int face, bool m, int ex, int ey, int order) {
+  return inside ? inner_pixel(face, m, ex, ey, order) : staged_pixel(face, m, ex, ey, order);
+}
+
 // Staging position pi of a tile's window (SW x SW) -> its LDS index.  A
 // mirrored tile walks the window's columns (iy fastest): they are the rows of
 // the mirrored face, so that consecutive lanes read consecutive pixels along x
@@ -518,11 +541,12 @@ void k_conv_fwd_tiled(ConvOp<T> op, const T* __restrict__ x, int rows,
   __shared__ T xs[NI * SA];
   int row;
   const TileInfo ti = tile_info<TS, true>(op.order, rows, row);
+  const bool inside = window_inside<TS>(ti.x0, ti.y0, op.order);
   const int64_t N = 12 * op.npf;
   const T* xr = x + size_t(row) * N * NI;
   for (int i = threadIdx.x; i < SA; i += TS * TS) {
     const int idx = stage_index<SW>(ti.m, i);
-    int64_t p = staged_pixel(ti.face, ti.m, ti.x0 - kWin + idx % SW, ti.y0 - kWin + idx / SW, op.order);
+    int64_t p = window_pixel(inside, ti.face, ti.m, ti.x0 - kWin + idx % SW, ti.y0 - kWin + idx / SW, op.order);
 #pragma unroll
     for (int j = 0; j < NI; ++j) xs[j * SA + idx] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
   }
@@ -626,12 +650,13 @@ __global__ __launch_bounds__(TS * TS) void k_conv_fwd_faces(ConvOp<T> op, const 
   unit_tile(tr, ntile, fr, tile, m);
   const int64_t base = tile * (TS * TS);
   const int x0 = int(compress_bits(uint32_t(base))), y0 = int(compress_bits(uint32_t(base >> 1)));
+  const bool inside = window_inside<TS>(x0, y0, op.order);
   const int64_t N = 12 * op.npf;
   const T* xr = x + size_t(row) * N * NI;
   for (int idx = threadIdx.x; idx < FW * SA; idx += TS * TS) {
     const int f = idx / SA, pi = stage_index<SW>(m, idx % SA);
     const int face = 4 * fr + fg * FW + f;
-    int64_t p = staged_pixel(face, m, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order);
+    int64_t p = window_pixel(inside, face, m, x0 - kWin + pi % SW, y0 - kWin + pi / SW, op.order);
 #pragma unroll
     for (int j = 0; j < NI; ++j) xs[(f * NI + j) * SA + pi] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
   }
@@ -804,6 +829,7 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
   __shared__ cplx_t<T> gs[RW * GP * NI * SA];
   int rowg;
   const TileInfo ti = tile_info<TS, true>(op.order, rows / RW, rowg);
+  const bool inside = window_inside<TS>(ti.x0, ti.y0, op.order);
   const int row = rowg * RW;
   const int64_t N = 12 * op.npf;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
@@ -825,7 +851,7 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
     const int idx = threadIdx.x + i * TS * TS;
     int wx, wy;
     const bool in = idx < Grid::kSlots && Grid::at(idx, wx, wy);
-    spix[i] = in ? int32_t(staged_pixel(ti.face, ti.m, ti.x0 - kWin + wx, ti.y0 - kWin + wy, op.order)) : -2;
+    spix[i] = in ? int32_t(window_pixel(inside, ti.face, ti.m, ti.x0 - kWin + wx, ti.y0 - kWin + wy, op.order)) : -2;
   }
   T acc[RW][NI];
 #pragma unroll
@@ -928,6 +954,7 @@ void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
   unit_tile(tr, ntile, fr, tile, m);
   const int64_t base = tile * (TS * TS);
   const int x0 = int(compress_bits(uint32_t(base))), y0 = int(compress_bits(uint32_t(base >> 1)));
+  const bool inside = window_inside<TS>(x0, y0, op.order);
   const int64_t N = 12 * op.npf;
   const cplx_t<T>* gr = gy + size_t(row) * kL * N * NI;
   const int li = threadIdx.x, rl = rep_lane<TS>(m, li);
@@ -948,7 +975,8 @@ void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
     const int f = idx / Grid::kSlots;
     int wx, wy;
     const bool in = idx < FW * Grid::kSlots && Grid::at(idx % Grid::kSlots, wx, wy);
-    spix[i] = in ? int32_t(staged_pixel(4 * fr + fg * FW + f, m, x0 - kWin + wx, y0 - kWin + wy, op.order)) : -2;
+    spix[i] = in ? int32_t(window_pixel(inside, 4 * fr + fg * FW + f, m, x0 - kWin + wx, y0 - kWin + wy, op.order))
+                 : -2;
   }
   T acc[FW][NI];
 #pragma unroll
