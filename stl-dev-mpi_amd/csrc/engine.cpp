@@ -865,7 +865,8 @@ class EngineT final : public Engine {
       int64_t N = N_[n];
       int K = gram_k(n), E = cross_ ? K * K : gram_e(n);
       int64_t tiles = (N + gram_tile(n) - 1) / gram_tile(n);
-      int nb = int(std::min<int64_t>(tiles, std::max<int64_t>(1, 2048 / (int64_t(rows) * kL))));
+      int nb = cross_ ? int(std::min<int64_t>(tiles, std::max<int64_t>(1, 2048 / (int64_t(rows) * kL))))
+                      : gram_blocks<T>(n, N, int64_t(rows));
       nblk_gram_.push_back(nb);
       gpart_[n].resize(size_t(nb) * rows * kL * E * 2);
       gpart2_[n].resize(size_t(nb) * rows * kL * E);

@@ -1239,12 +1239,39 @@ __device__ inline bool in_mask(const MaskView& m, int64_t p, int k) {
 }
 __device__ inline int mask_class(const MaskView& m, int64_t p) { return m.cls == nullptr ? 0 : int(m.cls[p]); }
 
+// One DPP step of a 64-bit value: each half moves by the same lane pattern
+// (CTRL); rows left out of ROW_MASK read 0.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ inline double dpp_f64(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(u)), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(u >> 32)), CTRL, ROW_MASK, 0xF, false);
+  return __builtin_bit_cast(double, (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+}
+
+// Sum over the 64 lanes of a wave in a fixed order, all lanes active: quad
+// swaps (lane ^ 1, lane ^ 2), the half-row and row mirrors (every lane of a
+// row then holds the row's sum), row 0 into row 1 and row 2 into row 3 (row
+// broadcast of lane 15), rows 0-1 into row 3 (broadcast of lane 31).  The
+// total is in lane 63.
+__device__ inline double wave_sum_lane63(double v) {
+  v += dpp_f64<0xB1>(v);        // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4E>(v);        // quad_perm [2, 3, 0, 1]
+  v += dpp_f64<0x141>(v);       // row_half_mirror
+  v += dpp_f64<0x140>(v);       // row_mirror
+  v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3
+  return v;
+}
+
 // Gram partial sums: block (tile stride, b * kL + l3).
 // Gram products at the levels of small K (all E = K (K + 1) / 2 entries of a
 // pixel fit in registers): each thread reads its pixels' V straight from HBM
 // (40 bytes per pixel at K = 5, coalesced) and accumulates every entry in
-// FP64; the workgroup then sums its threads by a fixed butterfly within each
-// wave and the 4 wave sums in order (deterministic).
+// FP64 (two chained FMAs per real and imaginary part; the diagonal entries'
+// imaginary parts are exactly zero and not formed); the workgroup then sums
+// each wave's lanes in a fixed DPP order and the 4 wave sums in order
+// (deterministic).
 template <typename T, int KT, bool MOM, bool MASKED>
 __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const cplx_t<T>* __restrict__ br,
                                                   const cplx_t<T>* __restrict__ z, double* __restrict__ part,
@@ -1288,25 +1315,30 @@ __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const
     for (int i = 0; i < KT; ++i)
 #pragma unroll
       for (int j = i; j < KT; ++j, ++e) {
-        double re = v[i].x * v[j].x + v[i].y * v[j].y;
-        double im = v[i].y * v[j].x - v[i].x * v[j].y;
-        are[e] += re;
-        aim[e] += im;
-        if (MOM) asq[e] += re * re + im * im;
+        if constexpr (MOM) {
+          const double re = v[i].x * v[j].x + v[i].y * v[j].y;
+          const double im = i == j ? 0.0 : v[i].y * v[j].x - v[i].x * v[j].y;
+          are[e] += re;
+          aim[e] += im;
+          asq[e] += re * re + im * im;
+        } else {
+          are[e] = fma(v[i].x, v[j].x, are[e]);
+          are[e] = fma(v[i].y, v[j].y, are[e]);
+          if (i != j) {
+            aim[e] = fma(v[i].y, v[j].x, aim[e]);
+            aim[e] = fma(-v[i].x, v[j].y, aim[e]);
+          }
+        }
       }
   }
   (void)n;
   const int lane = tid & 63, w = tid >> 6;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    double a = are[e], c = aim[e], q = MOM ? asq[e] : 0.0;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      a += __shfl_xor(a, off);
-      c += __shfl_xor(c, off);
-      if (MOM) q += __shfl_xor(q, off);
-    }
-    if (lane == 0) {
+    const double a = wave_sum_lane63(are[e]);
+    const double c = wave_sum_lane63(aim[e]);
+    const double q = MOM ? wave_sum_lane63(asq[e]) : 0.0;
+    if (lane == 63) {
       red[w][0][e] = a;
       red[w][1][e] = c;
       if (MOM) red[w][MOM ? 2 : 1][e] = q;
@@ -1314,10 +1346,8 @@ __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const
   }
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    double a = zs[c];
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) a += __shfl_xor(a, off);
-    if (lane == 0) zred[w][c] = a;
+    const double a = wave_sum_lane63(zs[c]);
+    if (lane == 63) zred[w][c] = a;
   }
   __syncthreads();
   const size_t base = (size_t(blockIdx.x) * gridDim.y + bl) * E;
@@ -1335,6 +1365,188 @@ __global__ __launch_bounds__(256) void k_gram_reg(int n, int64_t N, int B, const
   }
   if (tid < 4)
     zpart[(size_t(blockIdx.x) * gridDim.y + bl) * 4 + tid] = zred[0][tid] + zred[1][tid] + zred[2][tid] + zred[3][tid];
+}
+
+// Gram products at the middle levels, K = 4 R + 1 (R = n + 1 = 3..6 branch
+// rows d).  The upper triangle of the K x K Gram is R (R + 1) / 2 blocks of
+// 4 x 4 entries between rows a <= b, plus the z column.  Lane class q of a
+// wave owns block (a_q, b_q): its S = 64 / P lanes q S .. q S + S - 1 take S
+// consecutive pixels, load rows a_q, b_q and z there and accumulate the 16
+// block products, the 4 products of row a_q with z (kept from the diagonal
+// blocks), z z and the S1/S2 sums of z (kept from class 0), in FP64
+// registers: every lane runs the same code on its own block, and the pixel
+// loop touches no LDS.  At the end the lanes of each class are summed in
+// wave and lane order through LDS (deterministic).
+template <int R>
+struct GramClasses {
+  static constexpr int P = R * (R + 1) / 2;  // classes
+  static constexpr int S = 64 / P;           // lanes (pixels) per class and wave
+  static constexpr int K = 4 * R + 1, E = K * (K + 1) / 2;
+  // per-lane values: block re [16], block im [16], z column re [4], im [4],
+  // z z, the 4 z sums; second moments: block [16], z column [4], z z
+  static constexpr int GR = 0, GI = 16, ZR = 32, ZI = 36, ZZ = 40, ZS = 41, SQ = 45, ZSQ = 61, ZZSQ = 65;
+  static constexpr int nv(bool mom) { return mom ? 66 : 45; }
+  __host__ __device__ static constexpr int cls(int a, int b) { return a * R - a * (a - 1) / 2 + (b - a); }
+};
+constexpr int kGramChunk = 8;  // values per LDS round of the class sums
+
+template <typename T, int R, bool MOM, bool MASKED>
+__global__ __launch_bounds__(256) void k_gram_cls(int64_t N, int B, const cplx_t<T>* __restrict__ br,
+                                                  const cplx_t<T>* __restrict__ z, double* __restrict__ part,
+                                                  double* __restrict__ part2, double* __restrict__ zpart,
+                                                  MaskView mask) {
+  using G = GramClasses<R>;
+  constexpr int P = G::P, S = G::S, K = G::K, E = G::E, NV = G::nv(MOM);
+  __shared__ double buf[kGramChunk][256];
+  __shared__ double red[P][NV];
+  const int bl = blockIdx.y;  // (b K + k) kL + l3
+  const int l3 = bl % kL, mk = (bl / kL) % mask.K, b = bl / (kL * mask.K);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int q = lane / S, kk = lane % S;
+  const bool live = q < P;
+  int qa = 0, rem = live ? q : 0;
+  while (rem >= R - qa) {
+    rem -= R - qa;
+    ++qa;
+  }
+  const int qb = qa + rem;
+  const bool diag = qa == qb;
+  const cplx_t<T>* ra = br + ((size_t(qa) * B + b) * kL + l3) * size_t(N) * kL;
+  const cplx_t<T>* rb = br + ((size_t(qb) * B + b) * kL + l3) * size_t(N) * kL;
+  const cplx_t<T>* zr = z + (size_t(b) * kL + l3) * size_t(N);
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  const int64_t step = int64_t(gridDim.x) * 4 * S;
+  struct Pix {
+    cplx_t<T> a[4], b[4], z;
+  };
+  auto fetch = [&](int64_t p, Pix& v) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      v.a[c] = ra[p * kL + c];
+      v.b[c] = rb[p * kL + c];
+    }
+    v.z = zr[p];
+  };
+  auto body = [&](const Pix& pv, int64_t p) {
+    if (MASKED && !in_mask(mask, p, mk)) return;
+    double2 va[4], vb[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      va[c] = to_d2<T>(pv.a[c]);
+      vb[c] = to_d2<T>(pv.b[c]);
+    }
+    const double2 vz = to_d2<T>(pv.z);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        double& gre = acc[G::GR + r * 4 + c];
+        double& gim = acc[G::GI + r * 4 + c];
+        if constexpr (MOM) {
+          const double re = va[r].x * vb[c].x + va[r].y * vb[c].y;
+          double im = va[r].y * vb[c].x - va[r].x * vb[c].y;
+          if (r == c) im = diag ? 0.0 : im;
+          gre += re;
+          gim += im;
+          acc[G::SQ + r * 4 + c] += re * re + im * im;
+        } else {
+          gre = fma(va[r].x, vb[c].x, gre);
+          gre = fma(va[r].y, vb[c].y, gre);
+          gim = fma(va[r].y, vb[c].x, gim);
+          gim = fma(-va[r].x, vb[c].y, gim);
+        }
+      }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if constexpr (MOM) {
+        const double re = va[r].x * vz.x + va[r].y * vz.y;
+        const double im = va[r].y * vz.x - va[r].x * vz.y;
+        acc[G::ZR + r] += re;
+        acc[G::ZI + r] += im;
+        acc[G::ZSQ + r] += re * re + im * im;
+      } else {
+        acc[G::ZR + r] = fma(va[r].x, vz.x, acc[G::ZR + r]);
+        acc[G::ZR + r] = fma(va[r].y, vz.y, acc[G::ZR + r]);
+        acc[G::ZI + r] = fma(va[r].y, vz.x, acc[G::ZI + r]);
+        acc[G::ZI + r] = fma(-va[r].x, vz.y, acc[G::ZI + r]);
+      }
+    }
+    const double zz = vz.x * vz.x + vz.y * vz.y;
+    acc[G::ZZ] += zz;
+    if constexpr (MOM) acc[G::ZZSQ] += zz * zz;
+    const double h = hypot(vz.x, vz.y);
+    acc[G::ZS] += h;
+    acc[G::ZS + 1] += h * h;
+    acc[G::ZS + 2] += zz;
+    acc[G::ZS + 3] += (h * h) * (h * h);
+  };
+  // Two pixel buffers in turn: a pass's values load while the previous
+  // pass's products run (addresses clamped, so every load is unconditional).
+  const int64_t p0 = live ? (int64_t(blockIdx.x) * 4 + w) * S + kk : N;
+  const int64_t np = p0 < N ? (N - 1 - p0) / step + 1 : 0;
+  Pix v0, v1;
+  if (np > 0) fetch(p0, v0);
+  int64_t t = 0;
+  for (; t + 1 < np; t += 2) {
+    fetch(p0 + (t + 1) * step, v1);
+    body(v0, p0 + t * step);
+    fetch(std::min(p0 + (t + 2) * step, N - 1), v0);
+    body(v1, p0 + (t + 1) * step);
+  }
+  if (t < np) body(v0, p0 + t * step);
+  // Class sums: kGramChunk values of every lane per LDS round; the value of
+  // class q is summed over waves 0..3, then lanes q S .. q S + S - 1.
+#pragma unroll
+  for (int c0 = 0; c0 < NV; c0 += kGramChunk) {
+#pragma unroll
+    for (int v = 0; v < kGramChunk; ++v)
+      if (c0 + v < NV) buf[v][tid] = acc[c0 + v];
+    __syncthreads();
+    if (tid < P * kGramChunk) {
+      const int cq = tid / kGramChunk, v = tid % kGramChunk;
+      if (c0 + v < NV) {
+        double s = 0.0;
+        for (int ww = 0; ww < 4; ++ww)
+#pragma unroll
+          for (int k = 0; k < S; ++k) s += buf[v][ww * 64 + cq * S + k];
+        red[cq][c0 + v] = s;
+      }
+    }
+    __syncthreads();
+  }
+  const size_t base = (size_t(blockIdx.x) * gridDim.y + bl) * E;
+  for (int e = tid; e < E; e += 256) {
+    int i = 0, re = e;
+    while (re >= K - i) {
+      re -= K - i;
+      ++i;
+    }
+    const int j = i + re;
+    double vr, vi, vq = 0.0;
+    if (j == K - 1) {
+      if (i == K - 1) {
+        vr = red[0][G::ZZ];
+        vi = 0.0;
+        if (MOM) vq = red[0][G::ZZSQ];
+      } else {
+        const int cq = G::cls(i / 4, i / 4), r = i % 4;
+        vr = red[cq][G::ZR + r];
+        vi = red[cq][G::ZI + r];
+        if (MOM) vq = red[cq][G::ZSQ + r];
+      }
+    } else {
+      const int cq = G::cls(i / 4, j / 4), rc = (i % 4) * 4 + j % 4;
+      vr = red[cq][G::GR + rc];
+      vi = i == j ? 0.0 : red[cq][G::GI + rc];
+      if (MOM) vq = red[cq][G::SQ + rc];
+    }
+    part[(base + e) * 2] = vr;
+    part[(base + e) * 2 + 1] = vi;
+    if (MOM) part2[base + e] = vq;
+  }
+  if (tid < 4) zpart[(size_t(blockIdx.x) * gridDim.y + bl) * 4 + tid] = red[0][G::ZS + tid];
 }
 
 // Gram products of one (b, l3) per workgroup row.  A tile of V [TPX][KP]
@@ -2411,6 +2623,32 @@ int gram_tile(int n) {
   return std::max(16, std::min(256, tpx));
 }
 
+template <typename T>
+int gram_blocks(int n, int64_t N, int64_t rows) {
+  const void* fn = nullptr;
+  int64_t step = 0;  // pixels per workgroup pass
+  switch (n + 1) {
+    case 1: fn = reinterpret_cast<const void*>(&k_gram_reg<T, 5, false, false>); step = 256; break;
+    case 2: fn = reinterpret_cast<const void*>(&k_gram_reg<T, 9, false, false>); step = 256; break;
+#define STL_GB(RR) \
+  case RR: fn = reinterpret_cast<const void*>(&k_gram_cls<T, RR, false, false>); step = 4 * GramClasses<RR>::S; break;
+    STL_GB(3) STL_GB(4) STL_GB(5) STL_GB(6)
+#undef STL_GB
+    default: break;
+  }
+  if (fn == nullptr) {  // the LDS-tiled kernel: tiles of gram_tile(n) pixels
+    const int64_t tiles = (N + gram_tile(n) - 1) / gram_tile(n);
+    return int(std::min<int64_t>(tiles, std::max<int64_t>(1, 2048 / (rows * kL))));
+  }
+  int dev = 0, cus = 0, per_cu = 0;
+  STL_HIP(hipGetDevice(&dev));
+  STL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  STL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0));
+  const int64_t target = int64_t(std::max(per_cu, 1)) * std::max(cus, 1);
+  const int64_t nb = std::min((target + rows * kL - 1) / (rows * kL), (N + step - 1) / step);
+  return int(std::max<int64_t>(nb, 1));
+}
+
 
 template <typename T>
 void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>* z, bool moments, int nblk,
@@ -2429,6 +2667,22 @@ void launch_gram(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<
   if (K == 5) STL_GRAM_REG(5, false)
   if (K == 9 && !moments) STL_GRAM_REG(9, false)
 #undef STL_GRAM_REG
+#define STL_GRAM_CLS(RR)                                                                                   \
+  case RR:                                                                                                 \
+    if (moments) {                                                                                         \
+      if (masked) k_gram_cls<T, RR, true, true><<<grid, 256, 0, s>>>(N, B, branch, z, part, part2, zpart, mask);   \
+      else k_gram_cls<T, RR, true, false><<<grid, 256, 0, s>>>(N, B, branch, z, part, part2, zpart, mask);         \
+    } else {                                                                                               \
+      if (masked) k_gram_cls<T, RR, false, true><<<grid, 256, 0, s>>>(N, B, branch, z, part, part2, zpart, mask);  \
+      else k_gram_cls<T, RR, false, false><<<grid, 256, 0, s>>>(N, B, branch, z, part, part2, zpart, mask);        \
+    }                                                                                                      \
+    STL_HIP(hipGetLastError());                                                                            \
+    return;
+  switch (n + 1) {
+    STL_GRAM_CLS(3) STL_GRAM_CLS(4) STL_GRAM_CLS(5) STL_GRAM_CLS(6)
+    default: break;
+  }
+#undef STL_GRAM_CLS
   // Small K: 2 x 2 blocks waste fewer padded products; larger K: 4 x 4 blocks
   // halve the LDS reads per product.
   const int BS = K <= 13 ? 2 : 4;
@@ -2616,6 +2870,7 @@ void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s) {
   template void launch_dec_adj<T>(const DecOp<T>&, int, const T*, const T*, int, T*, hipStream_t);            \
   template void launch_gram<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, bool, int, double*,     \
                                double*, double*, const MaskView&, hipStream_t);                               \
+  template int gram_blocks<T>(int, int64_t, int64_t);                                                          \
   template void launch_zstats<T>(int64_t, int, const cplx_t<T>*, int, double*, const MaskView&, hipStream_t); \
   template void launch_meanvar<T>(int64_t, int, const T*, int, double*, const MaskView&, hipStream_t);        \
   template void launch_gv<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, const double2*,           \

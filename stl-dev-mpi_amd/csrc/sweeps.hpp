@@ -68,6 +68,11 @@ struct MaskView {
 
 // Pixels per LDS tile of the Gram kernel at level n.
 int gram_tile(int n);
+// Workgroups per (row, orientation) of launch_gram at level n for `rows`
+// (map, mask) rows: as many as the device's CUs hold at once over all rows,
+// at most one per pixel pass.  Any count >= 1 is valid for every Gram kernel.
+template <typename T>
+int gram_blocks(int n, int64_t N, int64_t rows);
 
 // Per-block partial sums of the Gram products V_i conj(V_j), i <= j, of
 // V = [branch(d)[l1][l3] for d <= n, l1 < kL; z[l3]] per (b, mask k, l3)
