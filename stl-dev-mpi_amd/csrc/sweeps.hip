@@ -1068,6 +1068,75 @@ __global__ __launch_bounds__(256) void k_conv_adj_fix(ConvOp<T> op, const cplx_t
   conv_adj_fix_body<T, NI>(op, gy, gx, blockIdx.x, blockIdx.y);
 }
 
+// Entries of a decimation row loaded before its window is staged (their
+// latency then overlaps the staging's).  Slots beyond the row's count read
+// the last stored slot's offset (padding offsets are 0, inside the window)
+// with a zero weight.
+constexpr int kDecPre = 8;
+
+template <typename T>
+__device__ inline void dec_prefetch(const int16_t* __restrict__ off, const T* __restrict__ val, int64_t R, int cap,
+                                    int64_t r, int cnt, int16_t (&eo)[kDecPre], T (&ev)[kDecPre]) {
+#pragma unroll
+  for (int s = 0; s < kDecPre; ++s) {
+    const size_t at = size_t(min(s, cap - 1)) * R + r;
+    eo[s] = off[at];
+    ev[s] = s < cnt ? val[at] : T(0);
+  }
+}
+
+// Window position of staging index idx in 4 x 4 pixel blocks (SW / 4 blocks
+// per side): 16 consecutive indices are one block, whose pixels are 16
+// consecutive NESTED ids (the window's corner is a multiple of 4, and the
+// neighbour-face maps send aligned blocks to aligned blocks), so a wave's
+// loads are 4 contiguous runs.
+template <int SW>
+__device__ inline void block_pos(int idx, int& wx, int& wy) {
+  static_assert(SW % 4 == 0, "window side");
+  const int blk = idx >> 4, w = idx & 15;
+  wx = 4 * (blk % (SW / 4)) + int(compress_bits(uint32_t(w)));
+  wy = 4 * (blk / (SW / 4)) + int(compress_bits(uint32_t(w >> 1)));
+}
+
+__device__ inline bool span_inside(int x0, int y0, int w, int order) {
+  const int nside = 1 << order;
+  return x0 >= 0 && y0 >= 0 && x0 + w <= nside && y0 + w <= nside;
+}
+
+// Stage a decimation window of SA pixels (position -> pixel by `pix`, -1
+// outside the sphere's extension) into LDS planes [j][SA].  The thread's
+// pixel ids come first (outside the face's interior they read the neighbour
+// tables, whose waits would otherwise also wait for data loads in flight),
+// then every data load is issued (missing pixels read pixel 0 of the row and
+// store 0) before the first LDS write, so their latencies overlap.
+template <typename T, int NI, int SA, int NT, typename Pix>
+__device__ __forceinline__ void stage_window(const T* __restrict__ xr, T* xs, Pix pix) {
+  constexpr int NIT = (SA + NT - 1) / NT;
+  int64_t px[NIT];
+  int at[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int idx = int(threadIdx.x) + it * NT;
+    const bool in = idx < SA;
+    int lp = 0;
+    const int64_t p = pix(in ? idx : 0, lp);
+    px[it] = in ? p : -1;
+    at[it] = in ? lp : -1;
+  }
+  T v[NIT][NI];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const size_t src = size_t(px[it] >= 0 ? px[it] : 0) * NI;
+#pragma unroll
+    for (int j = 0; j < NI; ++j) v[it][j] = xr[src + j];
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it)
+    if (at[it] >= 0)
+#pragma unroll
+      for (int j = 0; j < NI; ++j) xs[j * SA + at[it]] = px[it] >= 0 ? v[it][j] : T(0);
+}
+
 // LDS-tiled decimation: a workgroup is a TC x TC tile of coarse pixels of one
 // face and one row; the fine window of their candidates, fine offsets
 // kDecLo..kDecHi around the tile's children ((2 TC + 8)^2 pixels), is staged
@@ -1083,21 +1152,33 @@ __global__ __launch_bounds__(TC * TC) void k_dec_fwd_tiled(DecOp<T> op, const T*
   const int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
   const T* xr = x + size_t(row) * Nf * NI;
   const int fx0 = 2 * ti.x0 + kDecLo, fy0 = 2 * ti.y0 + kDecLo;
-  for (int idx = threadIdx.x; idx < SA; idx += TC * TC) {
-    int64_t p = ext_pixel(ti.face, fx0 + idx % SW, fy0 + idx / SW, op.order_f);
-#pragma unroll
-    for (int j = 0; j < NI; ++j) xs[j * SA + idx] = p >= 0 ? xr[size_t(p) * NI + j] : T(0);
-  }
-  __syncthreads();
   const int li = threadIdx.x;
+  const int64_t r = int64_t(ti.fr) * op.npf_c + ti.base + li;
+  const int cnt = op.cnt[r];
+  int16_t eo[kDecPre];
+  T ev[kDecPre];
+  dec_prefetch<T>(op.off, op.val, op.Rc, op.S, r, cnt, eo, ev);
+  const bool inside = span_inside(fx0, fy0, SW, op.order_f);
+  stage_window<T, NI, SA, TC * TC>(xr, xs, [&](int idx, int& lp) {
+    int wx, wy;
+    block_pos<SW>(idx, wx, wy);
+    lp = wy * SW + wx;
+    return inside ? inner_pixel(ti.face, false, fx0 + wx, fy0 + wy, op.order_f)
+                  : ext_pixel(ti.face, fx0 + wx, fy0 + wy, op.order_f);
+  });
+  __syncthreads();
   const int lx = int(compress_bits(uint32_t(li))), ly = int(compress_bits(uint32_t(li >> 1)));
   const int pos = (2 * ly - kDecLo) * SW + 2 * lx - kDecLo;
-  const int64_t r = int64_t(ti.fr) * op.npf_c + ti.base + li;
   T acc[NI];
 #pragma unroll
   for (int j = 0; j < NI; ++j) acc[j] = T(0);
-  const int cnt = op.cnt[r];
-  for (int s = 0; s < cnt; ++s) {
+#pragma unroll
+  for (int s = 0; s < kDecPre; ++s) {
+    const int src = pos + off_dy(eo[s]) * SW + off_dx(eo[s]);
+#pragma unroll
+    for (int j = 0; j < NI; ++j) acc[j] += ev[s] * xs[j * SA + src];
+  }
+  for (int s = kDecPre; s < cnt; ++s) {
     size_t at = size_t(s) * op.Rc + r;
     int16_t o = op.off[at];
     const T v = op.val[at];
@@ -1111,7 +1192,9 @@ __global__ __launch_bounds__(TC * TC) void k_dec_fwd_tiled(DecOp<T> op, const T*
 }
 
 // Its transpose: a TF x TF tile of fine cells gathers from the coarse window
-// of their parents +- kDecT ((TF / 2 + 2 kDecT)^2 pixels) staged in LDS.
+// of their parents +- kDecT ((TF / 2 + 2 kDecT)^2 pixels) staged in LDS.  The
+// cell's first entries and the fine term it adds to are loaded before the
+// staging.
 template <typename T, int NI, int TF>
 __global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T* __restrict__ gy, const T* add,
                                                            int rows, T* gx) {
@@ -1122,32 +1205,46 @@ __global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T*
   const int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
   const T* gr = gy + size_t(row) * Nc * NI;
   const int cx0 = ti.x0 / 2 - kDecT, cy0 = ti.y0 / 2 - kDecT;
-  for (int idx = threadIdx.x; idx < SA; idx += TF * TF) {
-    int64_t p = ext_pixel(ti.face, cx0 + idx % SW, cy0 + idx / SW, op.order_c);
-#pragma unroll
-    for (int j = 0; j < NI; ++j) gs[j * SA + idx] = p >= 0 ? gr[size_t(p) * NI + j] : T(0);
-  }
-  __syncthreads();
   const int li = threadIdx.x;
+  const int64_t q = int64_t(ti.fr) * op.npf_f + ti.base + li;
+  const int64_t p = int64_t(ti.face) * op.npf_f + ti.base + li;
+  const size_t o = (size_t(row) * Nf + p) * NI;
+  const int cnt = op.tcnt[q];
+  int16_t eo[kDecPre];
+  T ev[kDecPre];
+  dec_prefetch<T>(op.toff, op.tval, op.Rf, op.Tn, q, cnt, eo, ev);
+  T base[NI];
+#pragma unroll
+  for (int j = 0; j < NI; ++j) base[j] = add ? add[o + j] : T(0);
+  const bool inside = span_inside(cx0, cy0, SW, op.order_c);
+  stage_window<T, NI, SA, TF * TF>(gr, gs, [&](int idx, int& lp) {
+    const int wx = idx % SW, wy = idx / SW;
+    lp = idx;
+    return inside ? inner_pixel(ti.face, false, cx0 + wx, cy0 + wy, op.order_c)
+                  : ext_pixel(ti.face, cx0 + wx, cy0 + wy, op.order_c);
+  });
+  __syncthreads();
   const int lx = int(compress_bits(uint32_t(li))), ly = int(compress_bits(uint32_t(li >> 1)));
   const int pos = ((ly >> 1) + kDecT) * SW + (lx >> 1) + kDecT;
-  const int64_t q = int64_t(ti.fr) * op.npf_f + ti.base + li;
   T acc[NI];
 #pragma unroll
   for (int j = 0; j < NI; ++j) acc[j] = T(0);
-  const int cnt = op.tcnt[q];
-  for (int s = 0; s < cnt; ++s) {
+#pragma unroll
+  for (int s = 0; s < kDecPre; ++s) {
+    const int src = pos + off_dy(eo[s]) * SW + off_dx(eo[s]);
+#pragma unroll
+    for (int j = 0; j < NI; ++j) acc[j] += ev[s] * gs[j * SA + src];
+  }
+  for (int s = kDecPre; s < cnt; ++s) {
     size_t at = size_t(s) * op.Rf + q;
-    int16_t o = op.toff[at];
+    int16_t e = op.toff[at];
     const T v = op.tval[at];
-    const int src = pos + off_dy(o) * SW + off_dx(o);
+    const int src = pos + off_dy(e) * SW + off_dx(e);
 #pragma unroll
     for (int j = 0; j < NI; ++j) acc[j] += v * gs[j * SA + src];
   }
-  const int64_t p = int64_t(ti.face) * op.npf_f + ti.base + li;
-  const size_t o = (size_t(row) * Nf + p) * NI;
 #pragma unroll
-  for (int j = 0; j < NI; ++j) gx[o + j] = add ? acc[j] + add[o + j] : acc[j];
+  for (int j = 0; j < NI; ++j) gx[o + j] = add ? acc[j] + base[j] : acc[j];
 }
 
 template <typename T, int NI>
