@@ -532,6 +532,41 @@ __device__ inline BlockSlots fwd_block_slots(const ConvOp<T>& op, int fr, int64_
 template <typename T, int TS>
 constexpr int tile32_waves() { return TS == 32 && sizeof(T) == 4 ? 8 : 1; }
 
+// Stage COUNT window positions (position -> pixel by `pix`, -1 outside the
+// sphere's extension, and its LDS index) into LDS planes of PLANE elements,
+// NI planes per position.  The thread's pixel ids come first (outside a
+// face's interior they read the neighbour tables, whose waits would otherwise
+// also wait for data loads in flight), then every data load is issued
+// (missing pixels read pixel 0 of the row and store 0) before the first LDS
+// write, so their latencies overlap.
+template <typename T, int NI, int COUNT, int PLANE, int NT, typename Pix>
+__device__ __forceinline__ void stage_window(const T* __restrict__ xr, T* xs, Pix pix) {
+  constexpr int NIT = (COUNT + NT - 1) / NT;
+  int64_t px[NIT];
+  int at[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int idx = int(threadIdx.x) + it * NT;
+    const bool in = idx < COUNT;
+    int lp = 0;
+    const int64_t p = pix(in ? idx : 0, lp);
+    px[it] = in ? p : -1;
+    at[it] = in ? lp : -1;
+  }
+  T v[NIT][NI];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const size_t src = size_t(px[it] >= 0 ? px[it] : 0) * NI;
+#pragma unroll
+    for (int j = 0; j < NI; ++j) v[it][j] = xr[src + j];
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it)
+    if (at[it] >= 0)
+#pragma unroll
+      for (int j = 0; j < NI; ++j) xs[j * PLANE + at[it]] = px[it] >= 0 ? v[it][j] : T(0);
+}
+
 template <typename T, int NI, int TS>
 __global__ __launch_bounds__(TS * TS) __attribute__((amdgpu_waves_per_eu(tile32_waves<T, TS>())))
 void k_conv_fwd_tiled(ConvOp<T> op, const T* __restrict__ x, int rows,
@@ -795,6 +830,17 @@ __device__ __forceinline__ void pipelined_slots(const int32_t* __restrict__ off,
   if (c < nch) run(a0, c * U);
 }
 
+// Window positions staged per batch by the adjoints (every load of a batch
+// is in flight before its LDS writes): values of at most 48 VGPRs with a
+// budget (512 / waves per SIMD) of 128 or more, budget / 8 below.
+template <typename E, int PER, int VALS, int BUDGET>
+constexpr int stage_batch() {
+  constexpr int regs = VALS * int(sizeof(E)) / 4;
+  constexpr int cap = BUDGET >= 128 ? 48 : BUDGET / 8;
+  constexpr int ch = regs >= cap ? 1 : cap / regs;
+  return ch < PER ? ch : PER;
+}
+
 // Staging order of an adjoint's window (SW x SW positions per face): blocks
 // of 8 x 4 positions, x fastest, one block per 32 consecutive staging slots,
 // so that 32 lanes write 4 rows of 8 LDS words (conflict-free at the pitches
@@ -865,27 +911,48 @@ void k_conv_adj_tiled(ConvOp<T> op, const cplx_t<T>* __restrict__ gy, int rows,
     // pass and share them in L2.
     const int g0 = GP == kL ? 0 : ((pass * GP) ^ (2 * (ti.k & 1)));
     __syncthreads();
+    // In batches of CH positions: every load of a batch is issued (missing
+    // pixels read pixel 0 and store 0) before its LDS writes.
+    constexpr int kWaves = RW == 1 ? tile32_waves<T, TS>() : adj_rows_waves<RW>();
+    constexpr int CH = stage_batch<cplx_t<T>, kPer, GP * RW * NI, 512 / kWaves>();
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-      if (spix[i] == -2) continue;
-      const int64_t p = spix[i];
-      int wx, wy;
-      Grid::at(threadIdx.x + i * TS * TS, wx, wy);
-      const int pi = wy * PW + wx;
+    for (int i0 = 0; i0 < kPer; i0 += CH) {
+      cplx_t<T> sv[CH][GP][RW][NI];
 #pragma unroll
-      for (int gg = 0; gg < GP; ++gg) {
-        // The face shift's (and the mirror's) conjugation of a gauge is
-        // applied to the staged cotangent: a.y (-v.y) instead of (-a.y) v.y,
-        // the same products.
-        const bool cj = gauge_conj(ti.k, g0 + gg) != ti.m;
+      for (int c = 0; c < CH; ++c) {
+        if (i0 + c >= kPer) break;
+        const int64_t p = spix[i0 + c] >= 0 ? spix[i0 + c] : 0;
 #pragma unroll
-        for (int r = 0; r < RW; ++r)
+        for (int gg = 0; gg < GP; ++gg)
 #pragma unroll
-          for (int j = 0; j < NI; ++j) {
-            cplx_t<T> v = p >= 0 ? gr[((size_t(r) * kL + g0 + gg) * N + p) * NI + j] : make_c<T>(T(0), T(0));
-            if (cj) v.y = -v.y;
-            gs[((r * GP + gg) * NI + j) * SA + pi] = v;
-          }
+          for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int j = 0; j < NI; ++j) sv[c][gg][r][j] = gr[((size_t(r) * kL + g0 + gg) * N + p) * NI + j];
+      }
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int i = i0 + c;
+        if (i >= kPer) break;
+        if (spix[i] == -2) continue;
+        const bool ok = spix[i] >= 0;
+        int wx, wy;
+        Grid::at(threadIdx.x + i * TS * TS, wx, wy);
+        const int pi = wy * PW + wx;
+#pragma unroll
+        for (int gg = 0; gg < GP; ++gg) {
+          // The face shift's (and the mirror's) conjugation of a gauge is
+          // applied to the staged cotangent: a.y (-v.y) instead of (-a.y) v.y,
+          // the same products.
+          const bool cj = gauge_conj(ti.k, g0 + gg) != ti.m;
+#pragma unroll
+          for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int j = 0; j < NI; ++j) {
+              cplx_t<T> v = ok ? sv[c][gg][r][j] : make_c<T>(T(0), T(0));
+              if (cj) v.y = -v.y;
+              gs[((r * GP + gg) * NI + j) * SA + pi] = v;
+            }
+        }
       }
     }
     __syncthreads();
@@ -985,25 +1052,47 @@ void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
     for (int j = 0; j < NI; ++j) acc[f][j] = T(0);
   for (int pass = 0; pass < kL / GP; ++pass) {
     __syncthreads();
+    // In batches of CH positions: every load of a batch is issued (missing
+    // pixels read pixel 0 and store 0) before its LDS writes.
+    constexpr int CH = stage_batch<cplx_t<T>, kPer, GP * NI, 512 / adj_lds_waves<T, NI, TS, FW, GP>()>();
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-      if (spix[i] == -2) continue;
-      const int64_t p = spix[i];
-      const int idx = threadIdx.x + i * TS * TS;
-      const int f = idx / Grid::kSlots;
-      int wx, wy;
-      Grid::at(idx % Grid::kSlots, wx, wy);
-      const int pi = wy * PW + wx;
-      const int k = fg * FW + f;
+    for (int i0 = 0; i0 < kPer; i0 += CH) {
+      cplx_t<T> sv[CH][GP][NI];
 #pragma unroll
-      for (int gg = 0; gg < GP; ++gg) {
-        const int g = (pass * GP + gg) ^ (2 * (k & 1));
-        const bool cj = gauge_conj(k, g) != m;
+      for (int c = 0; c < CH; ++c) {
+        const int i = i0 + c;
+        if (i >= kPer) break;
+        const int64_t p = spix[i] >= 0 ? spix[i] : 0;
+        const int k = fg * FW + (threadIdx.x + i * TS * TS) / Grid::kSlots;
 #pragma unroll
-        for (int j = 0; j < NI; ++j) {
-          cplx_t<T> v = p >= 0 ? gr[(size_t(g) * N + p) * NI + j] : make_c<T>(T(0), T(0));
-          if (cj) v.y = -v.y;
-          gs[((f * GP + gg) * NI + j) * SA + pi] = v;
+        for (int gg = 0; gg < GP; ++gg) {
+          const int g = (pass * GP + gg) ^ (2 * (k & 1));
+#pragma unroll
+          for (int j = 0; j < NI; ++j) sv[c][gg][j] = gr[(size_t(g) * N + p) * NI + j];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int i = i0 + c;
+        if (i >= kPer) break;
+        if (spix[i] == -2) continue;
+        const bool ok = spix[i] >= 0;
+        const int idx = threadIdx.x + i * TS * TS;
+        const int f = idx / Grid::kSlots;
+        int wx, wy;
+        Grid::at(idx % Grid::kSlots, wx, wy);
+        const int pi = wy * PW + wx;
+        const int k = fg * FW + f;
+#pragma unroll
+        for (int gg = 0; gg < GP; ++gg) {
+          const int g = (pass * GP + gg) ^ (2 * (k & 1));
+          const bool cj = gauge_conj(k, g) != m;
+#pragma unroll
+          for (int j = 0; j < NI; ++j) {
+            cplx_t<T> v = ok ? sv[c][gg][j] : make_c<T>(T(0), T(0));
+            if (cj) v.y = -v.y;
+            gs[((f * GP + gg) * NI + j) * SA + pi] = v;
+          }
         }
       }
     }
@@ -1103,40 +1192,6 @@ __device__ inline bool span_inside(int x0, int y0, int w, int order) {
   return x0 >= 0 && y0 >= 0 && x0 + w <= nside && y0 + w <= nside;
 }
 
-// Stage a decimation window of SA pixels (position -> pixel by `pix`, -1
-// outside the sphere's extension) into LDS planes [j][SA].  The thread's
-// pixel ids come first (outside the face's interior they read the neighbour
-// tables, whose waits would otherwise also wait for data loads in flight),
-// then every data load is issued (missing pixels read pixel 0 of the row and
-// store 0) before the first LDS write, so their latencies overlap.
-template <typename T, int NI, int SA, int NT, typename Pix>
-__device__ __forceinline__ void stage_window(const T* __restrict__ xr, T* xs, Pix pix) {
-  constexpr int NIT = (SA + NT - 1) / NT;
-  int64_t px[NIT];
-  int at[NIT];
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    const int idx = int(threadIdx.x) + it * NT;
-    const bool in = idx < SA;
-    int lp = 0;
-    const int64_t p = pix(in ? idx : 0, lp);
-    px[it] = in ? p : -1;
-    at[it] = in ? lp : -1;
-  }
-  T v[NIT][NI];
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    const size_t src = size_t(px[it] >= 0 ? px[it] : 0) * NI;
-#pragma unroll
-    for (int j = 0; j < NI; ++j) v[it][j] = xr[src + j];
-  }
-#pragma unroll
-  for (int it = 0; it < NIT; ++it)
-    if (at[it] >= 0)
-#pragma unroll
-      for (int j = 0; j < NI; ++j) xs[j * SA + at[it]] = px[it] >= 0 ? v[it][j] : T(0);
-}
-
 // LDS-tiled decimation: a workgroup is a TC x TC tile of coarse pixels of one
 // face and one row; the fine window of their candidates, fine offsets
 // kDecLo..kDecHi around the tile's children ((2 TC + 8)^2 pixels), is staged
@@ -1159,7 +1214,7 @@ __global__ __launch_bounds__(TC * TC) void k_dec_fwd_tiled(DecOp<T> op, const T*
   T ev[kDecPre];
   dec_prefetch<T>(op.off, op.val, op.Rc, op.S, r, cnt, eo, ev);
   const bool inside = span_inside(fx0, fy0, SW, op.order_f);
-  stage_window<T, NI, SA, TC * TC>(xr, xs, [&](int idx, int& lp) {
+  stage_window<T, NI, SA, SA, TC * TC>(xr, xs, [&](int idx, int& lp) {
     int wx, wy;
     block_pos<SW>(idx, wx, wy);
     lp = wy * SW + wx;
@@ -1217,7 +1272,7 @@ __global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T*
 #pragma unroll
   for (int j = 0; j < NI; ++j) base[j] = add ? add[o + j] : T(0);
   const bool inside = span_inside(cx0, cy0, SW, op.order_c);
-  stage_window<T, NI, SA, TF * TF>(gr, gs, [&](int idx, int& lp) {
+  stage_window<T, NI, SA, SA, TF * TF>(gr, gs, [&](int idx, int& lp) {
     const int wx = idx % SW, wy = idx / SW;
     lp = idx;
     return inside ? inner_pixel(ti.face, false, cx0 + wx, cy0 + wy, op.order_c)
