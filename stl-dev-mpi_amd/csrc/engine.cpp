@@ -640,11 +640,13 @@ class EngineT final : public Engine {
     if (n + 1 < levels_) dec_adj(n, 1, mg_of(n + 1), mg, B_, mg);
   }
 
+  // The map cotangent out in FP64 (with the mean / variance terms, in one pass).
   void rev_finish(double* grad_dev) {
     if (!cross_)
       launch_meanvar_grad<T>(N_[0], B_, f_->maps[0].data(), nmask_ > 0 ? gmc_.data() : dgm_.data(),
-                             nmask_ > 0 ? gvc_.data() : dgv_.data(), f_->mg[0].data(), mask_view(0), s_);
-    cast_out(f_->mg[0].data(), grad_dev, int64_t(B_) * N_[0]);
+                             nmask_ > 0 ? gvc_.data() : dgv_.data(), f_->mg[0].data(), grad_dev, mask_view(0), s_);
+    else
+      cast_out(f_->mg[0].data(), grad_dev, int64_t(B_) * N_[0]);
   }
 
   // Staging of apply_operator: the operator's input and output at the
@@ -1458,7 +1460,7 @@ class EngineT final : public Engine {
         c.out0 = f_->qchain[n].data();
         chain.push_back(Step{{c}, dec_bytes(n, kL, (n + 1) * B_, true), tag("batch_dec_adj", n)});
       }
-      LevelTask<T> z = base(kQGz, n, B_ * kL, blocks(N, 256));
+      LevelTask<T> z = base(kQGz, n, B_, blocks(N, 256));
       z.in0 = f_->z[n].data();
       z.in1 = f_->qgzp[n].data();
       z.in2 = f_->qchain[n].data();

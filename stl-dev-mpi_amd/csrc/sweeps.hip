@@ -2336,24 +2336,33 @@ __global__ __launch_bounds__(256) void k_gv_small(int64_t N, int B, const cplx_t
   gv_small_body<T, NL>(N, B, br, z, H, gbr, gzp, blockIdx.x, blockIdx.y, lds);
 }
 
+// gz of every orientation of one pixel (thread per (b, p)): z, its gV part
+// and gz are [b][l][N] planes; the pixel's chain row ([n][b][N][kL]) is one
+// vector load; the S1/S2 cotangents are per (b, l, mask class).
 template <typename T>
 __device__ __forceinline__ void gz_body(int n, int64_t N, int B, const cplx_t<T>* __restrict__ z,
                                         const cplx_t<T>* __restrict__ gzp, const T* __restrict__ chain,
                                         const double* __restrict__ s1, const double* __restrict__ s2,
-                                        cplx_t<T>* __restrict__ gz, const MaskView& mask, int64_t bx, int bl) {
+                                        cplx_t<T>* __restrict__ gz, const MaskView& mask, int64_t bx, int b) {
   int64_t p = bx * 256 + threadIdx.x;
   if (p >= N) return;
-  int b = bl / kL, l = bl % kL;
-  size_t at = size_t(bl) * N + p;
-  double zr = double(z[at].x), zi = double(z[at].y);
-  double mag = hypot(zr, zi);
-  double den = mag == 0.0 ? 1.0 : mag;
-  double h = double(chain[((size_t(n) * B + b) * N + p) * kL + l]);
-  const size_t sc = size_t(bl) * mask.C + mask_class(mask, p);
-  double a = s1[sc], c = s2[sc];
-  double re = double(gzp[at].x) + (a * zr) / den + c * zr + (h * zr) / den;
-  double im = double(gzp[at].y) + (a * zi) / den + c * zi + (h * zi) / den;
-  gz[at] = make_c<T>(T(re), T(im));
+  T hv[kL];
+  Load<T>::template vec<kL>(chain + ((size_t(n) * B + b) * N + p) * kL, hv);
+  const int cls = mask_class(mask, p);
+#pragma unroll
+  for (int l = 0; l < kL; ++l) {
+    const int bl = b * kL + l;
+    const size_t at = size_t(bl) * N + p;
+    double zr = double(z[at].x), zi = double(z[at].y);
+    double mag = hypot(zr, zi);
+    double den = mag == 0.0 ? 1.0 : mag;
+    double h = double(hv[l]);
+    const size_t sc = size_t(bl) * mask.C + cls;
+    double a = s1[sc], c = s2[sc];
+    double re = double(gzp[at].x) + (a * zr) / den + c * zr + (h * zr) / den;
+    double im = double(gzp[at].y) + (a * zi) / den + c * zi + (h * zi) / den;
+    gz[at] = make_c<T>(T(re), T(im));
+  }
 }
 
 template <typename T>
@@ -2364,15 +2373,18 @@ __global__ __launch_bounds__(256) void k_gz(int n, int64_t N, int B, const cplx_
   gz_body<T>(n, N, B, z, gzp, chain, s1, s2, gz, mask, blockIdx.x, blockIdx.y);
 }
 
+// The map cotangent out (FP64): g0 plus the mean / variance terms, rounded
+// to the engine's precision first (as the map cotangent it completes).
 template <typename T>
 __global__ void k_meanvar_grad(int64_t N, const T* __restrict__ x, const double* __restrict__ gmean,
-                               const double* __restrict__ gvar, T* __restrict__ g0, MaskView mask) {
+                               const double* __restrict__ gvar, const T* __restrict__ g0, double* __restrict__ out,
+                               MaskView mask) {
   int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (p >= N) return;
   int b = blockIdx.y;
   size_t at = size_t(b) * N + p;
   const size_t sc = size_t(b) * mask.C + mask_class(mask, p);
-  g0[at] = T(double(g0[at]) + gmean[sc] + gvar[sc] * double(x[at]));
+  out[at] = double(T(double(g0[at]) + gmean[sc] + gvar[sc] * double(x[at])));
 }
 
 template <typename TO, typename TI>
@@ -2996,14 +3008,14 @@ void launch_gv(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>
 template <typename T>
 void launch_gz(int n, int64_t N, int B, const cplx_t<T>* z, const cplx_t<T>* gz_part, const T* chain,
                const double* s1, const double* s2, cplx_t<T>* gz, const MaskView& mask, hipStream_t s) {
-  k_gz<T><<<dim3(nblocks(N, 256), B * kL), 256, 0, s>>>(n, N, B, z, gz_part, chain, s1, s2, gz, mask);
+  k_gz<T><<<dim3(nblocks(N, 256), B), 256, 0, s>>>(n, N, B, z, gz_part, chain, s1, s2, gz, mask);
   STL_HIP(hipGetLastError());
 }
 
 template <typename T>
-void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, T* g0,
-                         const MaskView& mask, hipStream_t s) {
-  k_meanvar_grad<T><<<dim3(nblocks(N, 256), B), 256, 0, s>>>(N, x, gmean, gvar, g0, mask);
+void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, const T* g0,
+                         double* out, const MaskView& mask, hipStream_t s) {
+  k_meanvar_grad<T><<<dim3(nblocks(N, 256), B), 256, 0, s>>>(N, x, gmean, gvar, g0, out, mask);
   STL_HIP(hipGetLastError());
 }
 
@@ -3029,8 +3041,8 @@ void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s) {
                              cplx_t<T>*, cplx_t<T>*, const MaskView&, hipStream_t);                           \
   template void launch_gz<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, const T*, const double*,  \
                              const double*, cplx_t<T>*, const MaskView&, hipStream_t);                        \
-  template void launch_meanvar_grad<T>(int64_t, int, const T*, const double*, const double*, T*,              \
-                                       const MaskView&, hipStream_t);                                         \
+  template void launch_meanvar_grad<T>(int64_t, int, const T*, const double*, const double*, const T*,        \
+                                       double*, const MaskView&, hipStream_t);                                \
   template void launch_gram_x<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, bool, int, double*,   \
                                  double*, const MaskView&, hipStream_t);                                      \
   template void launch_zcross<T>(int64_t, int, const cplx_t<T>*, int, double*, const MaskView&, hipStream_t); \

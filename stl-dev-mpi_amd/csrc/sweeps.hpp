@@ -137,10 +137,10 @@ void launch_gv(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>
 template <typename T>
 void launch_gz(int n, int64_t N, int B, const cplx_t<T>* z, const cplx_t<T>* gz_part, const T* chain,
                const double* s1, const double* s2, cplx_t<T>* gz, const MaskView& mask, hipStream_t s);
-// g0 [B][N] += gmean[b,c] + gvar[b,c] x, c the pixel's mask class.
+// out [B][N] (FP64) = T(g0 + gmean[b,c] + gvar[b,c] x), c the pixel's mask class.
 template <typename T>
-void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, T* g0,
-                         const MaskView& mask, hipStream_t s);
+void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, const T* g0,
+                         double* out, const MaskView& mask, hipStream_t s);
 // Cross statistics of map pairs (the reference's cross plan, two channels,
 // c1 != c2): B pairs, field rows 2b (channel a) and 2b + 1 (channel b) of
 // the field layouts above.
