@@ -524,13 +524,14 @@ class EngineT final : public Engine {
   // levels J - 1 .. 1; rev_chain(0) needs both.
   void sweep_reverse(double* grad_dev) {
     rev_prelude();
-    rev_levels(0);
-    rev_finish(grad_dev);
+    rev_levels(0, grad_dev);
   }
 
   // The reverse of levels J - 1 .. down_to: the batched levels' steps (a
   // launch each for all of them), then the others level by level.
-  void rev_levels(int down_to) {
+  // grad_dev: level 0's rev_chain writes the map gradient (down_to 0; the
+  // batched levels start at 2).
+  void rev_levels(int down_to, double* grad_dev = nullptr) {
     int n = levels_ - 1;
     if (batch_on() && qc_ >= down_to) {
       run_batches(qrev_);
@@ -538,7 +539,7 @@ class EngineT final : public Engine {
     }
     for (; n >= down_to; --n) {
       rev_branch(n);
-      rev_chain(n);
+      rev_chain(n, n == 0 ? grad_dev : nullptr);
     }
   }
 
@@ -618,25 +619,57 @@ class EngineT final : public Engine {
     conv_adj(n, kL, gbr_of(n), (n + 1) * B_, chain_of(n));
   }
 
-  void rev_chain(int n) {
+  // Level n's chain (decimated coarser chain added), gz, z adjoint and map
+  // cotangent; at level 0 (grad_dev given) also the map gradient.  With a
+  // tiled level-0 decimation, level 0's chain (one row per map, read only by
+  // gz) and its map cotangent stay in registers: the decimation adjoints
+  // compute gz and the gradient in the same pass.
+  void rev_chain(int n, double* grad_dev = nullptr) {
     const int64_t N = N_[n];
     const MaskView mv = mask_view(n);
     T* chain = chain_of(n);
-    if (n + 1 < levels_) dec_adj(n, kL, chain_of(n + 1), chain, (n + 1) * B_, chain);
-
-    int id = prof_.start();
     const double* s1 = nmask_ > 0 ? s1c_[n].data() : ds1_[n].data();
     const double* s2 = nmask_ > 0 ? s2c_[n].data() : ds2_[n].data();
-    if (cross_)
-      launch_gz_x<T>(n, N, Bs_, z_of(n), gzp_of(n), chain, reinterpret_cast<const double2*>(s1), gz_of(n),
-                     mv, s_);
-    else
-      launch_gz<T>(n, N, B_, z_of(n), gzp_of(n), chain, s1, s2, gz_of(n), mv, s_);
-    // z, its gV part and gz (complex) and the chain row (real), L each per pixel.
-    prof_.stop(id, tag("gz", n), double(B_) * kL * N * (7 * sizeof(T)));
+    const bool fused = n == 0 && n + 1 < levels_ && !cross_ && dec_[0]->store.view().ttiled;
+    if (fused) {
+      int id = prof_.start();
+      launch_dec_adj_gz<T>(dec_[0]->store.view(), chain_of(1), chain, B_, z_of(0), gzp_of(0), s1, s2, gz_of(0), mv,
+                           s_);
+      // The decimation's entries, the coarse chain and the branch part in;
+      // z, its gV part in and gz out (complex, L per pixel).
+      prof_.stop(id, tag("dec_adj_gz", 0),
+                 dec_[0]->tslots * (sizeof(T) + 4) + double(B_) * (N_[1] + N) * kL * sizeof(T) +
+                     double(B_) * kL * N * (6 * sizeof(T)));
+    } else {
+      if (n + 1 < levels_) dec_adj(n, kL, chain_of(n + 1), chain, (n + 1) * B_, chain);
+      int id = prof_.start();
+      if (cross_)
+        launch_gz_x<T>(n, N, Bs_, z_of(n), gzp_of(n), chain, reinterpret_cast<const double2*>(s1), gz_of(n),
+                       mv, s_);
+      else
+        launch_gz<T>(n, N, B_, z_of(n), gzp_of(n), chain, s1, s2, gz_of(n), mv, s_);
+      // z, its gV part and gz (complex) and the chain row (real), L each per pixel.
+      prof_.stop(id, tag("gz", n), double(B_) * kL * N * (7 * sizeof(T)));
+    }
 
     T* mg = mg_of(n);
     conv_adj(n, 1, gz_of(n), B_, mg);
+    if (n == 0 && grad_dev != nullptr) {
+      if (fused) {
+        int id = prof_.start();
+        launch_dec_adj_grad<T>(dec_[0]->store.view(), mg_of(1), mg, B_, f_->maps[0].data(),
+                               nmask_ > 0 ? gmc_.data() : dgm_.data(), nmask_ > 0 ? gvc_.data() : dgv_.data(),
+                               grad_dev, mask_view(0), s_);
+        // The decimation's entries, the coarse and fine cotangents and the map in, the FP64 gradient out.
+        prof_.stop(id, tag("dec_adj_grad", 0),
+                   dec_[0]->tslots * (sizeof(T) + 4) + double(B_) * (N_[1] + 2 * N) * sizeof(T) +
+                       double(B_) * N * sizeof(double));
+        return;
+      }
+      if (n + 1 < levels_) dec_adj(n, 1, mg_of(n + 1), mg, B_, mg);
+      rev_finish(grad_dev);
+      return;
+    }
     if (n + 1 < levels_) dec_adj(n, 1, mg_of(n + 1), mg, B_, mg);
   }
 
@@ -1213,10 +1246,7 @@ class EngineT final : public Engine {
   // (which need the coarser chain) follow on the engine stream.
   std::vector<Step> reverse_steps(double* grad_dev) {
     auto coarser = [this](int down_to) { return [this, down_to] { rev_levels(down_to); }; };
-    auto last = [this, grad_dev] {
-      rev_chain(0);
-      rev_finish(grad_dev);
-    };
+    auto last = [this, grad_dev] { rev_chain(0, grad_dev); };
     auto prelude = [this] { rev_prelude(); };
     if (levels_ == 1 || f_->recompute) return {{[this, grad_dev] { sweep_reverse(grad_dev); }, 0, {}}};
     if (levels_ == 2)

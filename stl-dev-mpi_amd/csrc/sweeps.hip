@@ -1249,20 +1249,18 @@ __global__ __launch_bounds__(TC * TC) void k_dec_fwd_tiled(DecOp<T> op, const T*
 // Its transpose: a TF x TF tile of fine cells gathers from the coarse window
 // of their parents +- kDecT ((TF / 2 + 2 kDecT)^2 pixels) staged in LDS.  The
 // cell's first entries and the fine term it adds to are loaded before the
-// staging.
+// staging.  h = the cell's gather (+ add), rounded to T; p its fine pixel.
 template <typename T, int NI, int TF>
-__global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T* __restrict__ gy, const T* add,
-                                                           int rows, T* gx) {
+__device__ __forceinline__ void dec_adj_tile(const DecOp<T>& op, const T* __restrict__ gy, const T* add, int rows,
+                                             T* gs, T (&h)[NI], int64_t& p, int& row) {
   constexpr int SW = TF / 2 + 2 * kDecT, SA = SW * SW;
-  __shared__ T gs[SA * NI];
-  int row;
   const TileInfo ti = tile_info<TF, false>(op.order_f, rows, row);
   const int64_t Nf = 12 * op.npf_f, Nc = 12 * op.npf_c;
   const T* gr = gy + size_t(row) * Nc * NI;
   const int cx0 = ti.x0 / 2 - kDecT, cy0 = ti.y0 / 2 - kDecT;
   const int li = threadIdx.x;
   const int64_t q = int64_t(ti.fr) * op.npf_f + ti.base + li;
-  const int64_t p = int64_t(ti.face) * op.npf_f + ti.base + li;
+  p = int64_t(ti.face) * op.npf_f + ti.base + li;
   const size_t o = (size_t(row) * Nf + p) * NI;
   const int cnt = op.tcnt[q];
   int16_t eo[kDecPre];
@@ -1299,7 +1297,21 @@ __global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T*
     for (int j = 0; j < NI; ++j) acc[j] += v * gs[j * SA + src];
   }
 #pragma unroll
-  for (int j = 0; j < NI; ++j) gx[o + j] = add ? acc[j] + base[j] : acc[j];
+  for (int j = 0; j < NI; ++j) h[j] = add ? acc[j] + base[j] : acc[j];
+}
+
+template <typename T, int NI, int TF>
+__global__ __launch_bounds__(TF * TF) void k_dec_adj_tiled(DecOp<T> op, const T* __restrict__ gy, const T* add,
+                                                           int rows, T* gx) {
+  constexpr int SW = TF / 2 + 2 * kDecT;
+  __shared__ T gs[SW * SW * NI];
+  T h[NI];
+  int64_t p;
+  int row;
+  dec_adj_tile<T, NI, TF>(op, gy, add, rows, gs, h, p, row);
+  const size_t o = (size_t(row) * 12 * op.npf_f + p) * NI;
+#pragma unroll
+  for (int j = 0; j < NI; ++j) gx[o + j] = h[j];
 }
 
 template <typename T, int NI>
@@ -2336,6 +2348,18 @@ __global__ __launch_bounds__(256) void k_gv_small(int64_t N, int B, const cplx_t
   gv_small_body<T, NL>(N, B, br, z, H, gbr, gzp, blockIdx.x, blockIdx.y, lds);
 }
 
+// gz of one orientation from z, its gV part, the modulus cotangent h and the
+// S1/S2 cotangents a, c of its row and mask class.
+template <typename T>
+__device__ inline cplx_t<T> gz_value(cplx_t<T> zv, cplx_t<T> gp, double h, double a, double c) {
+  const double zr = double(zv.x), zi = double(zv.y);
+  const double mag = hypot(zr, zi);
+  const double den = mag == 0.0 ? 1.0 : mag;
+  const double re = double(gp.x) + (a * zr) / den + c * zr + (h * zr) / den;
+  const double im = double(gp.y) + (a * zi) / den + c * zi + (h * zi) / den;
+  return make_c<T>(T(re), T(im));
+}
+
 // gz of every orientation of one pixel (thread per (b, p)): z, its gV part
 // and gz are [b][l][N] planes; the pixel's chain row ([n][b][N][kL]) is one
 // vector load; the S1/S2 cotangents are per (b, l, mask class).
@@ -2353,16 +2377,58 @@ __device__ __forceinline__ void gz_body(int n, int64_t N, int B, const cplx_t<T>
   for (int l = 0; l < kL; ++l) {
     const int bl = b * kL + l;
     const size_t at = size_t(bl) * N + p;
-    double zr = double(z[at].x), zi = double(z[at].y);
-    double mag = hypot(zr, zi);
-    double den = mag == 0.0 ? 1.0 : mag;
-    double h = double(hv[l]);
     const size_t sc = size_t(bl) * mask.C + cls;
-    double a = s1[sc], c = s2[sc];
-    double re = double(gzp[at].x) + (a * zr) / den + c * zr + (h * zr) / den;
-    double im = double(gzp[at].y) + (a * zi) / den + c * zi + (h * zi) / den;
-    gz[at] = make_c<T>(T(re), T(im));
+    gz[at] = gz_value<T>(z[at], gzp[at], double(hv[l]), s1[sc], s2[sc]);
   }
+}
+
+// Level 0's chain decimation and gz in one pass (tiled decimation
+// operators): a cell's chain row (its branch adjoint part add plus the
+// decimated coarser chain gy, one row per map) is read only by gz, so it
+// stays in registers.
+template <typename T, int TF>
+__global__ __launch_bounds__(TF * TF) void k_dec_adj_gz_tiled(DecOp<T> op, const T* __restrict__ gy,
+                                                              const T* __restrict__ add, int B,
+                                                              const cplx_t<T>* __restrict__ z,
+                                                              const cplx_t<T>* __restrict__ gzp,
+                                                              const double* __restrict__ s1,
+                                                              const double* __restrict__ s2,
+                                                              cplx_t<T>* __restrict__ gz, MaskView mask) {
+  constexpr int SW = TF / 2 + 2 * kDecT;
+  __shared__ T gs[SW * SW * kL];
+  T h[kL];
+  int64_t p;
+  int b;
+  dec_adj_tile<T, kL, TF>(op, gy, add, B, gs, h, p, b);
+  const int64_t N = 12 * op.npf_f;
+  const int cls = mask_class(mask, p);
+#pragma unroll
+  for (int l = 0; l < kL; ++l) {
+    const int bl = b * kL + l;
+    const size_t at = size_t(bl) * N + p;
+    const size_t sc = size_t(bl) * mask.C + cls;
+    gz[at] = gz_value<T>(z[at], gzp[at], double(h[l]), s1[sc], s2[sc]);
+  }
+}
+
+// Level 0's map-cotangent decimation with the mean / variance terms and the
+// FP64 output (the map cotangent completed as by k_meanvar_grad).
+template <typename T, int TF>
+__global__ __launch_bounds__(TF * TF) void k_dec_adj_grad_tiled(DecOp<T> op, const T* __restrict__ gy,
+                                                                const T* __restrict__ add, int B,
+                                                                const T* __restrict__ x,
+                                                                const double* __restrict__ gmean,
+                                                                const double* __restrict__ gvar,
+                                                                double* __restrict__ out, MaskView mask) {
+  constexpr int SW = TF / 2 + 2 * kDecT;
+  __shared__ T gs[SW * SW];
+  T h[1];
+  int64_t p;
+  int b;
+  dec_adj_tile<T, 1, TF>(op, gy, add, B, gs, h, p, b);
+  const size_t at = size_t(b) * 12 * op.npf_f + p;
+  const size_t sc = size_t(b) * mask.C + mask_class(mask, p);
+  out[at] = double(T(double(h[0]) + gmean[sc] + gvar[sc] * double(x[at])));
 }
 
 template <typename T>
@@ -2745,6 +2811,7 @@ template <typename T>
 void launch_dec_fwd(const DecOp<T>& op, int NI, const T* x, int rows, T* y, hipStream_t s) {
   if (op.tiled) {
     STL_CHECK(NI == 1 || NI == kL, "decimation: unsupported inner width");
+    STL_CHECK(op.S >= 1, "decimation: empty operator store");
     if (op.order_c >= 4) {
       dim3 grid(unsigned(op.Rc / 256 * 4 * rows));
       if (NI == 1) k_dec_fwd_tiled<T, 1, 16><<<grid, 256, 0, s>>>(op, x, rows, y);
@@ -2768,6 +2835,7 @@ template <typename T>
 void launch_dec_adj(const DecOp<T>& op, int NI, const T* gy, const T* add, int rows, T* gx, hipStream_t s) {
   if (op.ttiled) {
     STL_CHECK(NI == 1 || NI == kL, "decimation adjoint: unsupported inner width");
+    STL_CHECK(op.Tn >= 1, "decimation adjoint: empty operator store");
     dim3 grid(unsigned(op.Rf / 256 * 4 * rows));
     if (NI == 1) k_dec_adj_tiled<T, 1, 16><<<grid, 256, 0, s>>>(op, gy, add, rows, gx);
     else k_dec_adj_tiled<T, kL, 16><<<grid, 256, 0, s>>>(op, gy, add, rows, gx);
@@ -3019,6 +3087,25 @@ void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, cons
   STL_HIP(hipGetLastError());
 }
 
+template <typename T>
+void launch_dec_adj_gz(const DecOp<T>& op, const T* gy, const T* add, int B, const cplx_t<T>* z,
+                       const cplx_t<T>* gz_part, const double* s1, const double* s2, cplx_t<T>* gz,
+                       const MaskView& mask, hipStream_t s) {
+  STL_CHECK(op.ttiled && op.Tn >= 1, "decimation adjoint with gz: needs the tiled operator");
+  dim3 grid(unsigned(op.Rf / 256 * 4 * B));
+  k_dec_adj_gz_tiled<T, 16><<<grid, 256, 0, s>>>(op, gy, add, B, z, gz_part, s1, s2, gz, mask);
+  STL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_dec_adj_grad(const DecOp<T>& op, const T* gy, const T* add, int B, const T* x, const double* gmean,
+                         const double* gvar, double* out, const MaskView& mask, hipStream_t s) {
+  STL_CHECK(op.ttiled && op.Tn >= 1, "decimation adjoint with the map cotangent: needs the tiled operator");
+  dim3 grid(unsigned(op.Rf / 256 * 4 * B));
+  k_dec_adj_grad_tiled<T, 16><<<grid, 256, 0, s>>>(op, gy, add, B, x, gmean, gvar, out, mask);
+  STL_HIP(hipGetLastError());
+}
+
 template <typename TO, typename TI>
 void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s) {
   k_cast<TO, TI><<<nblocks(n, 256), 256, 0, s>>>(n, in, out);
@@ -3035,6 +3122,11 @@ void launch_cast(int64_t n, const TI* in, TO* out, hipStream_t s) {
   template void launch_gram<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, bool, int, double*,     \
                                double*, double*, const MaskView&, hipStream_t);                               \
   template int gram_blocks<T>(int, int64_t, int64_t);                                                          \
+  template void launch_dec_adj_gz<T>(const DecOp<T>&, const T*, const T*, int, const cplx_t<T>*,               \
+                                     const cplx_t<T>*, const double*, const double*, cplx_t<T>*,              \
+                                     const MaskView&, hipStream_t);                                           \
+  template void launch_dec_adj_grad<T>(const DecOp<T>&, const T*, const T*, int, const T*, const double*,      \
+                                       const double*, double*, const MaskView&, hipStream_t);                 \
   template void launch_zstats<T>(int64_t, int, const cplx_t<T>*, int, double*, const MaskView&, hipStream_t); \
   template void launch_meanvar<T>(int64_t, int, const T*, int, double*, const MaskView&, hipStream_t);        \
   template void launch_gv<T>(int, int64_t, int, const cplx_t<T>*, const cplx_t<T>*, const double2*,           \

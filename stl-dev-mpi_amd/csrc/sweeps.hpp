@@ -137,6 +137,18 @@ void launch_gv(int n, int64_t N, int B, const cplx_t<T>* branch, const cplx_t<T>
 template <typename T>
 void launch_gz(int n, int64_t N, int B, const cplx_t<T>* z, const cplx_t<T>* gz_part, const T* chain,
                const double* s1, const double* s2, cplx_t<T>* gz, const MaskView& mask, hipStream_t s);
+// Level 0's chain decimation adjoint (tiled operator, one chain row per map:
+// chain = add + decimation transpose of gy) followed by launch_gz's gz, the
+// chain kept in registers (not written).
+template <typename T>
+void launch_dec_adj_gz(const DecOp<T>& op, const T* gy, const T* add, int B, const cplx_t<T>* z,
+                       const cplx_t<T>* gz_part, const double* s1, const double* s2, cplx_t<T>* gz,
+                       const MaskView& mask, hipStream_t s);
+// Level 0's map-cotangent decimation adjoint (g0 = add + transpose of gy)
+// followed by launch_meanvar_grad into out, g0 not written.
+template <typename T>
+void launch_dec_adj_grad(const DecOp<T>& op, const T* gy, const T* add, int B, const T* x, const double* gmean,
+                         const double* gvar, double* out, const MaskView& mask, hipStream_t s);
 // out [B][N] (FP64) = T(g0 + gmean[b,c] + gvar[b,c] x), c the pixel's mask class.
 template <typename T>
 void launch_meanvar_grad(int64_t N, int B, const T* x, const double* gmean, const double* gvar, const T* g0,
