@@ -88,11 +88,13 @@ __global__ __launch_bounds__(kThreads) void k_chunk_reduce(int64_t n, int kind, 
 VectorOps::VectorOps(int64_t n, hipStream_t s) : n_(n), s_(s) {
   nchunks_ = (n + kChunk - 1) / kChunk;
   partial_.resize(size_t(nchunks_));
-  host_.resize(size_t(nchunks_));
+  STL_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_), size_t(std::max<int64_t>(nchunks_, 1)) * sizeof(double),
+                        hipHostMallocDefault));
 }
 
 VectorOps::~VectorOps() {
   for (double* p : free_) (void)hipFree(p);
+  if (host_) (void)hipHostFree(host_);
 }
 
 double* VectorOps::acquire() {
@@ -158,11 +160,10 @@ void VectorOps::axpy(Vec& p, double a, const Vec& u) {
 double VectorOps::reduce(int kind, const double* a, const double* b, double t) {
   k_chunk_reduce<<<unsigned(nchunks_), kThreads, 0, s_>>>(n_, kind, a, b, t, partial_.data());
   STL_HIP(hipGetLastError());
-  STL_HIP(hipMemcpyAsync(host_.data(), partial_.data(), size_t(nchunks_) * sizeof(double), hipMemcpyDeviceToHost,
-                         s_));
+  STL_HIP(hipMemcpyAsync(host_, partial_.data(), size_t(nchunks_) * sizeof(double), hipMemcpyDeviceToHost, s_));
   STL_HIP(hipStreamSynchronize(s_));
   double r = 0.0;
-  for (double v : host_) r = kind <= 1 ? r + v : std::max(r, v);
+  for (int64_t c = 0; c < nchunks_; ++c) r = kind <= 1 ? r + host_[c] : std::max(r, host_[c]);
   return r;
 }
 

@@ -15,7 +15,9 @@ namespace stl {
 
 class VectorOps {
  public:
-  static constexpr int64_t kChunk = int64_t(1) << 16;
+  // Elements per reduction block: 1536 blocks for the 12.6 M doubles of an
+  // nside-1024 map, enough to fill the GPU (bandwidth-bound reductions).
+  static constexpr int64_t kChunk = int64_t(1) << 13;
 
   // A pooled device vector of n doubles; returns its buffer to the pool.
   class Vec {
@@ -69,7 +71,7 @@ class VectorOps {
   hipStream_t s_ = nullptr;
   std::vector<double*> free_;
   DevArray<double> partial_;
-  std::vector<double> host_;
+  double* host_ = nullptr;  // pinned: the per-block partials come back by a device copy
 };
 
 // Mean ms of one copy of `bytes` by a hand-written stream kernel (the
