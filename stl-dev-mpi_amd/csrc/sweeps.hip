@@ -1158,9 +1158,9 @@ __global__ __launch_bounds__(256) void k_conv_adj_fix(ConvOp<T> op, const cplx_t
 }
 
 // Entries of a decimation row loaded before its window is staged (their
-// latency then overlaps the staging's).  Slots beyond the row's count read
-// the last stored slot's offset (padding offsets are 0, inside the window)
-// with a zero weight.
+// latency then overlaps the staging's).  Slot s reads stored slot
+// min(s, capacity - 1); past the row's count that is padding (offset 0,
+// inside the window) or the row's last entry, taken with a zero weight.
 constexpr int kDecPre = 8;
 
 template <typename T>
