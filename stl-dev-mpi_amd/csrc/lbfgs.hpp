@@ -52,6 +52,7 @@ inline double cubic_interpolate(double x1, double f1, double g1, double x2, doub
 //   V fma_new(const V& d, double t, const V& x)       -> d * t + x (fused)
 //   void fma_into(V& out, const V& d, double t, const V& x)
 //   void axpy(V& p, double a, const V& u)            -> p = u * a + p (fused)
+//   double axpy_dot(V& p, double a, const V& u, const V& w) -> axpy(p, a, u), then dot(w, p)
 //   double dot(const V&, const V&); double abs_max(const V&); double abs_sum(const V&);
 //   double scaled_abs_max(const V& d, double t)      -> max |d * t|
 // Objective: (const V& x, V& grad) -> loss.
@@ -102,15 +103,33 @@ class Lbfgs {
           ro.push_back(1.0 / ys);
         }
         size_t num_old = old_dirs.size();
+        // The two loops of the recursion; each update is fused with the
+        // next pair's product (axpy_dot: the same values as axpy, then dot).
         V q = be_.neg(flat_grad);
-        for (size_t i = num_old; i-- > 0;) {
-          al[i] = be_.dot(old_stps[i], q) * ro[i];
-          be_.axpy(q, -al[i], old_dirs[i]);
+        if (num_old > 0) {
+          size_t i = num_old - 1;
+          double sq = be_.dot(old_stps[i], q);
+          for (;;) {
+            al[i] = sq * ro[i];
+            if (i == 0) {
+              be_.axpy(q, -al[0], old_dirs[0]);
+              break;
+            }
+            sq = be_.axpy_dot(q, -al[i], old_dirs[i], old_stps[i - 1]);
+            --i;
+          }
         }
         d = be_.scale(q, h_diag);
-        for (size_t i = 0; i < num_old; ++i) {
-          double be_i = be_.dot(old_dirs[i], d) * ro[i];
-          be_.axpy(d, al[i] - be_i, old_stps[i]);
+        if (num_old > 0) {
+          double yd = be_.dot(old_dirs[0], d);
+          for (size_t i = 0;; ++i) {
+            const double be_i = yd * ro[i];
+            if (i + 1 == num_old) {
+              be_.axpy(d, al[i] - be_i, old_stps[i]);
+              break;
+            }
+            yd = be_.axpy_dot(d, al[i] - be_i, old_stps[i], old_dirs[i + 1]);
+          }
         }
       }
       prev_flat_grad = be_.copy(flat_grad);
@@ -287,6 +306,11 @@ struct HostBackend {
   }
   void axpy(Vec& p, double a, const Vec& u) {
     for (size_t i = 0; i < p.size(); ++i) p[i] = std::fma(u[i], a, p[i]);
+  }
+  // p += a u, then dot(w, p).
+  double axpy_dot(Vec& p, double a, const Vec& u, const Vec& w) {
+    axpy(p, a, u);
+    return dot(w, p);
   }
   double dot(const Vec& a, const Vec& b) {
     double total = 0.0;

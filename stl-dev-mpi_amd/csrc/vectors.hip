@@ -83,6 +83,31 @@ __global__ __launch_bounds__(kThreads) void k_chunk_reduce(int64_t n, int kind, 
   if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
+// p = u * a + p (as k_fma), then the per-chunk sums of w * p over the
+// updated p, with k_chunk_reduce's chunks, strides and tree (kind 0): the
+// same values as the two kernels, in one pass.
+__global__ __launch_bounds__(kThreads) void k_axpy_dot(int64_t n, double a, const double* __restrict__ u,
+                                                       const double* __restrict__ w, double* __restrict__ p,
+                                                       double* __restrict__ out) {
+  __shared__ double sh[kThreads];
+  int64_t c0 = int64_t(blockIdx.x) * VectorOps::kChunk;
+  int64_t c1 = min(n, c0 + VectorOps::kChunk);
+  double acc = 0.0;
+  for (int64_t i = c0 + threadIdx.x; i < c1; i += kThreads) {
+    const double pn = fma(u[i], a, p[i]);
+    p[i] = pn;
+    double v = w[i] * pn;
+    acc += v;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = kThreads / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
 }  // namespace
 
 VectorOps::VectorOps(int64_t n, hipStream_t s) : n_(n), s_(s) {
@@ -160,6 +185,12 @@ void VectorOps::axpy(Vec& p, double a, const Vec& u) {
 double VectorOps::reduce(int kind, const double* a, const double* b, double t) {
   k_chunk_reduce<<<unsigned(nchunks_), kThreads, 0, s_>>>(n_, kind, a, b, t, partial_.data());
   STL_HIP(hipGetLastError());
+  return gather(kind);
+}
+
+// The chunk partials on the host, combined in chunk order (sum for kinds 0
+// and 1, max otherwise).
+double VectorOps::gather(int kind) {
   STL_HIP(hipMemcpyAsync(host_, partial_.data(), size_t(nchunks_) * sizeof(double), hipMemcpyDeviceToHost, s_));
   STL_HIP(hipStreamSynchronize(s_));
   double r = 0.0;
@@ -168,6 +199,12 @@ double VectorOps::reduce(int kind, const double* a, const double* b, double t) {
 }
 
 double VectorOps::dot(const Vec& a, const Vec& b) { return reduce(0, a.p, b.p, 0.0); }
+
+double VectorOps::axpy_dot(Vec& p, double a, const Vec& u, const Vec& w) {
+  k_axpy_dot<<<unsigned(nchunks_), kThreads, 0, s_>>>(n_, a, u.p, w.p, p.p, partial_.data());
+  STL_HIP(hipGetLastError());
+  return gather(0);
+}
 double VectorOps::abs_sum(const Vec& a) { return reduce(1, a.p, nullptr, 0.0); }
 double VectorOps::abs_max(const Vec& a) { return reduce(2, a.p, nullptr, 0.0); }
 double VectorOps::scaled_abs_max(const Vec& d, double t) { return reduce(3, d.p, nullptr, t); }

@@ -58,6 +58,8 @@ class VectorOps {
   void fma_into(Vec& out, const Vec& d, double t, const Vec& x);
   void axpy(Vec& p, double a, const Vec& u);
   double dot(const Vec& a, const Vec& b);
+  // p += a u, then dot(w, p) of the updated p: the values of axpy and dot, in one pass.
+  double axpy_dot(Vec& p, double a, const Vec& u, const Vec& w);
   double abs_sum(const Vec& a);
   double abs_max(const Vec& a);
   double scaled_abs_max(const Vec& d, double t);
@@ -66,6 +68,7 @@ class VectorOps {
   double* acquire();
   void release(double* p);
   double reduce(int kind, const double* a, const double* b, double t);
+  double gather(int kind);
 
   int64_t n_ = 0, nchunks_ = 0;
   hipStream_t s_ = nullptr;
