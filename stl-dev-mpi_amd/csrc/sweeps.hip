@@ -1002,7 +1002,7 @@ constexpr int adj_lds_waves() {
   constexpr int waves = wgs * TS * TS / 256;
   return waves < 1 ? 1 : (waves > 8 ? 8 : waves);
 }
-template <typename T, int NI, int TS, int FW, int GP, int kSlotUnroll = TS == 32 ? 4 : 2>
+template <typename T, int NI, int TS, int FW, int GP, int kSlotUnroll = TS == 32 ? 4 : 2, int PF = 0>
 __global__ __launch_bounds__(TS * TS) __attribute__((amdgpu_waves_per_eu(adj_lds_waves<T, NI, TS, FW, GP>(),
                                                                          adj_lds_waves<T, NI, TS, FW, GP>())))
 void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
@@ -1050,53 +1050,70 @@ void k_conv_adj_faces(ConvOp<T> op, const cplx_t<T>* __restrict__ gy,
   for (int f = 0; f < FW; ++f)
 #pragma unroll
     for (int j = 0; j < NI; ++j) acc[f][j] = T(0);
-  for (int pass = 0; pass < kL / GP; ++pass) {
-    __syncthreads();
-    // In batches of CH positions: every load of a batch is issued (missing
-    // pixels read pixel 0 and store 0) before its LDS writes.
-    constexpr int CH = stage_batch<cplx_t<T>, kPer, GP * NI, 512 / adj_lds_waves<T, NI, TS, FW, GP>()>();
+  // Cotangent values of window position i (missing pixels read pixel 0) in
+  // pass `pass`, and their LDS writes (0 for missing pixels).
+  auto stage_load = [&](int i, int pass, cplx_t<T> (&sv)[GP][NI]) {
+    const int64_t p = spix[i] >= 0 ? spix[i] : 0;
+    const int k = fg * FW + (threadIdx.x + i * TS * TS) / Grid::kSlots;
 #pragma unroll
-    for (int i0 = 0; i0 < kPer; i0 += CH) {
-      cplx_t<T> sv[CH][GP][NI];
+    for (int gg = 0; gg < GP; ++gg) {
+      const int g = (pass * GP + gg) ^ (2 * (k & 1));
 #pragma unroll
-      for (int c = 0; c < CH; ++c) {
-        const int i = i0 + c;
-        if (i >= kPer) break;
-        const int64_t p = spix[i] >= 0 ? spix[i] : 0;
-        const int k = fg * FW + (threadIdx.x + i * TS * TS) / Grid::kSlots;
+      for (int j = 0; j < NI; ++j) sv[gg][j] = gr[(size_t(g) * N + p) * NI + j];
+    }
+  };
+  auto stage_write = [&](int i, int pass, const cplx_t<T> (&sv)[GP][NI]) {
+    const bool ok = spix[i] >= 0;
+    const int idx = threadIdx.x + i * TS * TS;
+    const int f = idx / Grid::kSlots;
+    int wx, wy;
+    Grid::at(idx % Grid::kSlots, wx, wy);
+    const int pi = wy * PW + wx;
+    const int k = fg * FW + f;
 #pragma unroll
-        for (int gg = 0; gg < GP; ++gg) {
-          const int g = (pass * GP + gg) ^ (2 * (k & 1));
+    for (int gg = 0; gg < GP; ++gg) {
+      const int g = (pass * GP + gg) ^ (2 * (k & 1));
+      const bool cj = gauge_conj(k, g) != m;
 #pragma unroll
-          for (int j = 0; j < NI; ++j) sv[c][gg][j] = gr[(size_t(g) * N + p) * NI + j];
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < CH; ++c) {
-        const int i = i0 + c;
-        if (i >= kPer) break;
-        if (spix[i] == -2) continue;
-        const bool ok = spix[i] >= 0;
-        const int idx = threadIdx.x + i * TS * TS;
-        const int f = idx / Grid::kSlots;
-        int wx, wy;
-        Grid::at(idx % Grid::kSlots, wx, wy);
-        const int pi = wy * PW + wx;
-        const int k = fg * FW + f;
-#pragma unroll
-        for (int gg = 0; gg < GP; ++gg) {
-          const int g = (pass * GP + gg) ^ (2 * (k & 1));
-          const bool cj = gauge_conj(k, g) != m;
-#pragma unroll
-          for (int j = 0; j < NI; ++j) {
-            cplx_t<T> v = ok ? sv[c][gg][j] : make_c<T>(T(0), T(0));
-            if (cj) v.y = -v.y;
-            gs[((f * GP + gg) * NI + j) * SA + pi] = v;
-          }
-        }
+      for (int j = 0; j < NI; ++j) {
+        cplx_t<T> v = ok ? sv[gg][j] : make_c<T>(T(0), T(0));
+        if (cj) v.y = -v.y;
+        gs[((f * GP + gg) * NI + j) * SA + pi] = v;
       }
     }
+  };
+  // The first PF positions of the next pass are loaded into registers before
+  // the current pass's slot loop, so that their loads travel while it runs;
+  // the other positions are staged between the barriers.
+  constexpr int kPF = PF < kPer ? PF : kPer;
+  cplx_t<T> pv[kPF > 0 ? kPF : 1][GP][NI];
+#pragma unroll
+  for (int i = 0; i < kPF; ++i)
+    if (spix[i] != -2) stage_load(i, 0, pv[i]);
+  for (int pass = 0; pass < kL / GP; ++pass) {
     __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPF; ++i)
+      if (spix[i] != -2) stage_write(i, pass, pv[i]);
+    // In batches of CH positions: every load of a batch is issued before its
+    // LDS writes.
+    constexpr int CH = stage_batch<cplx_t<T>, kPer, GP * NI, 512 / adj_lds_waves<T, NI, TS, FW, GP>()>();
+#pragma unroll
+    for (int i0 = kPF; i0 < kPer; i0 += CH) {
+      cplx_t<T> sv[CH][GP][NI];
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+        if (i0 + c < kPer) stage_load(i0 + c, pass, sv[c]);
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+        if (i0 + c < kPer && spix[i0 + c] != -2) stage_write(i0 + c, pass, sv[c]);
+    }
+    __syncthreads();
+    if (pass + 1 < kL / GP) {
+#pragma unroll
+      for (int i = 0; i < kPF; ++i)
+        if (spix[i] != -2) stage_load(i, pass + 1, pv[i]);
+    }
     // Chunks of kSlotUnroll slots; 2 where the window has fewer than 8 planes
     // (the 4-face z adjoint: two workgroups per CU leave 64 VGPRs).
     pipelined_slots<(FW * GP * NI >= 8 ? kSlotUnroll : 2), GP, cplx_t<T>>(
@@ -2743,19 +2760,25 @@ bool conv_adj_faces(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, 
   return false;
 }
 
-// Multi-face adjoint on 32 x 32 tiles (118 KB of LDS at most, one
+// Multi-face adjoint on 32 x 32 tiles (147 KB of LDS at most, one
 // workgroup per CU): NI = kL: 2 faces, 1 gauge per pass (FP32 only); NI = 1:
-// 4 faces, 1 gauge per pass.
+// 4 faces, 1 gauge per pass (74 KB in FP32, two workgroups per CU).
+// Window positions of the next pass prefetched into registers by the 32 x 32
+// multi-face adjoints: the FP32 branch adjoint (one workgroup per CU) takes 4
+// of its 5, 32 VGPRs.  The FP32 z adjoint runs two workgroups per CU within 64
+// VGPRs (a prefetch spills there); the FP64 z adjoint measured no gain.
+template <typename T, int NI>
+constexpr int adj_prefetch() { return NI == 1 ? 0 : 4; }
 template <typename T>
 void conv_adj_faces32(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
   if (NI == 1) {
     dim3 grid(unsigned(3 * op.npf / (32 * 32) * rows));
-    k_conv_adj_faces<T, 1, 32, 4, 1><<<grid, 32 * 32, 0, s>>>(op, gy, rows, gx);
+    k_conv_adj_faces<T, 1, 32, 4, 1, 4, adj_prefetch<T, 1>()><<<grid, 32 * 32, 0, s>>>(op, gy, rows, gx);
     return;
   }
   if constexpr (sizeof(T) == 4) {
     dim3 grid(unsigned(3 * op.npf / (32 * 32) * 2 * rows));
-    k_conv_adj_faces<T, kL, 32, 2, 1><<<grid, 32 * 32, 0, s>>>(op, gy, rows, gx);
+    k_conv_adj_faces<T, kL, 32, 2, 1, 4, adj_prefetch<T, kL>()><<<grid, 32 * 32, 0, s>>>(op, gy, rows, gx);
   }
 }
 
