@@ -2643,15 +2643,15 @@ template <typename T, int TS>
 void conv_adj_tiled(const ConvOp<T>& op, int NI, const cplx_t<T>* gy, int rows, T* gx, hipStream_t s) {
   dim3 grid(unsigned(3 * op.npf / (TS * TS) * 4 * rows));
   if constexpr (TS == 32) {
-    // 42 rows of pitch 44: one gauge per pass for NI = kL (59 KB FP32,
-    // 118 KB FP64), 4 (FP32) or 2 (FP64) gauges for NI = 1.
+    // 42 rows of pitch 56 (FP32) or 44 (FP64): one gauge per pass for NI = kL
+    // (74 KB FP32, 116 KB FP64), 4 (FP32) or 2 (FP64) gauges for NI = 1.
     constexpr int GP1 = sizeof(T) == 8 ? 2 : 4;
     if (NI == 1) {
       k_conv_adj_tiled<T, 1, TS, GP1><<<grid, TS * TS, 0, s>>>(op, gy, rows, gx);
       return;
     }
     if constexpr (sizeof(T) == 4) {
-      // Two rows per workgroup (118 KB of LDS, one workgroup per CU). An odd
+      // Two rows per workgroup (147 KB of LDS, one workgroup per CU). An odd
       // last row goes to the one-row kernel where the faces hold at least
       // 256 x 256 pixels; below that the second launch costs more than the
       // shared loads save.
