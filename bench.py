@@ -30,7 +30,7 @@ HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
 FAMILIES = ["mean", "var", "S1", "S2", "S3", "S4"]
 ERROR_BAR = 1e-6          # the metric's relative-error bar, per statistic family and on the gradient
 SPEED_RATIO_TARGET = 3.0  # mixed / FP64 evaluations per second asked for
-HOST_THREADS = 16         # host threads one GPU's job gets on the GPU boxes
+HOST_THREADS = 16         # host threads per GPU on a one-GPU MI355X host (its CPU share)
 HEADLINE = ROOT / "tests" / "golden" / "headline"
 
 
