@@ -370,7 +370,7 @@ def main():
                     help="L-BFGS iterations of the synthesis measurement at the bench nside (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the whole sky fits one "
+        print(f"bench.py: runs on one GPU only (asked for {args.gpus}); the whole sky fits one "
               f"MI355X, and the multi-GPU sky split is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
     import torch

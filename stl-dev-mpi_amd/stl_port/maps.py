@@ -155,7 +155,7 @@ def initial_maps(target, members=1, seed=11, init=None, init_units="target", ini
         out = np.stack([pixel_normal(int(seed) + m, 0, npix, library) for m in range(members)])
         return out, None
     if isinstance(init_noise, str) and init_noise == "conditional":
-        raise NotImplementedError("--init-noise conditional is not in this build")
+        raise NotImplementedError("--init-noise conditional is not supported by this library")
     path = os.fspath(init)
     per_member = "{m}" in path
 

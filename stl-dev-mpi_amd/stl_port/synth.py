@@ -99,7 +99,7 @@ def parse_args(argv):
         elif k == "--init-noise":
             v = value()
             if v == "conditional":
-                raise ValueError("--init-noise conditional is not in this build")
+                raise ValueError("--init-noise conditional is not supported by this library")
             a["init_noise"] = v if v == "auto" else float(v)
         elif k == "--seed":
             a["seed"] = int(value())
@@ -134,12 +134,12 @@ def parse_args(argv):
         elif k in ("--L", "--kernel-size", "--groups"):
             v = int(value())
             if v != {"--L": 4, "--kernel-size": 5, "--groups": 1}[k]:
-                raise ValueError(f"{k} {v} is not in this build")
+                raise ValueError(f"{k} {v} is not supported by this library")
         elif k in _ZERO_TERMS:
             if float(value()) != 0.0:
-                raise ValueError(f"{k}: this loss term is not in this build")
+                raise ValueError(f"{k}: this loss term is not supported by this library")
         else:
-            raise ValueError(f"{k}: not in this build")
+            raise ValueError(f"{k}: not supported by this library")
     if a["target"] is None:
         raise ValueError("--target is required")
     return a

@@ -1,7 +1,7 @@
 """The recompute field store (the reference's FieldStore::Recompute, crates/stl-gpu/src/
 engine.rs:172-186, docs/gpu.md:241-256): the forward keeps the maps and moduli only, and the
 reverse recomputes each level's z and branches.  The reference holds Keep and Recompute to the
-same bits (crates/stl-gpu/tests/stats.rs:185-217); so does this build, since the recomputation
+same bits (crates/stl-gpu/tests/stats.rs:185-217); so does this library, since the recomputation
 runs the forward's own kernels on the same inputs.
 
 - keep vs recompute: values, variances and gradients equal bit for bit with every level launched

@@ -322,8 +322,8 @@ def test_synth_command_line():
                       "--init-noise", "0.3", "--weights", "ergodic", "--gpu", "--L", "4"])
     assert a["ensemble"] == 4 and a["init_noise"] == 0.3 and a["weights"] == "ergodic"
     assert a["family_weights"] == [("S1", 10.0), ("S2", 2.5)]
-    for bad, msg in ((["--target", "t.npy", "--scatter-weight", "1"], "not in this build"),
-                     (["--target", "t.npy", "--L", "2"], "not in this build"),
+    for bad, msg in ((["--target", "t.npy", "--scatter-weight", "1"], "not supported by this library"),
+                     (["--target", "t.npy", "--L", "2"], "not supported by this library"),
                      (["--target", "t.npy", "--family-weights", "S5=1"], "unknown family"),
                      (["--target", "t.npy", "--init-noise", "conditional"], "conditional"),
                      (["--niter", "3"], "--target is required")):
